@@ -1,0 +1,235 @@
+/*
+ * qop_mi355x.h -- C ABI of the MI355X-native asqtad hot path.
+ *
+ * Drop-in for the asqtad subset of the QOPQDP C API (Nc = 3, double "D3"
+ * and single "F3" precision).  The type definitions below are laid out
+ * exactly as in the reference's include/qop_int.h so code compiled against
+ * QOPQDP's headers passes the same structs; the functions carry the same
+ * names, arguments and conventions as the reference prototypes in
+ * include/qop_pc.h (instantiated by include/qop_d3.h / qop_f3.h).
+ *
+ * Conventions kept from the reference:
+ *  - fields in the "raw" layout: vector QOP_Real[V][3][2], link matrix
+ *    QOP_Real[V][3][3][2] (row major), site index = lexicographic
+ *    ((x0*L1+x1)*L2+x2)*L3+x3 halved, even sites first
+ *    (lib/layout_user.c:237-258);
+ *  - fermion links are stored internally multiplied by 1/2
+ *    (lib/asqtad_dslash_p.c:201-206);
+ *  - an EVEN (ODD) subset solve returns out = m A^-1 in with
+ *    A = m^2 - D_eo D_oe, and res_arg->final_rsq is |r|^2/|in|^2
+ *    (README:24-48, lib/asqtad_invert_p.c:1-40, 603-610).
+ *
+ * The QDP-typed entry points (*_qdp) of the reference are not provided:
+ * there is no QDP/C here.  All compute runs on the GPU (hand-written HIP
+ * kernels for gfx950); there is no CPU path.
+ *
+ * Entry points named QOPX_* are extensions of this library with no
+ * counterpart in the reference: seeded inputs, batched mixed-precision
+ * solves, a continuous-batching request loop and timing helpers.
+ */
+#ifndef QOP_MI355X_H
+#define QOP_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- types: same layout as include/qop_int.h:1-186 ------------------- */
+#define QOP_VERB_OFF 0
+#define QOP_VERB_LOW 1
+#define QOP_VERB_MED 2
+#define QOP_VERB_HI 3
+#define QOP_VERB_DEBUG 4
+
+typedef enum {
+  QOP_SUCCESS = 0,
+  QOP_FAIL = 1,
+  QOP_BC_ERROR,
+  QOP_MEM_ERROR,
+  QOP_TOP_ERROR
+} QOP_status_t; /* include/qop_int.h:15-21 */
+
+typedef enum { QOP_EVEN = 0, QOP_ODD = 1, QOP_EVENODD = 2 } QOP_evenodd_t; /* :23-27 */
+
+typedef float QOP_F_Real;
+typedef double QOP_D_Real;
+
+typedef struct {
+  int (*node_number)(const int coords[]);
+  int (*node_index)(const int coords[]);
+  int latdim;
+  int *latsize;
+  int machdim;
+  int *machsize;
+  int this_node;
+  int sites_on_node;
+} QOP_layout_t; /* :51-60 */
+#define QOP_LAYOUT_ZERO ((QOP_layout_t){NULL, NULL, 0, NULL, 0, NULL, 0, 0})
+
+typedef struct {
+  double re;
+  double im;
+} QOP_Complex;
+
+typedef struct {
+  QOP_Complex *phase;
+} QOP_bc_t; /* :70-72 */
+
+typedef struct {
+  int *signmask;
+} QOP_staggered_sign_t; /* :78-80 */
+
+typedef struct {
+  char *tag;
+  double value;
+  void *extra;
+} QOP_opt_t; /* :79-83 */
+
+typedef struct {
+  double final_sec;
+  double final_flop;
+  QOP_status_t status;
+  int count1, count2;
+} QOP_info_t; /* :88-94 */
+#define QOP_INFO_ZERO ((QOP_info_t){0, 0, QOP_SUCCESS, 0, 0})
+
+typedef struct {
+  double mixed_rsq;
+  int max_iter;
+  int restart;
+  int max_restarts;
+  QOP_evenodd_t evenodd;
+} QOP_invert_arg_t; /* :118-125 */
+#define QOP_INVERT_ARG_DEFAULT ((QOP_invert_arg_t){1e9, 2000, 1000, 5, QOP_EVENODD})
+
+typedef struct {
+  double rsqmin;
+  double final_rsq;
+  double relmin;
+  double final_rel;
+  int final_iter;
+  int final_restart;
+} QOP_resid_arg_t; /* :128-136 */
+#define QOP_RESID_ARG_DEFAULT ((QOP_resid_arg_t){1e-6, 0, 0, 0, 0, 0})
+
+typedef struct {
+  double one_link;
+  double three_staple;
+  double five_staple;
+  double seven_staple;
+  double lepage;
+  double naik;
+} QOP_asqtad_coeffs_t; /* :150-157 */
+#define QOP_ASQTAD_COEFFS_ZERO ((QOP_asqtad_coeffs_t){0, 0, 0, 0, 0, 0})
+
+/* opaque handles (include/qop_pc.h:10-18) */
+typedef struct QOP_D3_ColorVector_struct QOP_D3_ColorVector;
+typedef struct QOP_F3_ColorVector_struct QOP_F3_ColorVector;
+typedef struct QOP_D3_GaugeField_struct QOP_D3_GaugeField;
+typedef struct QOP_F3_GaugeField_struct QOP_F3_GaugeField;
+typedef struct QOP_D3_FermionLinksAsqtad_struct QOP_D3_FermionLinksAsqtad;
+typedef struct QOP_F3_FermionLinksAsqtad_struct QOP_F3_FermionLinksAsqtad;
+
+/* ---- general routines (include/qop_int.h:230-251; lib/common.c) ------ */
+QOP_status_t QOP_init(QOP_layout_t *layout);         /* lib/common.c:39-83 */
+QOP_status_t QOP_finalize(void);                     /* lib/common.c:85-94 */
+int QOP_is_initialized(void);                        /* lib/common.c:96-100 */
+const char *QOP_version_str(void);                   /* lib/common.c:104-108 */
+int QOP_version_int(void);                           /* lib/common.c:110-116 */
+int QOP_verbose(int level);                          /* lib/common.c:118-124 */
+int QOP_profcontrol(int level);                      /* lib/common.c:126-132 */
+int QOP_node_number_raw(int coords[]);               /* lib/common.c:134-139 */
+int QOP_node_index_raw_V(int coords[], QOP_evenodd_t evenodd); /* lib/common.c:141-146 */
+int QOP_node_index_raw_G(int coords[], QOP_evenodd_t evenodd); /* lib/common.c:155-160 */
+int QOP_sites_on_node_raw_V(QOP_evenodd_t evenodd);  /* lib/common.c:169-173 */
+int QOP_sites_on_node_raw_G(QOP_evenodd_t evenodd);  /* lib/common.c:181-185 */
+/* option tags st, ns, nm, cg, eigcg_* as lib/asqtad_invert.c:68-112; tags
+ * starting "qmx_" select this library's solvers (see QOPX options). */
+QOP_status_t QOP_asqtad_invert_set_opts(QOP_opt_t opts[], int nopts);
+
+/* ---- fields (include/qop_pc.h:49-121; lib/common_p.c) ---------------- */
+QOP_D3_ColorVector *QOP_D3_create_V_from_raw(QOP_D_Real *src, QOP_evenodd_t evenodd); /* common_p.c:35-48 */
+QOP_F3_ColorVector *QOP_F3_create_V_from_raw(QOP_F_Real *src, QOP_evenodd_t evenodd);
+void QOP_D3_extract_V_to_raw(QOP_D_Real *dest, QOP_D3_ColorVector *src, QOP_evenodd_t evenodd); /* common_p.c:99-104 */
+void QOP_F3_extract_V_to_raw(QOP_F_Real *dest, QOP_F3_ColorVector *src, QOP_evenodd_t evenodd);
+void QOP_D3_destroy_V(QOP_D3_ColorVector *field); /* qop_pc.h:64 */
+void QOP_F3_destroy_V(QOP_F3_ColorVector *field);
+QOP_D3_ColorVector *QOP_D3_convert_V_from_raw(QOP_D_Real *src, QOP_evenodd_t evenodd); /* qop_pc.h:72 */
+QOP_F3_ColorVector *QOP_F3_convert_V_from_raw(QOP_F_Real *src, QOP_evenodd_t evenodd);
+QOP_D_Real *QOP_D3_convert_V_to_raw(QOP_D3_ColorVector *src, QOP_evenodd_t evenodd); /* qop_pc.h:80 */
+QOP_F_Real *QOP_F3_convert_V_to_raw(QOP_F3_ColorVector *src, QOP_evenodd_t evenodd);
+
+QOP_D3_GaugeField *QOP_D3_create_G_from_raw(QOP_D_Real *links[], QOP_evenodd_t evenodd); /* common_p.c:65-78 */
+QOP_F3_GaugeField *QOP_F3_create_G_from_raw(QOP_F_Real *links[], QOP_evenodd_t evenodd);
+void QOP_D3_extract_G_to_raw(QOP_D_Real *dest[], QOP_D3_GaugeField *src, QOP_evenodd_t evenodd); /* qop_pc.h:58 */
+void QOP_F3_extract_G_to_raw(QOP_F_Real *dest[], QOP_F3_GaugeField *src, QOP_evenodd_t evenodd);
+void QOP_D3_destroy_G(QOP_D3_GaugeField *field); /* qop_pc.h:66 */
+void QOP_F3_destroy_G(QOP_F3_GaugeField *field);
+/* staggered signs and boundary phases in place (common_p.c:455-526) */
+void QOP_D3_rephase_G(QOP_D3_GaugeField *links, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign);
+void QOP_F3_rephase_G(QOP_F3_GaugeField *links, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign);
+
+/* ---- asqtad links (include/qop_pc.h:217-271; lib/asqtad_dslash_p.c) -- */
+QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_raw(QOP_D_Real *fatlinks[], QOP_D_Real *longlinks[], QOP_evenodd_t evenodd); /* :305-329 */
+QOP_F3_FermionLinksAsqtad *QOP_F3_asqtad_create_L_from_raw(QOP_F_Real *fatlinks[], QOP_F_Real *longlinks[], QOP_evenodd_t evenodd);
+QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_G(QOP_info_t *info, QOP_asqtad_coeffs_t *coeffs, QOP_D3_GaugeField *gauge); /* :642-664 */
+QOP_F3_FermionLinksAsqtad *QOP_F3_asqtad_create_L_from_G(QOP_info_t *info, QOP_asqtad_coeffs_t *coeffs, QOP_F3_GaugeField *gauge);
+QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_G2(QOP_info_t *info, QOP_asqtad_coeffs_t *coeffs, QOP_D3_GaugeField *gFat, QOP_D3_GaugeField *gLong); /* :666-689 */
+void QOP_D3_asqtad_extract_L_to_raw(QOP_D_Real *fatlinks[], QOP_D_Real *longlinks[], QOP_D3_FermionLinksAsqtad *src, QOP_evenodd_t evenodd); /* :915-938 */
+void QOP_F3_asqtad_extract_L_to_raw(QOP_F_Real *fatlinks[], QOP_F_Real *longlinks[], QOP_F3_FermionLinksAsqtad *src, QOP_evenodd_t evenodd);
+void QOP_D3_asqtad_load_L_from_raw(QOP_D3_FermionLinksAsqtad *asqtad, QOP_D_Real *fatlinks[], QOP_D_Real *longlinks[], QOP_evenodd_t evenodd); /* :331-360 */
+void QOP_D3_asqtad_load_L_from_G(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *asqtad, QOP_asqtad_coeffs_t *coeffs, QOP_D3_GaugeField *gauge); /* :860-877 */
+void QOP_D3_asqtad_destroy_L(QOP_D3_FermionLinksAsqtad *field); /* :940-968 */
+void QOP_F3_asqtad_destroy_L(QOP_F3_FermionLinksAsqtad *field);
+void QOP_D3_asqtad_rephase_L(QOP_D3_FermionLinksAsqtad *fla, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign); /* :1447-1482 */
+/* precision copies (include/qop_poc.h:19-20; asqtad_dslash_p.c:742-838) */
+QOP_F3_FermionLinksAsqtad *QOP_FD3_asqtad_create_L_from_L(QOP_D3_FermionLinksAsqtad *fla_src);
+QOP_D3_FermionLinksAsqtad *QOP_DF3_asqtad_create_L_from_L(QOP_F3_FermionLinksAsqtad *fla_src);
+
+/* ---- the hot path (include/qop_pc.h:300-371) --------------------------- */
+/* out_{eo_out} = D(mass) in_{eo_in} (asqtad_dslash_p.c:1025-1092) */
+void QOP_D3_asqtad_dslash(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *asqtad, QOP_D_Real mass,
+                          QOP_D3_ColorVector *out, QOP_D3_ColorVector *in, QOP_evenodd_t eo_out,
+                          QOP_evenodd_t eo_in);
+void QOP_F3_asqtad_dslash(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *asqtad, QOP_F_Real mass,
+                          QOP_F3_ColorVector *out, QOP_F3_ColorVector *in, QOP_evenodd_t eo_out,
+                          QOP_evenodd_t eo_in);
+/* out = in/mass on eo (asqtad_dslash_p.c:1037-1104) */
+void QOP_D3_asqtad_diaginv(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *asqtad, QOP_D_Real mass,
+                           QOP_D3_ColorVector *out, QOP_D3_ColorVector *in, QOP_evenodd_t eo);
+void QOP_F3_asqtad_diaginv(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *asqtad, QOP_F_Real mass,
+                           QOP_F3_ColorVector *out, QOP_F3_ColorVector *in, QOP_evenodd_t eo);
+/* asqtad_invert_p.c:134-144, 649-851 */
+void QOP_D3_asqtad_invert(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *asqtad,
+                          QOP_invert_arg_t *inv_arg, QOP_resid_arg_t *res_arg, QOP_D_Real mass,
+                          QOP_D3_ColorVector *out_pt, QOP_D3_ColorVector *in_pt);
+void QOP_F3_asqtad_invert(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *asqtad,
+                          QOP_invert_arg_t *inv_arg, QOP_resid_arg_t *res_arg, QOP_F_Real mass,
+                          QOP_F3_ColorVector *out_pt, QOP_F3_ColorVector *in_pt);
+/* asqtad_invert_p.c:853-1059 */
+void QOP_D3_asqtad_invert_multi(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *asqtad,
+                                QOP_invert_arg_t *inv_arg, QOP_resid_arg_t **res_arg[],
+                                QOP_D_Real *masses[], int nmass[], QOP_D3_ColorVector **out_pt[],
+                                QOP_D3_ColorVector *in_pt[], int nsrc);
+void QOP_F3_asqtad_invert_multi(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *asqtad,
+                                QOP_invert_arg_t *inv_arg, QOP_resid_arg_t **res_arg[],
+                                QOP_F_Real *masses[], int nmass[], QOP_F3_ColorVector **out_pt[],
+                                QOP_F3_ColorVector *in_pt[], int nsrc);
+
+/* ==== extensions (no counterpart in the reference) ===================== */
+
+/* Seeded inputs in the raw layout (host).  Gauge: U = reunit(exp(i eps H)),
+ * H traceless Hermitian Gaussian; vector: unit-variance complex Gaussian on
+ * the sites of `evenodd`, zero elsewhere. */
+void QOPX_gen_gauge(const int L[4], unsigned long long seed, double eps, double *U[4]);
+void QOPX_gen_gaussian_V(const int L[4], unsigned long long seed, int evenodd, double *v);
+
+/* Library status: 1 when a HIP device is usable. */
+int QOPX_device_ok(void);
+/* Last error message (empty when none). */
+const char *QOPX_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QOP_MI355X_H */

@@ -1,0 +1,96 @@
+/*
+ * CPU oracle for the asqtad staggered hot path (TEST INFRASTRUCTURE ONLY).
+ *
+ * This is a plain-C FP64 restatement of the reference algorithms, used only
+ * as the checker by tests/, by the smoke check and by the cpu_baseline leg of
+ * bench.py.  Nothing in the product library links or calls it.
+ *
+ * All fields are full-lattice arrays in the QOP "raw" layout:
+ *   vector : double[V][3][2]          (colour, re/im)
+ *   matrix : double[V][3][3][2]       (row, column, re/im)
+ *   site index: l = ((x0*L1 + x1)*L2 + x2)*L3 + x3, even sites first:
+ *   idx = l/2 (even), Vh + l/2 (odd)   (lib/layout_user.c:237-258)
+ *
+ * Every function cites the reference file:line it follows.  Parity pins
+ * for each function live in tests/test_oracle.py (closed forms, invariants,
+ * an independent second stencil restatement and brute force); see the header
+ * of qop_oracle.c for the list.
+ */
+#ifndef QOP_ORACLE_H
+#define QOP_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_EVEN = 0, OR_ODD = 1, OR_EVENODD = 2 };
+
+typedef struct {
+  double one_link, three_staple, five_staple, seven_staple, lepage, naik;
+} or_asqtad_coeffs_t;
+
+/* Lattice geometry (L = {L0,L1,L2,L3}, all even). */
+long or_volume(const int L[4]);
+long or_raw_index(const int L[4], const int x[4]);
+void or_raw_coords(const int L[4], long idx, int x[4]);
+
+/* Rephase gauge links in place (lib/common_p.c:464-526). bc = 4 complex
+ * (re,im) pairs, signmask = 4 ints, r0 = 4 ints. */
+void or_rephase_G(const int L[4], double *U[4], const int r0[4],
+                  const double bc[8], const int signmask[4]);
+
+/* Asqtad fattening (lib/asqtad_dslash_p.c:458-633).  lng may be NULL. */
+void or_make_imp_links(const int L[4], const or_asqtad_coeffs_t *c,
+                       double *const gf[4], double *const gfLong[4],
+                       double *fat[4], double *lng[4]);
+
+/* Plaquette, normalised as examples/test_common.c:171-228. */
+double or_plaquette(const int L[4], double *const U[4]);
+
+/* Link object: user-normalised fat/long (lng may be NULL) -> internal
+ * links scaled by 0.5 and double stored (asqtad_dslash_p.c:79-133,152-223). */
+typedef struct or_links_s or_links_t;
+or_links_t *or_links_create(const int L[4], double *const fat[4],
+                            double *const lng[4]);
+void or_links_destroy(or_links_t *l);
+
+/* QOP_asqtad_dslash_qdp semantics (asqtad_dslash_p.c:1048-1092). */
+void or_dslash(or_links_t *l, double mass, double *out, const double *in,
+               int eo_out, int eo_in);
+/* Independent restatement of the full operator from the user links,
+ * following examples/asqtad_test.c:37-81 (check_op). Writes all sites. */
+void or_dslash_check_op(const int L[4], double *const fat[4],
+                        double *const lng[4], double mass, double *out,
+                        const double *in);
+/* QOP_asqtad_ddagd_norm2 (asqtad_dslash_p.c:1121-1138). */
+double or_ddagd_norm2(or_links_t *l, double mass, double *out,
+                      const double *in, int eo);
+
+typedef struct {
+  double mixed_rsq;
+  int max_iter, restart, max_restarts, evenodd;
+} or_invert_arg_t;
+typedef struct {
+  double rsqmin, final_rsq, relmin, final_rel;
+  int final_iter, final_restart;
+} or_resid_arg_t;
+
+/* QOP_asqtad_invert (asqtad_invert_p.c:134-144, 649-851). */
+void or_invert(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t *ra,
+               double mass, double *out, const double *in,
+               double *final_flop);
+/* QOP_asqtad_invert_multi (asqtad_invert_p.c:853-1059).  masses, out, ra
+ * are flattened [src][mass] with nmass[src] entries per source. */
+void or_invert_multi(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t *ra,
+                     const double *masses, const int *nmass, double **out,
+                     double *const *in, int nsrc, double *final_flop);
+/* Fermilab relative residual (invert_utilities_p.c:7-40). */
+double or_relnorm2(or_links_t *l, const double *r, const double *x, int eo);
+
+/* Number of iterations done by the last inner solver calls (diagnostics). */
+int or_set_verbose(int v);
+int or_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,706 @@
+// The QOP C ABI (asqtad subset, Nc = 3) on top of the device runtime.
+// Each entry point follows the reference function cited beside it.
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "../../include/qop_mi355x.h"
+#include "solver.h"
+
+using namespace qop;
+
+struct QOP_D3_ColorVector_struct {
+  DevVec v;
+  double *raw = nullptr;  // set by convert_V_from_raw
+};
+struct QOP_F3_ColorVector_struct {
+  DevVec v;
+  float *raw = nullptr;
+};
+// Gauge fields are kept on the device in the raw layout (FP64), one array
+// per direction.
+struct QOP_D3_GaugeField_struct {
+  Lattice *lat = nullptr;
+  double *U[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+struct QOP_F3_GaugeField_struct {
+  Lattice *lat = nullptr;
+  double *U[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+struct QOP_D3_FermionLinksAsqtad_struct {
+  Links L;
+};
+struct QOP_F3_FermionLinksAsqtad_struct {
+  Links L;
+};
+
+namespace {
+
+struct Global {
+  int inited = 0;
+  int verbosity = 0;
+  int proflevel = 0;
+  Lattice *lat = nullptr;
+} G;
+
+std::string g_last_error;
+
+void check_init(const char *fn) {
+  if (!G.inited) {
+    fprintf(stderr, "Error: QOP not initialized in function %s\n", fn);
+    exit(1);  // CHECK_INIT, include/qop_internal.h:67-71
+  }
+}
+#define CHECK_INIT check_init(__func__)
+
+void eo_range(const Lattice *lat, int eo, long &b, long &e) {
+  b = eo == QOP_ODD ? lat->Vh : 0;
+  e = eo == QOP_EVEN ? lat->Vh : lat->V;
+}
+
+// Upload the eo sites of a host raw array (T = double/float) into a device
+// vector.  Sites outside eo keep their previous contents.
+template <typename T>
+void upload_vec(DevVec &v, const T *raw, int eo) {
+  Lattice *lat = v.lat;
+  long b, e;
+  eo_range(lat, eo, b, e);
+  T *d;
+  QOP_CHECK_HIP(hipMalloc(&d, (size_t)lat->V * 6 * sizeof(T)));
+  QOP_CHECK_HIP(hipMemcpyAsync(d + 6 * b, raw + 6 * b, (e - b) * 6 * sizeof(T), hipMemcpyHostToDevice, stream()));
+  if constexpr (sizeof(T) == 8) launch_raw_to_vec(lat, (const double *)d, v, eo, stream());
+  else launch_rawf_to_vec(lat, (const float *)d, v, eo, stream());
+  sync();
+  QOP_CHECK_HIP(hipFree(d));
+}
+
+template <typename T>
+void download_vec(T *raw, const DevVec &v, int eo) {
+  Lattice *lat = v.lat;
+  long b, e;
+  eo_range(lat, eo, b, e);
+  T *d;
+  QOP_CHECK_HIP(hipMalloc(&d, (size_t)lat->V * 6 * sizeof(T)));
+  if constexpr (sizeof(T) == 8) launch_vec_to_raw(lat, v, (double *)d, eo, stream());
+  else launch_vec_to_rawf(lat, v, (float *)d, eo, stream());
+  QOP_CHECK_HIP(hipMemcpyAsync(raw + 6 * b, d + 6 * b, (e - b) * 6 * sizeof(T), hipMemcpyDeviceToHost, stream()));
+  sync();
+  QOP_CHECK_HIP(hipFree(d));
+}
+
+// Host raw matrices (T) -> device FP64 raw matrices.
+template <typename T>
+double *upload_mat(Lattice *lat, const T *raw, int eo) {
+  long b, e;
+  eo_range(lat, eo, b, e);
+  size_t n = (size_t)lat->V * 18;
+  double *d;
+  QOP_CHECK_HIP(hipMalloc(&d, n * sizeof(double)));
+  QOP_CHECK_HIP(hipMemsetAsync(d, 0, n * sizeof(double), stream()));
+  if constexpr (sizeof(T) == 8) {
+    QOP_CHECK_HIP(hipMemcpyAsync(d + 18 * b, raw + 18 * b, (e - b) * 18 * sizeof(double), hipMemcpyHostToDevice, stream()));
+    sync();
+  } else {
+    std::vector<double> tmp((e - b) * 18);
+    for (long i = 0; i < (e - b) * 18; i++) tmp[i] = raw[18 * b + i];
+    QOP_CHECK_HIP(hipMemcpyAsync(d + 18 * b, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, stream()));
+    sync();
+  }
+  return d;
+}
+
+template <typename T>
+void download_mat(T *raw, const double *d, Lattice *lat, int eo) {
+  long b, e;
+  eo_range(lat, eo, b, e);
+  std::vector<double> tmp((e - b) * 18);
+  QOP_CHECK_HIP(hipMemcpy(tmp.data(), d + 18 * b, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < tmp.size(); i++) raw[18 * b + i] = (T)tmp[i];
+}
+
+template <typename T>
+void build_links_from_raw(Links &L, T *fat[], T *lng[], int eo) {
+  const double *f[4], *l[4];
+  double *fd[4], *ld[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (int i = 0; i < 4; i++) {
+    fd[i] = upload_mat(L.lat, fat[i], eo);
+    f[i] = fd[i];
+    if (lng) { ld[i] = upload_mat(L.lat, lng[i], eo); l[i] = ld[i]; }
+  }
+  build_links64(L, f, lng ? l : nullptr, stream());
+  for (int i = 0; i < 4; i++) {
+    QOP_CHECK_HIP(hipFree(fd[i]));
+    if (ld[i]) QOP_CHECK_HIP(hipFree(ld[i]));
+  }
+}
+
+void fatten(Links &L, Lattice *lat, const double *const gf[4], const double *const gl[4],
+            const QOP_asqtad_coeffs_t *c, QOP_info_t *info) {
+  double coef[6] = {c->one_link, c->three_staple, c->five_staple, c->seven_staple, c->lepage, c->naik};
+  bool naik = c->naik != 0;
+  double *fat[4], *lng[4];
+  size_t n = (size_t)lat->V * 18 * sizeof(double);
+  for (int i = 0; i < 4; i++) {
+    QOP_CHECK_HIP(hipMalloc(&fat[i], n));
+    lng[i] = nullptr;
+    if (naik) QOP_CHECK_HIP(hipMalloc(&lng[i], n));
+  }
+  const double *g[4] = {gf[0], gf[1], gf[2], gf[3]}, *gL[4] = {gl[0], gl[1], gl[2], gl[3]};
+  double t0 = wall_time();
+  launch_fatten(lat, g, gL, coef, fat, naik ? lng : nullptr, stream());
+  double dt = wall_time() - t0;
+  L.lat = lat;
+  const double *fc[4] = {fat[0], fat[1], fat[2], fat[3]}, *lc[4] = {lng[0], lng[1], lng[2], lng[3]};
+  build_links64(L, fc, naik ? lc : nullptr, stream());
+  for (int i = 0; i < 4; i++) {
+    QOP_CHECK_HIP(hipFree(fat[i]));
+    if (lng[i]) QOP_CHECK_HIP(hipFree(lng[i]));
+  }
+  bool have3 = c->three_staple != 0 || c->five_staple != 0 || c->seven_staple != 0 || c->lepage != 0;
+  if (info) {  // asqtad_dslash_p.c:536, 624-626
+    info->final_sec = dt;
+    info->final_flop = (have3 || naik) ? 61632.0 * lat->V : 0.0;
+    info->status = QOP_SUCCESS;
+  }
+}
+
+void rephase_dev(Lattice *lat, double *const U[4], int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign,
+                 int fat_slices_only, int nslice_long, double *const Ulong[4], int sign_rel) {
+  int z[4] = {0, 0, 0, 0};
+  int *rr = r0 ? r0 : z;
+  for (int mu = 0; mu < 4; mu++) {
+    int bcdir = bc && (bc->phase[mu].re != 1. || bc->phase[mu].im != 0.);
+    double br = bc ? bc->phase[mu].re : 1, bi = bc ? bc->phase[mu].im : 0;
+    int bits = sign ? sign->signmask[mu] : 0;
+    if (!bcdir && !bits) continue;
+    launch_rephase(lat, U[mu], mu, rr, br, bi, bcdir, bits, 1, sign_rel, stream());
+    if (Ulong && Ulong[mu]) launch_rephase(lat, Ulong[mu], mu, rr, br, bi, bcdir, bits, nslice_long, sign_rel, stream());
+  }
+  (void)fat_slices_only;
+  sync();
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- general ------------------------------------------------------------
+QOP_status_t QOP_init(QOP_layout_t *layout) {
+  if (G.inited) {
+    fprintf(stderr, "Error: QOP is initialized in function %s\n", __func__);
+    exit(1);
+  }
+  if (!layout || layout->latdim != 4) return QOP_FAIL;
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || nd < 1) {
+    g_last_error = "no HIP device";
+    fprintf(stderr, "QOP-MI355X: no HIP device available\n");
+    return QOP_FAIL;
+  }
+  G.inited = 1;
+  G.verbosity = 0;
+  G.lat = lattice_get(layout->latsize);
+  return QOP_SUCCESS;
+}
+
+QOP_status_t QOP_finalize(void) {
+  CHECK_INIT;
+  QOP_CHECK_HIP(hipDeviceSynchronize());
+  G.inited = 0;
+  return QOP_SUCCESS;
+}
+
+int QOP_is_initialized(void) { return G.inited; }
+const char *QOP_version_str(void) { return "0.21.1-mi355x"; }
+int QOP_version_int(void) { return (0 * 1000 + 21) * 1000 + 1; }
+int QOP_verbose(int level) {
+  int old = G.verbosity;
+  G.verbosity = level;
+  options().verbose = level >= QOP_VERB_MED;
+  return old;
+}
+int QOP_profcontrol(int level) {
+  int old = G.proflevel;
+  G.proflevel = level;
+  return old;
+}
+int QOP_node_number_raw(int coords[]) {
+  CHECK_INIT;
+  (void)coords;
+  return 0;
+}
+static int raw_index(const int x[4]) {
+  const int *L = G.lat->L;
+  long l = ((long)(x[0] * L[1] + x[1]) * L[2] + x[2]) * L[3] + x[3];
+  int p = (x[0] + x[1] + x[2] + x[3]) & 1;
+  return (int)(p ? (l + G.lat->V) / 2 : l / 2);
+}
+int QOP_node_index_raw_V(int coords[], QOP_evenodd_t) { CHECK_INIT; return raw_index(coords); }
+int QOP_node_index_raw_G(int coords[], QOP_evenodd_t) { CHECK_INIT; return raw_index(coords); }
+int QOP_sites_on_node_raw_V(QOP_evenodd_t) { CHECK_INIT; return (int)G.lat->V; }
+int QOP_sites_on_node_raw_G(QOP_evenodd_t) { CHECK_INIT; return (int)G.lat->V; }
+
+// lib/asqtad_invert.c:68-112 plus "qmx_*" tags of this library
+QOP_status_t QOP_asqtad_invert_set_opts(QOP_opt_t opts[], int nopts) {
+  CHECK_INIT;
+  Options &o = options();
+  int st = o.style, ns = o.nsvec, nm = o.nvec;
+  auto get = [&](const char *tag, auto &var) {
+    for (int i = 0; i < nopts; i++)
+      if (!strcmp(opts[i].tag, tag)) var = (std::remove_reference_t<decltype(var)>)opts[i].value;
+  };
+  auto valid_nvec = [](int nv, int s) { return nv == 1 || nv == 2 || nv == 4 || nv == 8 || (s == 1 && nv == 16); };
+  get("st", st);
+  if (st < 0 || st > 3) return QOP_FAIL;
+  if (!valid_nvec(ns, st)) ns = ns < 1 ? 1 : 8;
+  if (!valid_nvec(nm, st)) nm = nm < 1 ? 1 : 8;
+  get("ns", ns);
+  get("nm", nm);
+  if (!valid_nvec(ns, st) || !valid_nvec(nm, st)) return QOP_FAIL;
+  int cg = o.cgtype, nev = o.eigcg_nev, m = o.eigcg_m, numax = o.eigcg_numax;
+  get("cg", cg);
+  get("eigcg_nev", nev);
+  get("eigcg_m", m);
+  get("eigcg_l", numax);
+  int solver = o.solver, prec = o.inner_prec == P32 ? 32 : o.inner_prec == P16 ? 16 : 64;
+  double delta = o.delta;
+  int chunk = o.chunk, graph = o.use_graph;
+  get("qmx_solver", solver);
+  get("qmx_prec", prec);
+  get("qmx_delta", delta);
+  get("qmx_chunk", chunk);
+  get("qmx_graph", graph);
+  if (solver < 0 || solver > 1) return QOP_FAIL;
+  if (prec != 64 && prec != 32 && prec != 16) return QOP_FAIL;
+  if (!(delta > 0 && delta < 1) || chunk < 1 || chunk > 1024) return QOP_FAIL;
+  o.style = st; o.nsvec = ns; o.nvec = nm; o.cgtype = cg;
+  o.eigcg_nev = nev; o.eigcg_m = m; o.eigcg_numax = numax;
+  o.solver = solver;
+  o.inner_prec = prec == 64 ? P64 : prec == 32 ? P32 : P16;
+  o.delta = delta;
+  o.chunk = chunk;
+  o.use_graph = graph;
+  return QOP_SUCCESS;
+}
+
+// ---- vectors --------------------------------------------------------------
+QOP_D3_ColorVector *QOP_D3_create_V_from_raw(QOP_D_Real *src, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  auto *v = new QOP_D3_ColorVector;
+  devvec_alloc(v->v, G.lat, P64);
+  upload_vec(v->v, src, eo);
+  return v;
+}
+QOP_F3_ColorVector *QOP_F3_create_V_from_raw(QOP_F_Real *src, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  auto *v = new QOP_F3_ColorVector;
+  devvec_alloc(v->v, G.lat, P32);
+  upload_vec(v->v, src, eo);
+  return v;
+}
+void QOP_D3_extract_V_to_raw(QOP_D_Real *dest, QOP_D3_ColorVector *src, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  download_vec(dest, src->v, eo);
+}
+void QOP_F3_extract_V_to_raw(QOP_F_Real *dest, QOP_F3_ColorVector *src, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  download_vec(dest, src->v, eo);
+}
+void QOP_D3_destroy_V(QOP_D3_ColorVector *f) {
+  if (!f) return;
+  sync();
+  devvec_free(f->v);
+  delete f;
+}
+void QOP_F3_destroy_V(QOP_F3_ColorVector *f) {
+  if (!f) return;
+  sync();
+  devvec_free(f->v);
+  delete f;
+}
+QOP_D3_ColorVector *QOP_D3_convert_V_from_raw(QOP_D_Real *src, QOP_evenodd_t eo) {
+  QOP_D3_ColorVector *v = QOP_D3_create_V_from_raw(src, eo);
+  v->raw = src;
+  return v;
+}
+QOP_F3_ColorVector *QOP_F3_convert_V_from_raw(QOP_F_Real *src, QOP_evenodd_t eo) {
+  QOP_F3_ColorVector *v = QOP_F3_create_V_from_raw(src, eo);
+  v->raw = src;
+  return v;
+}
+// common_p.c:225-240; the raw array is sized for all sites (the reference
+// allocates sites_on_node reals only, a bug not reproduced here).
+QOP_D_Real *QOP_D3_convert_V_to_raw(QOP_D3_ColorVector *src, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  double *r = src->raw;
+  if (!r) r = (double *)calloc((size_t)src->v.lat->V * 6, sizeof(double));
+  download_vec(r, src->v, eo);
+  QOP_D3_destroy_V(src);
+  return r;
+}
+QOP_F_Real *QOP_F3_convert_V_to_raw(QOP_F3_ColorVector *src, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  float *r = src->raw;
+  if (!r) r = (float *)calloc((size_t)src->v.lat->V * 6, sizeof(float));
+  download_vec(r, src->v, eo);
+  QOP_F3_destroy_V(src);
+  return r;
+}
+
+// ---- gauge fields -----------------------------------------------------------
+QOP_D3_GaugeField *QOP_D3_create_G_from_raw(QOP_D_Real *links[], QOP_evenodd_t eo) {
+  CHECK_INIT;
+  auto *g = new QOP_D3_GaugeField;
+  g->lat = G.lat;
+  for (int i = 0; i < 4; i++) g->U[i] = upload_mat(G.lat, links[i], eo);
+  return g;
+}
+QOP_F3_GaugeField *QOP_F3_create_G_from_raw(QOP_F_Real *links[], QOP_evenodd_t eo) {
+  CHECK_INIT;
+  auto *g = new QOP_F3_GaugeField;
+  g->lat = G.lat;
+  for (int i = 0; i < 4; i++) g->U[i] = upload_mat(G.lat, links[i], eo);
+  return g;
+}
+void QOP_D3_extract_G_to_raw(QOP_D_Real *dest[], QOP_D3_GaugeField *src, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  for (int i = 0; i < 4; i++) download_mat(dest[i], src->U[i], src->lat, eo);
+}
+void QOP_F3_extract_G_to_raw(QOP_F_Real *dest[], QOP_F3_GaugeField *src, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  for (int i = 0; i < 4; i++) download_mat(dest[i], src->U[i], src->lat, eo);
+}
+void QOP_D3_destroy_G(QOP_D3_GaugeField *g) {
+  if (!g) return;
+  for (int i = 0; i < 4; i++) if (g->U[i]) QOP_CHECK_HIP(hipFree(g->U[i]));
+  delete g;
+}
+void QOP_F3_destroy_G(QOP_F3_GaugeField *g) {
+  if (!g) return;
+  for (int i = 0; i < 4; i++) if (g->U[i]) QOP_CHECK_HIP(hipFree(g->U[i]));
+  delete g;
+}
+// common_p.c:455-526: sign uses absolute coordinates, bc on the last slice
+void QOP_D3_rephase_G(QOP_D3_GaugeField *g, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign) {
+  CHECK_INIT;
+  rephase_dev(g->lat, g->U, r0, bc, sign, 1, 1, nullptr, 0);
+}
+void QOP_F3_rephase_G(QOP_F3_GaugeField *g, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign) {
+  CHECK_INIT;
+  rephase_dev(g->lat, g->U, r0, bc, sign, 1, 1, nullptr, 0);
+}
+
+// ---- links --------------------------------------------------------------------
+QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_raw(QOP_D_Real *fat[], QOP_D_Real *lng[], QOP_evenodd_t eo) {
+  CHECK_INIT;
+  auto *f = new QOP_D3_FermionLinksAsqtad;
+  f->L.lat = G.lat;
+  build_links_from_raw(f->L, fat, lng, eo);
+  return f;
+}
+QOP_F3_FermionLinksAsqtad *QOP_F3_asqtad_create_L_from_raw(QOP_F_Real *fat[], QOP_F_Real *lng[], QOP_evenodd_t eo) {
+  CHECK_INIT;
+  auto *f = new QOP_F3_FermionLinksAsqtad;
+  f->L.lat = G.lat;
+  build_links_from_raw(f->L, fat, lng, eo);
+  return f;
+}
+QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_G(QOP_info_t *info, QOP_asqtad_coeffs_t *c, QOP_D3_GaugeField *g) {
+  CHECK_INIT;
+  auto *f = new QOP_D3_FermionLinksAsqtad;
+  fatten(f->L, g->lat, g->U, g->U, c, info);
+  return f;
+}
+QOP_F3_FermionLinksAsqtad *QOP_F3_asqtad_create_L_from_G(QOP_info_t *info, QOP_asqtad_coeffs_t *c, QOP_F3_GaugeField *g) {
+  CHECK_INIT;
+  auto *f = new QOP_F3_FermionLinksAsqtad;
+  fatten(f->L, g->lat, g->U, g->U, c, info);
+  return f;
+}
+QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_G2(QOP_info_t *info, QOP_asqtad_coeffs_t *c, QOP_D3_GaugeField *gf, QOP_D3_GaugeField *gl) {
+  CHECK_INIT;
+  auto *f = new QOP_D3_FermionLinksAsqtad;
+  fatten(f->L, gf->lat, gf->U, gl->U, c, info);
+  return f;
+}
+static void extract_links(Links &L, double *fat[], double *lng[], int eo, float *ffat[], float *flng[]) {
+  Lattice *lat = L.lat;
+  double *fd[4], *ld[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t n = (size_t)lat->V * 18 * sizeof(double);
+  bool want_long = (lng || flng) && L.nl == 16;
+  for (int i = 0; i < 4; i++) {
+    QOP_CHECK_HIP(hipMalloc(&fd[i], n));
+    if (want_long) QOP_CHECK_HIP(hipMalloc(&ld[i], n));
+  }
+  extract_links64(L, fd, want_long ? ld : nullptr, stream());
+  for (int i = 0; i < 4; i++) {
+    if (fat) download_mat(fat[i], fd[i], lat, eo);
+    if (ffat) download_mat(ffat[i], fd[i], lat, eo);
+    if (want_long && lng) download_mat(lng[i], ld[i], lat, eo);
+    if (want_long && flng) download_mat(flng[i], ld[i], lat, eo);
+    QOP_CHECK_HIP(hipFree(fd[i]));
+    if (ld[i]) QOP_CHECK_HIP(hipFree(ld[i]));
+  }
+}
+void QOP_D3_asqtad_extract_L_to_raw(QOP_D_Real *fat[], QOP_D_Real *lng[], QOP_D3_FermionLinksAsqtad *src, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  extract_links(src->L, fat, lng, eo, nullptr, nullptr);
+}
+void QOP_F3_asqtad_extract_L_to_raw(QOP_F_Real *fat[], QOP_F_Real *lng[], QOP_F3_FermionLinksAsqtad *src, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  extract_links(src->L, nullptr, nullptr, eo, fat, lng);
+}
+void QOP_D3_asqtad_load_L_from_raw(QOP_D3_FermionLinksAsqtad *f, QOP_D_Real *fat[], QOP_D_Real *lng[], QOP_evenodd_t eo) {
+  CHECK_INIT;
+  build_links_from_raw(f->L, fat, lng, eo);
+}
+void QOP_D3_asqtad_load_L_from_G(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *f, QOP_asqtad_coeffs_t *c, QOP_D3_GaugeField *g) {
+  CHECK_INIT;
+  fatten(f->L, g->lat, g->U, g->U, c, info);
+}
+static void destroy_links(Links &L) {
+  sync();
+  linkset_free(L.l64);
+  linkset_free(L.l32);
+  linkset_free(L.l16);
+}
+void QOP_D3_asqtad_destroy_L(QOP_D3_FermionLinksAsqtad *f) {
+  if (!f) return;
+  destroy_links(f->L);
+  delete f;
+}
+void QOP_F3_asqtad_destroy_L(QOP_F3_FermionLinksAsqtad *f) {
+  if (!f) return;
+  destroy_links(f->L);
+  delete f;
+}
+// asqtad_dslash_p.c:1385-1482: sign relative to r0, bc on the last slice for
+// fat links and the last three slices for long links; then re-double-store.
+void QOP_D3_asqtad_rephase_L(QOP_D3_FermionLinksAsqtad *f, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign) {
+  CHECK_INIT;
+  Links &L = f->L;
+  Lattice *lat = L.lat;
+  double *fd[4], *ld[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t n = (size_t)lat->V * 18 * sizeof(double);
+  for (int i = 0; i < 4; i++) {
+    QOP_CHECK_HIP(hipMalloc(&fd[i], n));
+    if (L.nl == 16) QOP_CHECK_HIP(hipMalloc(&ld[i], n));
+  }
+  extract_links64(L, fd, L.nl == 16 ? ld : nullptr, stream());
+  rephase_dev(lat, fd, r0, bc, sign, 1, 3, L.nl == 16 ? ld : nullptr, 1);
+  const double *fc[4] = {fd[0], fd[1], fd[2], fd[3]}, *lc[4] = {ld[0], ld[1], ld[2], ld[3]};
+  build_links64(L, fc, L.nl == 16 ? lc : nullptr, stream());
+  for (int i = 0; i < 4; i++) {
+    QOP_CHECK_HIP(hipFree(fd[i]));
+    if (ld[i]) QOP_CHECK_HIP(hipFree(ld[i]));
+  }
+}
+// asqtad_dslash_p.c:742-790: the single-precision copy of double links
+QOP_F3_FermionLinksAsqtad *QOP_FD3_asqtad_create_L_from_L(QOP_D3_FermionLinksAsqtad *src) {
+  CHECK_INIT;
+  auto *f = new QOP_F3_FermionLinksAsqtad;
+  f->L.lat = src->L.lat;
+  f->L.nl = src->L.nl;
+  // keep an FP64 master with the FP32-rounded values so both views agree
+  LinkSet tmp;
+  convert_links(src->L.lat, src->L.l64, tmp, P32, stream());
+  f->L.l64.prec = P64;
+  f->L.l64.nl = src->L.nl;
+  size_t n = (size_t)2 * src->L.nl * 9 * src->L.lat->Vh;
+  QOP_CHECK_HIP(hipMalloc(&f->L.l64.data, n * sizeof(double2)));
+  {
+    // widen FP32 -> FP64 through the vector pack kernels' lincomb path:
+    // treat the link array as a batch of n/(3 Vh) parity halves.
+    LincombArgs a;
+    long Vh = src->L.lat->Vh;
+    a.Vh = (int)Vh;
+    a.n = (int)(n / (3 * Vh));
+    a.out.data = f->L.l64.data;
+    a.ostride = 3 * Vh;
+    a.v[0].data = tmp.data;
+    a.stride[0] = 3 * Vh;
+    a.cr[0] = 1.0;
+    int pin[1] = {P32};
+    launch_lincomb(a, P64, pin, 1, false, stream());
+  }
+  f->L.l32 = tmp;
+  sync();
+  return f;
+}
+QOP_D3_FermionLinksAsqtad *QOP_DF3_asqtad_create_L_from_L(QOP_F3_FermionLinksAsqtad *src) {
+  CHECK_INIT;
+  auto *f = new QOP_D3_FermionLinksAsqtad;
+  f->L.lat = src->L.lat;
+  f->L.nl = src->L.nl;
+  f->L.l64.prec = P64;
+  f->L.l64.nl = src->L.nl;
+  size_t n = (size_t)2 * src->L.nl * 9 * src->L.lat->Vh * sizeof(double2);
+  QOP_CHECK_HIP(hipMalloc(&f->L.l64.data, n));
+  QOP_CHECK_HIP(hipMemcpyAsync(f->L.l64.data, src->L.l64.data, n, hipMemcpyDeviceToDevice, stream()));
+  sync();
+  return f;
+}
+
+// ---- hot path ------------------------------------------------------------------
+void QOP_D3_asqtad_dslash(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *f, QOP_D_Real mass,
+                          QOP_D3_ColorVector *out, QOP_D3_ColorVector *in, QOP_evenodd_t eo_out,
+                          QOP_evenodd_t eo_in) {
+  CHECK_INIT;
+  (void)info;
+  dslash(f->L, mass, out->v, in->v, eo_out, eo_in);
+  sync();
+}
+void QOP_F3_asqtad_dslash(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *f, QOP_F_Real mass,
+                          QOP_F3_ColorVector *out, QOP_F3_ColorVector *in, QOP_evenodd_t eo_out,
+                          QOP_evenodd_t eo_in) {
+  CHECK_INIT;
+  (void)info;
+  dslash(f->L, mass, out->v, in->v, eo_out, eo_in);
+  sync();
+}
+void QOP_D3_asqtad_diaginv(QOP_info_t *, QOP_D3_FermionLinksAsqtad *, QOP_D_Real mass,
+                           QOP_D3_ColorVector *out, QOP_D3_ColorVector *in, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  scal(out->v, Cplx{1.0 / mass, 0}, in->v, eo);
+  sync();
+}
+void QOP_F3_asqtad_diaginv(QOP_info_t *, QOP_F3_FermionLinksAsqtad *, QOP_F_Real mass,
+                           QOP_F3_ColorVector *out, QOP_F3_ColorVector *in, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  scal(out->v, Cplx{(double)(1.0f / mass), 0}, in->v, eo);
+  sync();
+}
+
+}  // extern "C"
+
+// solver dispatch: reference algorithm, or this library's mixed precision
+namespace qop {
+void invert_dispatch(Links &L, InvArg *ia, ResArg *ra, double mass, DevVec &out, const DevVec &in,
+                     double *flop, double *sec);
+void invert_multi_dispatch(Links &L, InvArg *ia, ResArg **ra, const double *masses, const int *nmass,
+                           DevVec **out, DevVec *const *in, int nsrc, double *flop, double *sec);
+}  // namespace qop
+
+static InvArg to_ia(const QOP_invert_arg_t *a) {
+  return InvArg{a->mixed_rsq, a->max_iter, a->restart, a->max_restarts, (int)a->evenodd};
+}
+static ResArg to_ra(const QOP_resid_arg_t *r) {
+  return ResArg{r->rsqmin, r->final_rsq, r->relmin, r->final_rel, r->final_iter, r->final_restart};
+}
+static void from_ra(QOP_resid_arg_t *r, const ResArg &a) {
+  r->final_rsq = a.final_rsq;
+  r->final_rel = a.final_rel;
+  r->final_iter = a.final_iter;
+  r->final_restart = a.final_restart;
+}
+
+extern "C" {
+
+void QOP_D3_asqtad_invert(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *f, QOP_invert_arg_t *inv_arg,
+                          QOP_resid_arg_t *res_arg, QOP_D_Real mass, QOP_D3_ColorVector *out,
+                          QOP_D3_ColorVector *in) {
+  CHECK_INIT;
+  InvArg ia = to_ia(inv_arg);
+  ResArg ra = to_ra(res_arg);
+  double flop = 0, sec = 0;
+  invert_dispatch(f->L, &ia, &ra, mass, out->v, in->v, &flop, &sec);
+  from_ra(res_arg, ra);
+  if (info) { info->final_sec = sec; info->final_flop = flop; info->status = QOP_SUCCESS; }
+}
+
+// Single precision: the solve runs with FP64 vectors on the FP64 master of
+// the links and the result is rounded to FP32.
+void QOP_F3_asqtad_invert(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *f, QOP_invert_arg_t *inv_arg,
+                          QOP_resid_arg_t *res_arg, QOP_F_Real mass, QOP_F3_ColorVector *out,
+                          QOP_F3_ColorVector *in) {
+  CHECK_INIT;
+  InvArg ia = to_ia(inv_arg);
+  ResArg ra = to_ra(res_arg);
+  TmpVec o(out->v.lat, P64), i(in->v.lat, P64);
+  copy(o.v, out->v, 2);
+  copy(i.v, in->v, 2);
+  double flop = 0, sec = 0;
+  invert_dispatch(f->L, &ia, &ra, mass, o.v, i.v, &flop, &sec);
+  copy(out->v, o.v, inv_arg->evenodd);
+  sync();
+  from_ra(res_arg, ra);
+  if (info) { info->final_sec = sec; info->final_flop = flop; info->status = QOP_SUCCESS; }
+}
+
+void QOP_D3_asqtad_invert_multi(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *f, QOP_invert_arg_t *inv_arg,
+                                QOP_resid_arg_t **res_arg[], QOP_D_Real *masses[], int nmass[],
+                                QOP_D3_ColorVector **out_pt[], QOP_D3_ColorVector *in_pt[], int nsrc) {
+  CHECK_INIT;
+  InvArg ia = to_ia(inv_arg);
+  int nsolve = 0;
+  for (int i = 0; i < nsrc; i++) nsolve += nmass[i];
+  std::vector<ResArg> ra(nsolve);
+  std::vector<ResArg *> rap(nsolve);
+  std::vector<double> ms(nsolve);
+  std::vector<DevVec *> out(nsolve), in(nsrc);
+  int k = 0;
+  for (int i = 0; i < nsrc; i++) {
+    in[i] = &in_pt[i]->v;
+    for (int j = 0; j < nmass[i]; j++, k++) {
+      ra[k] = to_ra(res_arg[i][j]);
+      rap[k] = &ra[k];
+      ms[k] = masses[i][j];
+      out[k] = &out_pt[i][j]->v;
+    }
+  }
+  double flop = 0, sec = 0;
+  invert_multi_dispatch(f->L, &ia, rap.data(), ms.data(), nmass, out.data(), in.data(), nsrc, &flop, &sec);
+  k = 0;
+  for (int i = 0; i < nsrc; i++)
+    for (int j = 0; j < nmass[i]; j++, k++) from_ra(res_arg[i][j], ra[k]);
+  if (info) { info->final_sec = sec; info->final_flop = flop; info->status = QOP_SUCCESS; }
+}
+
+void QOP_F3_asqtad_invert_multi(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *f, QOP_invert_arg_t *inv_arg,
+                                QOP_resid_arg_t **res_arg[], QOP_F_Real *masses[], int nmass[],
+                                QOP_F3_ColorVector **out_pt[], QOP_F3_ColorVector *in_pt[], int nsrc) {
+  CHECK_INIT;
+  InvArg ia = to_ia(inv_arg);
+  int nsolve = 0;
+  for (int i = 0; i < nsrc; i++) nsolve += nmass[i];
+  std::vector<ResArg> ra(nsolve);
+  std::vector<ResArg *> rap(nsolve);
+  std::vector<double> ms(nsolve);
+  std::vector<TmpVec *> ov(nsolve), iv(nsrc);
+  std::vector<DevVec *> out(nsolve), in(nsrc);
+  int k = 0;
+  for (int i = 0; i < nsrc; i++) {
+    iv[i] = new TmpVec(in_pt[i]->v.lat, P64);
+    copy(iv[i]->v, in_pt[i]->v, 2);
+    in[i] = &iv[i]->v;
+    for (int j = 0; j < nmass[i]; j++, k++) {
+      ra[k] = to_ra(res_arg[i][j]);
+      rap[k] = &ra[k];
+      ms[k] = masses[i][j];
+      ov[k] = new TmpVec(out_pt[i][j]->v.lat, P64);
+      copy(ov[k]->v, out_pt[i][j]->v, 2);
+      out[k] = &ov[k]->v;
+    }
+  }
+  double flop = 0, sec = 0;
+  invert_multi_dispatch(f->L, &ia, rap.data(), ms.data(), nmass, out.data(), in.data(), nsrc, &flop, &sec);
+  int cg = inv_arg->evenodd == QOP_EVENODD ? QOP_EVEN : inv_arg->evenodd;
+  k = 0;
+  for (int i = 0; i < nsrc; i++)
+    for (int j = 0; j < nmass[i]; j++, k++) {
+      copy(out_pt[i][j]->v, ov[k]->v, cg);
+      from_ra(res_arg[i][j], ra[k]);
+    }
+  sync();
+  for (auto *p : ov) delete p;
+  for (auto *p : iv) delete p;
+  if (info) { info->final_sec = sec; info->final_flop = flop; info->status = QOP_SUCCESS; }
+}
+
+int QOPX_device_ok(void) {
+  int nd = 0;
+  return hipGetDeviceCount(&nd) == hipSuccess && nd > 0;
+}
+const char *QOPX_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

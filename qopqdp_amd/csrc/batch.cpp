@@ -1,0 +1,566 @@
+// Batched mixed-precision CG driver (see batch.h) and the hop autotuner.
+#include "batch.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+namespace qop {
+
+// ---------------------------------------------------------------------------
+// autotuner: time the multi-RHS hop for each launch shape and keep the best.
+HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &dst, int eo_out) {
+  static std::map<std::tuple<const Lattice *, int, int, int>, HopShape> cache;
+  auto key = std::make_tuple((const Lattice *)L.lat, prec, L.nl, nrhs);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  Lattice *lat = L.lat;
+  LinkSet &ls = L.get(prec);
+  hipStream_t st = stream();
+  HopArgs a;
+  a.g = lat->g;
+  a.parity = eo_out;
+  a.links = ls.parity_data(lat, eo_out);
+  a.lscale = ls.parity_scale(lat, eo_out);
+  a.src = src.slot(0);
+  a.dst = dst.slot(0);
+  a.sstride = a.dstride = 3 * lat->Vh;
+  a.nstride = lat->Vh;
+  a.n = nrhs;
+  hipEvent_t e0, e1;
+  QOP_CHECK_HIP(hipEventCreate(&e0));
+  QOP_CHECK_HIP(hipEventCreate(&e1));
+  HopShape best;
+  float best_ms = 1e30f;
+  const int nbs[] = {1, 2, 4, 8};
+  const int blocks[] = {64, 128, 256};
+  const int reps = lat->Vh > 200000 ? 3 : 10;
+  for (int nb : nbs) {
+    if (nb > 1 && nb > nrhs) continue;
+    for (int blk : blocks)
+      for (int rm = 0; rm < 2; rm++) {
+        a.remap = rm;
+        launch_hop_args(a, prec, ls.nl, nb, blk, RED_NONE, false, st);
+        QOP_CHECK_HIP(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; r++) launch_hop_args(a, prec, ls.nl, nb, blk, RED_NONE, false, st);
+        QOP_CHECK_HIP(hipEventRecord(e1, st));
+        QOP_CHECK_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        QOP_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) { best_ms = ms; best = HopShape{nb, blk, rm}; }
+      }
+  }
+  QOP_CHECK_HIP(hipEventDestroy(e0));
+  QOP_CHECK_HIP(hipEventDestroy(e1));
+  if (options().verbose)
+    printf("TUNE: prec %d nrhs %d -> nb %d block %d remap %d (%.3f ms/hop)\n", prec, nrhs, best.nb,
+           best.block, best.remap, best_ms / reps);
+  cache[key] = best;
+  return best;
+}
+
+// ---------------------------------------------------------------------------
+BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double delta)
+    : L_(L), lat_(L.lat), eo_(eo), cap_(capacity), sp_(sloppy_prec), delta_(delta) {
+  QOP_REQUIRE(eo == 0 || eo == 1, "batch solver: subset must be even or odd");
+  QOP_REQUIRE(capacity >= 1 && capacity <= 256, "batch solver: capacity 1..256");
+  QOP_REQUIRE(sp_ == P32 || sp_ == P16 || sp_ == P64, "batch solver: bad precision");
+  nch_ = (int)((lat_->Vh + 63) / 64);
+  devbatch_alloc(x64_, lat_, P64, cap_);
+  devbatch_alloc(r64_, lat_, P64, cap_);
+  devbatch_alloc(b64_, lat_, P64, cap_);
+  devbatch_alloc(ax64_, lat_, P64, cap_);
+  devbatch_alloc(t64_, lat_, P64, cap_);
+  devbatch_alloc(rs_, lat_, sp_, cap_);
+  devbatch_alloc(ps_, lat_, sp_, cap_);
+  devbatch_alloc(aps_, lat_, sp_, cap_);
+  devbatch_alloc(ts_, lat_, sp_, cap_);
+  devbatch_alloc(xs_, lat_, P32, cap_);
+  QOP_CHECK_HIP(hipMalloc(&rhs_, cap_ * sizeof(BsRhs)));
+  QOP_CHECK_HIP(hipHostMalloc(&rhs_h_, cap_ * sizeof(BsRhs), hipHostMallocDefault));
+  QOP_CHECK_HIP(hipMalloc(&active_, 2 * cap_ * sizeof(int)));
+  sel_ = active_ + cap_;
+  QOP_CHECK_HIP(hipHostMalloc(&active_h_, 2 * cap_ * sizeof(int), hipHostMallocDefault));
+  sel_h_ = active_h_ + cap_;
+  QOP_CHECK_HIP(hipMalloc(&glob_, sizeof(BsGlobal)));
+  QOP_CHECK_HIP(hipHostMalloc(&glob_h_, sizeof(BsGlobal), hipHostMallocDefault));
+  QOP_CHECK_HIP(hipMalloc(&partA_, 3 * (size_t)cap_ * nch_ * sizeof(double)));
+  partB_ = partA_ + (size_t)cap_ * nch_;
+  partC_ = partB_ + (size_t)cap_ * nch_;
+  QOP_CHECK_HIP(hipMalloc(&coef_, (2 + 6) * (size_t)cap_ * sizeof(double)));
+  lcoef_ = coef_ + 2 * cap_;
+  QOP_CHECK_HIP(hipMalloc(&sums_, 2 * (size_t)cap_ * sizeof(double)));
+  QOP_CHECK_HIP(hipHostMalloc(&sums_h_, 2 * (size_t)cap_ * sizeof(double), hipHostMallocDefault));
+  memset(rhs_h_, 0, cap_ * sizeof(BsRhs));
+  memset(active_h_, 0, 2 * cap_ * sizeof(int));
+  glob_h_->status = 0;
+  glob_h_->pending = 0;
+  hipStream_t st = stream();
+  QOP_CHECK_HIP(hipMemcpyAsync(rhs_, rhs_h_, cap_ * sizeof(BsRhs), hipMemcpyHostToDevice, st));
+  QOP_CHECK_HIP(hipMemcpyAsync(active_, active_h_, 2 * cap_ * sizeof(int), hipMemcpyHostToDevice, st));
+  QOP_CHECK_HIP(hipMemcpyAsync(glob_, glob_h_, sizeof(BsGlobal), hipMemcpyHostToDevice, st));
+  loaded_.assign(cap_, 0);
+  mass_.assign(cap_, 0.0);
+  HopShape hs = tune_hop(L_, sp_, cap_, ps_, ts_, 1 - eo_);
+  nb_ = hs.nb;
+  block_ = hs.block;
+  remap_ = hs.remap;
+  sync();
+}
+
+BatchSolver::~BatchSolver() {
+  sync();
+  if (graph_) (void)hipGraphExecDestroy(graph_);
+  for (DevBatch *b : {&x64_, &r64_, &b64_, &ax64_, &t64_, &rs_, &ps_, &aps_, &ts_, &xs_}) devbatch_free(*b);
+  (void)hipFree(rhs_);
+  (void)hipHostFree(rhs_h_);
+  (void)hipFree(active_);
+  (void)hipHostFree(active_h_);
+  (void)hipFree(glob_);
+  (void)hipHostFree(glob_h_);
+  (void)hipFree(partA_);
+  (void)hipFree(coef_);
+  (void)hipFree(sums_);
+  (void)hipHostFree(sums_h_);
+}
+
+int BatchSolver::free_slot() const {
+  for (int i = 0; i < cap_; i++)
+    if (!loaded_[i]) return i;
+  return -1;
+}
+
+int BatchSolver::nactive() const {
+  int n = 0;
+  for (int i = 0; i < cap_; i++) n += active_h_[i];
+  return n;
+}
+
+// helper: lincomb over the batch with per-slot selection
+static void batch_lincomb(Lattice *lat, int cap, const DevBatch &out, int nin, const DevBatch *const *in,
+                          const double *cr, const double *ci, const double *dev_coef, const int *sel,
+                          double *partial, hipStream_t st) {
+  LincombArgs a;
+  a.Vh = (int)lat->Vh;
+  a.n = cap;
+  a.out = out.slot(0);
+  a.ostride = 3 * lat->Vh;
+  a.nstride = lat->Vh;
+  int pin[3];
+  for (int i = 0; i < nin; i++) {
+    a.v[i] = in[i]->slot(0);
+    a.stride[i] = 3 * lat->Vh;
+    if (cr) { a.cr[i] = cr[i]; a.ci[i] = ci ? ci[i] : 0.0; }
+    pin[i] = in[i]->prec;
+  }
+  a.coef = dev_coef;
+  a.active = sel;
+  a.partial = partial;
+  launch_lincomb(a, out.prec, pin, nin, partial != nullptr, st);
+}
+
+void BatchSolver::apply_A64(const std::vector<int> &slots) {
+  (void)slots;  // sel_ already uploaded
+  hipStream_t st = stream();
+  LinkSet &ls = L_.get(P64);
+  const int opp = 1 - eo_;
+  HopArgs a;
+  a.g = lat_->g;
+  a.parity = opp;
+  a.links = ls.parity_data(lat_, opp);
+  a.lscale = ls.parity_scale(lat_, opp);
+  a.src = x64_.slot(0);
+  a.dst = t64_.slot(0);
+  a.sstride = a.dstride = a.xstride = 3 * lat_->Vh;
+  a.nstride = lat_->Vh;
+  a.n = cap_;
+  a.active = sel_;
+  launch_hop_args(a, P64, ls.nl, std::min(nb_, 4), 256, RED_NONE, false, st);
+  HopArgs b = a;
+  b.parity = eo_;
+  b.links = ls.parity_data(lat_, eo_);
+  b.lscale = ls.parity_scale(lat_, eo_);
+  b.src = t64_.slot(0);
+  b.dst = ax64_.slot(0);
+  b.x = x64_.slot(0);
+  b.coef = coef_;
+  launch_hop_args(b, P64, ls.nl, std::min(nb_, 4), 256, RED_NONE, true, st);
+}
+
+void BatchSolver::load(int slot, const DevVec &src, const DevVec *x0, double mass, double tol,
+                       int max_iter) {
+  QOP_REQUIRE(slot >= 0 && slot < cap_ && !loaded_[slot], "batch solver: slot not free");
+  QOP_REQUIRE(src.lat == lat_, "batch solver: lattice mismatch");
+  hipStream_t st = stream();
+  const long Vh = lat_->Vh;
+  auto copy_half = [&](VecView dst, int pdst, VecView s, int ps_) {
+    LincombArgs a;
+    a.Vh = (int)Vh;
+    a.n = 1;
+    a.out = dst;
+    a.v[0] = s;
+    a.cr[0] = 1.0;
+    int pin[1] = {ps_};
+    launch_lincomb(a, pdst, pin, 1, false, st);
+  };
+  copy_half(b64_.slot(slot), P64, src.half(eo_), P64);
+  if (x0) copy_half(x64_.slot(slot), P64, x0->half(eo_), P64);
+  else QOP_CHECK_HIP(hipMemsetAsync(x64_.slot(slot).data, 0, 3 * Vh * sizeof(double2), st));
+  // hop coefficients of this slot: A = m^2 - D D
+  double hc[2] = {-1.0, mass * mass};
+  QOP_CHECK_HIP(hipMemcpyAsync(coef_ + 2 * slot, hc, sizeof hc, hipMemcpyHostToDevice, st));
+  // r = b - A x (or b)
+  for (int i = 0; i < cap_; i++) sel_h_[i] = i == slot;
+  QOP_CHECK_HIP(hipMemcpyAsync(sel_, sel_h_, cap_ * sizeof(int), hipMemcpyHostToDevice, st));
+  if (x0) {
+    std::vector<int> s{slot};
+    apply_A64(s);
+    const DevBatch *in[2] = {&b64_, &ax64_};
+    double cr[2] = {1.0, -1.0};
+    batch_lincomb(lat_, cap_, r64_, 2, in, cr, nullptr, nullptr, sel_, partC_, st);
+  } else {
+    const DevBatch *in[1] = {&b64_};
+    double cr[1] = {1.0};
+    batch_lincomb(lat_, cap_, r64_, 1, in, cr, nullptr, nullptr, sel_, partC_, st);
+  }
+  bs_launch_sum(partC_, nch_, cap_, sel_, sums_, st);
+  // |b|^2
+  {
+    DotArgs d;
+    d.Vh = (int)Vh;
+    d.n = cap_;
+    d.u = b64_.slot(0);
+    d.ustride = 3 * Vh;
+    d.nstride = Vh;
+    d.partial = partA_;
+    d.active = sel_;
+    launch_dot(d, P64, P64, 0, st);
+    bs_launch_sum(partA_, nch_, cap_, sel_, sums_ + cap_, st);
+  }
+  // rs = r, ps = rs, xs = 0
+  {
+    const DevBatch *in[1] = {&r64_};
+    double cr[1] = {1.0};
+    batch_lincomb(lat_, cap_, rs_, 1, in, cr, nullptr, nullptr, sel_, nullptr, st);
+    const DevBatch *in2[1] = {&rs_};
+    batch_lincomb(lat_, cap_, ps_, 1, in2, cr, nullptr, nullptr, sel_, nullptr, st);
+    QOP_CHECK_HIP(hipMemsetAsync(xs_.slot(slot).data, 0, 3 * Vh * sizeof(float2), st));
+  }
+  QOP_CHECK_HIP(hipMemcpyAsync(sums_h_, sums_, 2 * cap_ * sizeof(double), hipMemcpyDeviceToHost, st));
+  sync();
+  BsRhs &R = rhs_h_[slot];
+  memset(&R, 0, sizeof R);
+  R.m2 = mass * mass;
+  R.r2 = sums_h_[slot];
+  R.r2_rel = R.r2;
+  R.b2 = sums_h_[cap_ + slot];
+  R.stop2 = tol * tol * R.b2;
+  R.max_iter = max_iter;
+  R.delta2 = delta_ * delta_;
+  mass_[slot] = mass;
+  loaded_[slot] = 1;
+  const bool done = R.r2 <= R.stop2;
+  if (done) R.reason = 1;
+  active_h_[slot] = done ? 0 : 1;
+  QOP_CHECK_HIP(hipMemcpyAsync(rhs_ + slot, &R, sizeof R, hipMemcpyHostToDevice, st));
+  QOP_CHECK_HIP(hipMemcpyAsync(active_ + slot, active_h_ + slot, sizeof(int), hipMemcpyHostToDevice, st));
+  sync();
+}
+
+void BatchSolver::enqueue_iteration() {
+  hipStream_t st = stream();
+  LinkSet &ls = L_.get(sp_);
+  const int opp = 1 - eo_;
+  const long Vh = lat_->Vh;
+  HopArgs a;
+  a.g = lat_->g;
+  a.parity = opp;
+  a.links = ls.parity_data(lat_, opp);
+  a.lscale = ls.parity_scale(lat_, opp);
+  a.src = ps_.slot(0);
+  a.dst = ts_.slot(0);
+  a.sstride = a.dstride = a.xstride = 3 * Vh;
+  a.nstride = Vh;
+  a.n = cap_;
+  a.active = active_;
+  a.status = &glob_->status;
+  a.partial = partA_;
+  a.remap = remap_;
+  launch_hop_args(a, sp_, ls.nl, nb_, block_, RED_OUT2, false, st);
+  HopArgs b = a;
+  b.parity = eo_;
+  b.links = ls.parity_data(lat_, eo_);
+  b.lscale = ls.parity_scale(lat_, eo_);
+  b.src = ts_.slot(0);
+  b.dst = aps_.slot(0);
+  b.x = ps_.slot(0);
+  b.coef = coef_;
+  b.partial = partB_;
+  launch_hop_args(b, sp_, ls.nl, nb_, block_, RED_X2, true, st);
+  bs_launch_scalar1(rhs_, active_, &glob_->status, partA_, partB_, nch_, cap_, st);
+  bs_launch_update(rhs_, active_, &glob_->status, sp_, (int)Vh, rs_.slot(0), aps_.slot(0), ps_.slot(0),
+                   xs_.slot(0), 3 * Vh, partC_, cap_, st);
+  bs_launch_scalar2(rhs_, active_, glob_, partC_, nch_, cap_, st);
+  bs_launch_pupdate(rhs_, active_, &glob_->status, sp_, (int)Vh, rs_.slot(0), ps_.slot(0), 3 * Vh, cap_, 1, st);
+  bs_launch_commit(glob_, st);
+}
+
+void BatchSolver::build_graph() {
+  const int chunk = options().chunk > 0 ? options().chunk : 16;
+  if (graph_ && graph_chunk_ == chunk) return;
+  if (graph_) QOP_CHECK_HIP(hipGraphExecDestroy(graph_));
+  hipStream_t st = stream();
+  hipGraph_t g;
+  QOP_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  for (int c = 0; c < chunk; c++) enqueue_iteration();
+  QOP_CHECK_HIP(hipStreamEndCapture(st, &g));
+  QOP_CHECK_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+  QOP_CHECK_HIP(hipGraphDestroy(g));
+  graph_chunk_ = chunk;
+}
+
+void BatchSolver::reliable_update(const std::vector<int> &slots) {
+  hipStream_t st = stream();
+  const long Vh = lat_->Vh;
+  for (int i = 0; i < cap_; i++) sel_h_[i] = 0;
+  for (int s : slots) sel_h_[s] = 1;
+  QOP_CHECK_HIP(hipMemcpyAsync(sel_, sel_h_, cap_ * sizeof(int), hipMemcpyHostToDevice, st));
+  // x += xs
+  {
+    const DevBatch *in[2] = {&x64_, &xs_};
+    double cr[2] = {1.0, 1.0};
+    batch_lincomb(lat_, cap_, x64_, 2, in, cr, nullptr, nullptr, sel_, nullptr, st);
+  }
+  // r = b - A x, |r|^2
+  apply_A64(slots);
+  {
+    const DevBatch *in[2] = {&b64_, &ax64_};
+    double cr[2] = {1.0, -1.0};
+    batch_lincomb(lat_, cap_, r64_, 2, in, cr, nullptr, nullptr, sel_, partC_, st);
+    bs_launch_sum(partC_, nch_, cap_, sel_, sums_, st);
+  }
+  // rs = r ; xs = 0
+  {
+    const DevBatch *in[1] = {&r64_};
+    double cr[1] = {1.0};
+    batch_lincomb(lat_, cap_, rs_, 1, in, cr, nullptr, nullptr, sel_, nullptr, st);
+    for (int s : slots) QOP_CHECK_HIP(hipMemsetAsync(xs_.slot(s).data, 0, 3 * Vh * sizeof(float2), st));
+  }
+  // <rs, ps> for the re-orthogonalisation
+  {
+    DotArgs d;
+    d.Vh = (int)Vh;
+    d.n = cap_;
+    d.u = rs_.slot(0);
+    d.v = ps_.slot(0);
+    d.ustride = d.vstride = 3 * Vh;
+    d.nstride = Vh;
+    d.partial = partA_;  // [2*cap][nch]
+    d.active = sel_;
+    launch_dot(d, sp_, sp_, 1, st);
+  }
+  // partA_ has room for cap*nch doubles per buffer; the complex dot writes 2*cap
+  // rows, i.e. partA_ and partB_ together.
+  std::vector<int> sel2(2 * cap_, 0);
+  for (int s : slots) { sel2[2 * s] = 1; sel2[2 * s + 1] = 1; }
+  QOP_CHECK_HIP(hipMemcpyAsync(rhs_h_, rhs_, cap_ * sizeof(BsRhs), hipMemcpyDeviceToHost, st));
+  QOP_CHECK_HIP(hipMemcpyAsync(sums_h_, sums_, cap_ * sizeof(double), hipMemcpyDeviceToHost, st));
+  sync();
+  std::vector<double> rp_re(cap_, 0), rp_im(cap_, 0);
+  {
+    // reduce the dot partials for selected slots (2 rows per slot)
+    double *dsum;
+    QOP_CHECK_HIP(hipMalloc(&dsum, 2 * cap_ * sizeof(double)));
+    int *dsel;
+    QOP_CHECK_HIP(hipMalloc(&dsel, 2 * cap_ * sizeof(int)));
+    QOP_CHECK_HIP(hipMemcpyAsync(dsel, sel2.data(), 2 * cap_ * sizeof(int), hipMemcpyHostToDevice, st));
+    bs_launch_sum(partA_, nch_, 2 * cap_, dsel, dsum, st);
+    std::vector<double> h(2 * cap_);
+    QOP_CHECK_HIP(hipMemcpyAsync(h.data(), dsum, 2 * cap_ * sizeof(double), hipMemcpyDeviceToHost, st));
+    sync();
+    for (int s : slots) { rp_re[s] = h[2 * s]; rp_im[s] = h[2 * s + 1]; }
+    QOP_CHECK_HIP(hipFree(dsum));
+    QOP_CHECK_HIP(hipFree(dsel));
+  }
+  // decide per slot; continuing slots get ps = (1 - beta rp) rs + beta ps
+  std::vector<double> lc(6 * cap_, 0.0);
+  for (int i = 0; i < cap_; i++) sel_h_[i] = 0;
+  bool any = false;
+  for (int s : slots) {
+    BsRhs &R = rhs_h_[s];
+    double r2t = sums_h_[s];
+    R.nupdate++;
+    R.need_update = 0;
+    if (r2t <= R.stop2) { R.reason = 1; active_h_[s] = 0; R.r2 = r2t; R.r2_rel = r2t; continue; }
+    if (R.iter >= R.max_iter) { R.reason = 2; active_h_[s] = 0; R.r2 = r2t; R.r2_rel = r2t; continue; }
+    if (!(R.pAp > 0)) { R.reason = 3; active_h_[s] = 0; R.r2 = r2t; R.r2_rel = r2t; continue; }
+    double rpr = rp_re[s] / r2t, rpi = rp_im[s] / r2t;
+    double beta = r2t / R.r2_old;
+    R.beta = beta;
+    R.r2 = r2t;
+    R.r2_rel = r2t;
+    // ps <- rs + beta (ps - rp rs) = (1 - beta rp) rs + beta ps
+    lc[6 * s + 0] = 1.0 - beta * rpr;
+    lc[6 * s + 1] = -beta * rpi;
+    lc[6 * s + 2] = beta;
+    lc[6 * s + 3] = 0.0;
+    sel_h_[s] = 1;
+    any = true;
+  }
+  QOP_CHECK_HIP(hipMemcpyAsync(rhs_, rhs_h_, cap_ * sizeof(BsRhs), hipMemcpyHostToDevice, st));
+  QOP_CHECK_HIP(hipMemcpyAsync(active_, active_h_, cap_ * sizeof(int), hipMemcpyHostToDevice, st));
+  if (any) {
+    QOP_CHECK_HIP(hipMemcpyAsync(lcoef_, lc.data(), 6 * cap_ * sizeof(double), hipMemcpyHostToDevice, st));
+    QOP_CHECK_HIP(hipMemcpyAsync(sel_, sel_h_, cap_ * sizeof(int), hipMemcpyHostToDevice, st));
+    const DevBatch *in[2] = {&rs_, &ps_};
+    batch_lincomb(lat_, cap_, ps_, 2, in, nullptr, nullptr, lcoef_, sel_, nullptr, st);
+  }
+  sync();
+}
+
+void BatchSolver::step(std::vector<int> &done) {
+  hipStream_t st = stream();
+  bool any = false;
+  for (int i = 0; i < cap_; i++) any |= active_h_[i] != 0;
+  if (any) {
+    if (options().use_graph) {
+      build_graph();
+      QOP_CHECK_HIP(hipGraphLaunch(graph_, st));
+    } else {
+      const int chunk = options().chunk > 0 ? options().chunk : 16;
+      for (int c = 0; c < chunk; c++) enqueue_iteration();
+    }
+    QOP_CHECK_HIP(hipMemcpyAsync(glob_h_, glob_, sizeof(BsGlobal), hipMemcpyDeviceToHost, st));
+    sync();
+    if (glob_h_->status != 0) {
+      QOP_CHECK_HIP(hipMemcpyAsync(rhs_h_, rhs_, cap_ * sizeof(BsRhs), hipMemcpyDeviceToHost, st));
+      sync();
+      std::vector<int> flagged;
+      for (int i = 0; i < cap_; i++)
+        if (active_h_[i] && rhs_h_[i].need_update) flagged.push_back(i);
+      if (!flagged.empty()) reliable_update(flagged);
+      glob_h_->status = 0;
+      glob_h_->pending = 0;
+      QOP_CHECK_HIP(hipMemcpyAsync(glob_, glob_h_, sizeof(BsGlobal), hipMemcpyHostToDevice, st));
+    }
+  }
+  for (int i = 0; i < cap_; i++)
+    if (loaded_[i] && !active_h_[i] && loaded_[i] == 1) {
+      loaded_[i] = 2;  // finished, not yet released
+      done.push_back(i);
+    }
+}
+
+void BatchSolver::run_all(std::vector<int> &done) {
+  while (nactive() > 0) step(done);
+  step(done);
+}
+
+void BatchSolver::result(int slot, DevVec &out, bool times_mass, SlotResult *res) {
+  QOP_REQUIRE(slot >= 0 && slot < cap_ && loaded_[slot], "batch solver: slot not loaded");
+  hipStream_t st = stream();
+  LincombArgs a;
+  a.Vh = (int)lat_->Vh;
+  a.n = 1;
+  a.out = out.half(eo_);
+  a.v[0] = x64_.slot(slot);
+  a.cr[0] = times_mass ? mass_[slot] : 1.0;
+  int pin[1] = {P64};
+  launch_lincomb(a, out.prec, pin, 1, false, st);
+  sync();
+  if (res) {
+    QOP_CHECK_HIP(hipMemcpy(rhs_h_ + slot, rhs_ + slot, sizeof(BsRhs), hipMemcpyDeviceToHost));
+    const BsRhs &R = rhs_h_[slot];
+    res->iter = R.iter;
+    res->nupdate = R.nupdate;
+    res->reason = R.reason;
+    res->true_rsq = R.b2 > 0 ? R.r2_rel / R.b2 : 0.0;
+  }
+}
+
+void BatchSolver::release(int slot) {
+  QOP_REQUIRE(slot >= 0 && slot < cap_, "batch solver: bad slot");
+  loaded_[slot] = 0;
+  active_h_[slot] = 0;
+  QOP_CHECK_HIP(hipMemcpyAsync(active_ + slot, active_h_ + slot, sizeof(int), hipMemcpyHostToDevice, stream()));
+}
+
+// ---------------------------------------------------------------------------
+// Solver dispatch for the QOP entry points: the reference algorithm by
+// default; with option qmx_solver = 1, EVEN/ODD single-mass solves use the
+// reliable-update mixed-precision solver and multi-mass solves use one batch
+// of per-mass systems.
+void invert_dispatch(Links &L, InvArg *ia, ResArg *ra, double mass, DevVec &out, const DevVec &in,
+                     double *flop, double *sec) {
+  const Options &o = options();
+  if (o.solver == 0 || ia->evenodd == 2) {
+    invert(L, ia, ra, mass, out, in, flop, sec);
+    return;
+  }
+  double t0 = wall_time();
+  BatchSolver bs(L, ia->evenodd, 1, o.inner_prec, o.delta);
+  double tol = std::sqrt(ra->rsqmin);
+  bs.load(0, in, nullptr, mass, tol, ia->max_iter);
+  std::vector<int> done;
+  bs.run_all(done);
+  SlotResult r;
+  bs.result(0, out, true, &r);
+  ra->final_iter = r.iter;
+  ra->final_rsq = r.true_rsq;
+  ra->final_rel = 0;
+  ra->final_restart = r.nupdate;
+  double nflop = 0.5 * (20 * 3 + 2 * 8 * 9 * L.nl);
+  if (flop) *flop = nflop * r.iter * (double)L.lat->V;
+  if (sec) *sec = wall_time() - t0;
+}
+
+void invert_multi_dispatch(Links &L, InvArg *ia, ResArg **ra, const double *masses, const int *nmass,
+                           DevVec **out, DevVec *const *in, int nsrc, double *flop, double *sec) {
+  const Options &o = options();
+  if (o.solver == 0 || ia->evenodd == 2) {
+    invert_multi(L, ia, ra, masses, nmass, out, in, nsrc, flop, sec);
+    return;
+  }
+  double t0 = wall_time();
+  int nsolve = 0;
+  for (int i = 0; i < nsrc; i++) nsolve += nmass[i];
+  const int cap = std::min(nsolve, 16);
+  BatchSolver bs(L, ia->evenodd, cap, o.inner_prec, o.delta);
+  std::vector<int> slot_of(cap, -1);
+  std::vector<int> src_of(nsolve);
+  {
+    int k = 0;
+    for (int i = 0; i < nsrc; i++)
+      for (int j = 0; j < nmass[i]; j++) src_of[k++] = i;
+  }
+  int next = 0, finished = 0;
+  long iters = 0;
+  while (finished < nsolve) {
+    int s;
+    while (next < nsolve && (s = bs.free_slot()) >= 0) {
+      bs.load(s, *in[src_of[next]], nullptr, masses[next], std::sqrt(ra[next]->rsqmin), ia->max_iter);
+      slot_of[s] = next++;
+    }
+    std::vector<int> done;
+    bs.step(done);
+    for (int d : done) {
+      int k = slot_of[d];
+      SlotResult r;
+      bs.result(d, *out[k], true, &r);
+      ra[k]->final_iter = r.iter;
+      ra[k]->final_rsq = r.true_rsq;
+      ra[k]->final_rel = 0;
+      ra[k]->final_restart = r.nupdate;
+      iters += r.iter;
+      bs.release(d);
+      finished++;
+    }
+  }
+  double nflop = 0.5 * (20 * 3 + 2 * 8 * 9 * L.nl);
+  if (flop) *flop = nflop * (double)iters * (double)L.lat->V;
+  if (sec) *sec = wall_time() - t0;
+}
+
+}  // namespace qop

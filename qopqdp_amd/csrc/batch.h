@@ -1,0 +1,126 @@
+// Batched mixed-precision CG with reliable updates.
+//
+// Each slot b solves (m_b^2 - D_eo D_oe) x_b = src_b on one parity of one
+// lattice, sharing the links of the batch.  The Krylov iteration runs in a
+// sloppy precision (FP32, or FP16 with per-site float norms) with FP64
+// reductions and scalars; whenever the iterated residual has dropped by
+// `delta` since the last reliable update (or appears converged, or hits the
+// iteration limit) the slot's FP64 solution is updated, its FP64 true
+// residual b - A x is recomputed and the search direction is re-orthogonalised
+// against it.  A slot stops when its FP64 true relative residual reaches its
+// tolerance.
+//
+// Every per-slot quantity (hop, reductions in a fixed order, updates) is
+// computed the same way whatever the other slots hold, so a slot's result
+// does not depend on what it is batched with.  Slots can be loaded and
+// released between iteration chunks (continuous batching).
+#pragma once
+#include "solver.h"
+
+namespace qop {
+
+struct BsRhs {
+  int need_update;  // reliable update requested by the scalar kernel
+  int iter;         // sloppy iterations done
+  int max_iter;
+  int reason;       // 0 running, 1 converged, 2 iteration limit, 3 breakdown
+  int nupdate;      // reliable updates done
+  int pad;
+  double m2;        // mass^2 of the slot's operator
+  double r2;        // iterated residual norm^2
+  double r2_old;    // iterated residual norm^2 of the previous iteration
+  double r2_rel;    // true residual norm^2 at the last reliable update
+  double pAp, alpha, beta;
+  double stop2;     // tol^2 |src|^2
+  double b2;        // |src|^2
+  double delta2;    // reliable-update threshold (delta^2)
+};
+
+struct BsGlobal {
+  int status;   // 0 running, 1 host attention needed (read by every kernel)
+  int pending;  // set by slots during an iteration, committed at its end
+};
+
+// Kernels (batch.hip)
+void bs_launch_scalar1(BsRhs *rhs, const int *active, const int *status,
+                       const double *partA, const double *partB, int nch, int n,
+                       hipStream_t st);
+void bs_launch_update(BsRhs *rhs, const int *active, const int *status, int prec,
+                      int Vh, VecView rs, VecView Aps, VecView ps, VecView xs,
+                      long stride, double *partC, int n, hipStream_t st);
+void bs_launch_scalar2(BsRhs *rhs, const int *active, BsGlobal *glob,
+                       const double *partC, int nch, int n, hipStream_t st);
+void bs_launch_commit(BsGlobal *glob, hipStream_t st);
+void bs_launch_pupdate(const BsRhs *rhs, const int *active, const int *status,
+                       int prec, int Vh, VecView rs, VecView ps, long stride, int n,
+                       int skip_flagged, hipStream_t st);
+void bs_launch_sum(const double *part, int nch, int n, const int *sel,
+                   double *out, hipStream_t st);
+
+struct SlotResult {
+  int iter = 0, nupdate = 0, reason = 0;
+  double true_rsq = 0;  // |src - A x|^2 / |src|^2 in FP64
+};
+
+class BatchSolver {
+ public:
+  BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double delta);
+  ~BatchSolver();
+  BatchSolver(const BatchSolver &) = delete;
+  BatchSolver &operator=(const BatchSolver &) = delete;
+
+  int capacity() const { return cap_; }
+  int free_slot() const;
+  int nactive() const;
+  // Load a system into a free slot.  src is a full FP64 vector (its `eo`
+  // half is used); x0 (optional) is an initial guess for A^-1 src.
+  void load(int slot, const DevVec &src, const DevVec *x0, double mass, double tol,
+            int max_iter);
+  // Run one chunk of iterations and service reliable updates.  Slots that
+  // finished are appended to `done`.
+  void step(std::vector<int> &done);
+  // Write mass * x (the m/A normalisation) or x into out's eo half.
+  void result(int slot, DevVec &out, bool times_mass, SlotResult *res);
+  void release(int slot);
+  // Run until every loaded slot is done.
+  void run_all(std::vector<int> &done);
+  double mass(int slot) const { return mass_[slot]; }
+
+ private:
+  void reliable_update(const std::vector<int> &slots);
+  void enqueue_iteration();
+  void build_graph();
+  void apply_A64(const std::vector<int> &slots);  // ax64 = A x64 on selected
+
+  Links &L_;
+  Lattice *lat_;
+  int eo_, cap_, sp_;
+  double delta_;
+  int nch_;
+  // FP64 per slot
+  DevBatch x64_, r64_, b64_, ax64_, t64_;
+  // sloppy
+  DevBatch rs_, ps_, aps_, ts_, xs_;
+  BsRhs *rhs_ = nullptr, *rhs_h_ = nullptr;
+  int *active_ = nullptr, *active_h_ = nullptr;
+  int *sel_ = nullptr, *sel_h_ = nullptr;
+  BsGlobal *glob_ = nullptr, *glob_h_ = nullptr;
+  double *partA_ = nullptr, *partB_ = nullptr, *partC_ = nullptr;
+  double *coef_ = nullptr;      // [cap][2] hop coefficients (-1, m2)
+  double *lcoef_ = nullptr;     // [cap][3][2] lincomb coefficients
+  double *sums_ = nullptr, *sums_h_ = nullptr;
+  std::vector<int> loaded_;
+  std::vector<double> mass_;
+  hipGraphExec_t graph_ = nullptr;
+  int graph_chunk_ = 0;
+  int nb_ = 4, block_ = 256, remap_ = 1;  // tuned hop launch shape
+};
+
+// Tuned hop launch shape for (lattice, precision, nl, nrhs); see tune.cpp.
+struct HopShape {
+  int nb = 1, block = 256, remap = 1;
+};
+HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &dst,
+                  int eo_out);
+
+}  // namespace qop

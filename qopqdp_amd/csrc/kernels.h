@@ -1,0 +1,63 @@
+// Kernel argument blocks and launchers (see kernels.hip).
+#pragma once
+#include "common.h"
+
+namespace qop {
+
+// Hop reductions per RHS (FP64): RED_OUT2 = |out|^2, RED_X2 = |x|^2.
+enum { RED_NONE = 0, RED_OUT2 = 1, RED_X2 = 2 };
+int hop_partial_blocks(const Lattice *lat);
+
+// Hop: dst[b] = alpha_b * sum_k W_k src[b](x+d_k) + beta_b * x[b], for the
+// output parity; slot b of a view is at data + b*stride complex elements and
+// norm + b*nstride floats.
+struct HopArgs {
+  Geom g;
+  int parity = 0;
+  const void *links = nullptr;   // links for this output parity: [nl][9][Vh]
+  const float *lscale = nullptr; // FP16: [nl][Vh]
+  VecView src{}, dst{}, x{};
+  long sstride = 0, dstride = 0, xstride = 0, nstride = 0;
+  int n = 1;
+  const double *coef = nullptr;  // device [n][2] (alpha, beta) or null
+  double alpha = 1.0, beta = 0.0;
+  double *partial = nullptr;     // [n][nchunk]
+  const int *active = nullptr;   // device [n] or null
+  const int *status = nullptr;   // device flag: kernel is a no-op when nonzero
+  int remap = 1;
+};
+void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
+                     bool hasx, hipStream_t st);
+
+// out[b] = sum_i c_i v_i[b] (complex c_i; coef device [n][3][2] or host cr/ci)
+struct LincombArgs {
+  int Vh = 0;
+  int n = 1;
+  VecView out{};
+  VecView v[3]{};
+  long ostride = 0, stride[3] = {0, 0, 0}, nstride = 0;
+  double cr[3] = {0, 0, 0}, ci[3] = {0, 0, 0};
+  const double *coef = nullptr;
+  int coef_neg = 0;               // bit i: negate device coefficient i
+  double *partial = nullptr;      // |out|^2 per chunk when requested
+  const int *active = nullptr;
+  const int *status = nullptr;
+};
+void launch_lincomb(const LincombArgs &a, int po, const int *pin, int nin,
+                    bool red, hipStream_t st);
+
+struct DotArgs {
+  int Vh = 0;
+  int n = 1;
+  VecView u{}, v{};
+  long ustride = 0, vstride = 0, nstride = 0;
+  double *partial = nullptr;  // mode 0: [n][nchunk]; mode 1: [2n][nchunk]
+  const int *active = nullptr;
+  const int *status = nullptr;
+};
+void launch_dot(const DotArgs &a, int pu, int pv, int mode, hipStream_t st);
+
+void launch_finalize(const double *partial, int m, int n, double *result,
+                     hipStream_t st, const int *status);
+
+}  // namespace qop

@@ -1,0 +1,572 @@
+// HIP kernels for the asqtad hot path on gfx950 (MI355X):
+//  * the even/odd staggered hop (1-hop fat + 3-hop Naik, 16 double-stored
+//    links per output site), single and multi right-hand side, in FP64,
+//    FP32 and FP16 (+ per-link scale / per-site norm), with a fused
+//    axpy epilogue and FP64 wavefront-shuffle reductions;
+//  * FP64 reductions (norm2, complex dot) and elementwise vector kernels;
+//  * raw <-> structure-of-arrays pack/unpack.
+//
+// The hop computes, for each output site x of parity p,
+//   out(x) = sum_k W_k(x) psi(x + d_k)        (lib/asqtad_dslash_p.c:1246-1279)
+// with d_k = (+mu, +3mu, -mu, -3mu) for k = 4mu+j (lib/asqtad_invert.c:27-36).
+//
+// Reductions are done per 64-site chunk (one wavefront) into a partial array
+// indexed by chunk, then summed in a fixed order by launch_finalize, so each
+// right-hand side's result does not depend on the block size, on the number of
+// right-hand sides per thread or on which other right-hand sides share the
+// launch.
+#include "common.h"
+#include "kernels.h"
+
+namespace qop {
+
+// ---------------------------------------------------------------------------
+// precision policies
+template <int P> struct PT;
+template <> struct PT<P64> { using R = double; using S = double2; };
+template <> struct PT<P32> { using R = float;  using S = float2; };
+template <> struct PT<P16> { using R = float;  using S = __half2; };
+
+template <int P>
+__device__ __forceinline__ void ld_c(const void *base, long i, float scl,
+                                     typename PT<P>::R &re, typename PT<P>::R &im) {
+  if constexpr (P == P64) {
+    double2 v = ((const double2 *)base)[i];
+    re = v.x; im = v.y;
+  } else if constexpr (P == P32) {
+    float2 v = ((const float2 *)base)[i];
+    re = v.x; im = v.y;
+  } else {
+    __half2 v = ((const __half2 *)base)[i];
+    float2 f = __half22float2(v);
+    re = f.x * scl; im = f.y * scl;
+  }
+}
+
+// Load the 3 colour components of site h from a parity-half view.
+template <int P>
+__device__ __forceinline__ void vload(const void *data, const float *norm, long h,
+                                      long Vh, typename PT<P>::R v[6]) {
+  float n = 1.0f;
+  if constexpr (P == P16) n = norm[h];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if constexpr (P == P64) {
+      double2 t = ((const double2 *)data)[c * Vh + h];
+      v[2 * c] = t.x; v[2 * c + 1] = t.y;
+    } else if constexpr (P == P32) {
+      float2 t = ((const float2 *)data)[c * Vh + h];
+      v[2 * c] = t.x; v[2 * c + 1] = t.y;
+    } else {
+      float2 t = __half22float2(((const __half2 *)data)[c * Vh + h]);
+      v[2 * c] = t.x * n; v[2 * c + 1] = t.y * n;
+    }
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void vstore(void *data, float *norm, long h, long Vh,
+                                       const typename PT<P>::R v[6]) {
+  if constexpr (P == P64) {
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      ((double2 *)data)[c * Vh + h] = make_double2(v[2 * c], v[2 * c + 1]);
+  } else if constexpr (P == P32) {
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      ((float2 *)data)[c * Vh + h] = make_float2(v[2 * c], v[2 * c + 1]);
+  } else {
+    float m = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 6; i++) m = fmaxf(m, fabsf(v[i]));
+    float inv = m > 0.0f ? 1.0f / m : 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      ((__half2 *)data)[c * Vh + h] =
+          __floats2half2_rn(v[2 * c] * inv, v[2 * c + 1] * inv);
+    norm[h] = m;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// wavefront reduction of a double (64 lanes) via cross-lane shuffles
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Site coordinates of output site h on parity p (raw order, x3 fastest).
+__device__ __forceinline__ void site_coords(const Geom &g, int h, int p, int x[4],
+                                            int &lex) {
+  int t = h / g.Lh3;
+  int x3h = h - t * g.Lh3;
+  int t2 = t / g.L[2];
+  x[2] = t - t2 * g.L[2];
+  x[0] = t2 / g.L[1];
+  x[1] = t2 - x[0] * g.L[1];
+  x[3] = 2 * x3h + ((x[0] + x[1] + x[2] + p) & 1);
+  lex = ((x[0] * g.L[1] + x[1]) * g.L[2] + x[2]) * g.L[3] + x[3];
+}
+
+__device__ __forceinline__ int nbr(const Geom &g, const int x[4], int lex, int mu,
+                                   int d) {
+  int y = x[mu] + d;
+  int l = lex + d * g.stride[mu];
+  if (y >= g.L[mu]) l -= g.L[mu] * g.stride[mu];
+  else if (y < 0) l += g.L[mu] * g.stride[mu];
+  return l >> 1;
+}
+
+// XCD-aware block remap: hardware dispatches block b to XCD b%8; give each
+// XCD a contiguous range of logical blocks so neighbouring sites share an L2.
+__device__ __forceinline__ int logical_block(int remap) {
+  if (!remap) return blockIdx.x;
+  int per = gridDim.x >> 3;
+  return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// hop kernel
+template <int P, int NL, int NB, int RED, bool HASX>
+__global__ void __launch_bounds__(256)
+hop_kernel(const HopArgs a) {
+  using R = typename PT<P>::R;
+  const Geom g = a.g;
+  const int Vh = g.Vh;
+  const int lb = logical_block(a.remap);
+  const int h = lb * blockDim.x + threadIdx.x;
+  const int b0 = blockIdx.y * NB;
+  if (a.status && *a.status != 0) return;
+
+  double red[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) red[b] = 0.0;
+
+  if (h < Vh) {
+    bool act[NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) act[b] = (b0 + b < a.n) && (!a.active || a.active[b0 + b]);
+    int x[4], lex;
+    site_coords(g, h, a.parity, x, lex);
+    R acc[NB][6];
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+      for (int i = 0; i < 6; i++) acc[b][i] = 0;
+
+    constexpr int PER = NL / 4;  // links per direction: 4 (fat+long) or 2
+#pragma unroll 1
+    for (int mu = 0; mu < 4; mu++) {
+#pragma unroll
+      for (int j = 0; j < PER; j++) {
+        const int d = PER == 4 ? (j == 0 ? 1 : j == 1 ? 3 : j == 2 ? -1 : -3)
+                               : (j == 0 ? 1 : -1);
+        const int k = PER * mu + j;
+        const int nb = nbr(g, x, lex, mu, d);
+        const long lbase = (long)k * 9 * Vh + h;
+        float scl = 1.0f;
+        if constexpr (P == P16) scl = a.lscale[(long)k * Vh + h];
+        R w[18];
+#pragma unroll
+        for (int e = 0; e < 9; e++) ld_c<P>(a.links, lbase + (long)e * Vh, scl, w[2 * e], w[2 * e + 1]);
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+          if (!act[b]) continue;
+          const char *sd = (const char *)a.src.data + (long)(b0 + b) * a.sstride * sizeof(typename PT<P>::S);
+          const float *sn = a.src.norm ? a.src.norm + (long)(b0 + b) * a.nstride : nullptr;
+          R v[6];
+          vload<P>(sd, sn, nb, Vh, v);
+#pragma unroll
+          for (int r = 0; r < 3; r++) {
+            R re = acc[b][2 * r], im = acc[b][2 * r + 1];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+              const R wr = w[6 * r + 2 * c], wi = w[6 * r + 2 * c + 1];
+              re = fma(wr, v[2 * c], re);
+              re = fma(-wi, v[2 * c + 1], re);
+              im = fma(wr, v[2 * c + 1], im);
+              im = fma(wi, v[2 * c], im);
+            }
+            acc[b][2 * r] = re; acc[b][2 * r + 1] = im;
+          }
+        }
+      }
+    }
+    // epilogue
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      if (!act[b]) continue;
+      const int rb = b0 + b;
+      double al = a.alpha, be = a.beta;
+      if (a.coef) { al = a.coef[2 * rb]; be = a.coef[2 * rb + 1]; }
+      R o[6];
+      R xv[6];
+      if constexpr (HASX) {
+        const char *xd = (const char *)a.x.data + (long)rb * a.xstride * sizeof(typename PT<P>::S);
+        const float *xn = a.x.norm ? a.x.norm + (long)rb * a.nstride : nullptr;
+        vload<P>(xd, xn, h, Vh, xv);
+#pragma unroll
+        for (int i = 0; i < 6; i++) o[i] = fma((R)be, xv[i], (R)al * acc[b][i]);
+      } else {
+#pragma unroll
+        for (int i = 0; i < 6; i++) o[i] = (R)al * acc[b][i];
+      }
+      char *dd = (char *)a.dst.data + (long)rb * a.dstride * sizeof(typename PT<P>::S);
+      float *dn = a.dst.norm ? a.dst.norm + (long)rb * a.nstride : nullptr;
+      vstore<P>(dd, dn, h, Vh, o);
+      if constexpr (RED == RED_OUT2) {
+        double s = 0;
+        if constexpr (P == P16) {  // reduce what was stored
+          R q[6];
+          vload<P>(dd, dn, h, Vh, q);
+#pragma unroll
+          for (int i = 0; i < 6; i++) s += (double)q[i] * (double)q[i];
+        } else {
+#pragma unroll
+          for (int i = 0; i < 6; i++) s += (double)o[i] * (double)o[i];
+        }
+        red[b] = s;
+      } else if constexpr (RED == RED_X2) {
+        double s = 0;
+#pragma unroll
+        for (int i = 0; i < 6; i++) s += (double)xv[i] * (double)xv[i];
+        red[b] = s;
+      }
+    }
+  }
+  if constexpr (RED != RED_NONE) {
+    const long nchunk = (Vh + 63) / 64;
+    const int chunk = h >> 6;
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      double s = wave_sum(red[b]);
+      if ((threadIdx.x & 63) == 0 && chunk < nchunk && b0 + b < a.n)
+        a.partial[(long)(b0 + b) * nchunk + chunk] = s;
+    }
+  }
+}
+
+int hop_partial_blocks(const Lattice *lat) { return (int)((lat->Vh + 63) / 64); }
+
+template <int P, int NL, int NB>
+static void hop_dispatch_red(const HopArgs &a, dim3 grid, dim3 block, int red,
+                             bool hasx, hipStream_t st) {
+#define QOP_LAUNCH(RD, HX) hipLaunchKernelGGL((hop_kernel<P, NL, NB, RD, HX>), grid, block, 0, st, a)
+  if (red == RED_NONE) { if (hasx) QOP_LAUNCH(RED_NONE, true); else QOP_LAUNCH(RED_NONE, false); }
+  else if (red == RED_OUT2) { if (hasx) QOP_LAUNCH(RED_OUT2, true); else QOP_LAUNCH(RED_OUT2, false); }
+  else { QOP_LAUNCH(RED_X2, true); }
+#undef QOP_LAUNCH
+}
+
+template <int P, int NL>
+static void hop_dispatch_nb(const HopArgs &a, int nb, dim3 grid, dim3 block,
+                            int red, bool hasx, hipStream_t st) {
+  switch (nb) {
+    case 1: hop_dispatch_red<P, NL, 1>(a, grid, block, red, hasx, st); break;
+    case 2: hop_dispatch_red<P, NL, 2>(a, grid, block, red, hasx, st); break;
+    case 4: hop_dispatch_red<P, NL, 4>(a, grid, block, red, hasx, st); break;
+    default: hop_dispatch_red<P, NL, 8>(a, grid, block, red, hasx, st); break;
+  }
+}
+
+void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
+                     bool hasx, hipStream_t st) {
+  QOP_REQUIRE(nb == 1 || nb == 2 || nb == 4 || nb == 8, "hop: rhs per thread must be 1,2,4,8");
+  QOP_REQUIRE(block % 64 == 0 && block >= 64 && block <= 256, "hop: block must be 64..256, multiple of 64");
+  QOP_REQUIRE(red == RED_NONE || hasx || red == RED_OUT2, "hop: RED_X2 needs x");
+  QOP_REQUIRE(a.n >= 1, "hop: n >= 1");
+  int nblk = (a.g.Vh + block - 1) / block;
+  if (a.remap) nblk = (nblk + 7) / 8 * 8;
+  dim3 grid(nblk, (a.n + nb - 1) / nb), blk(block);
+  if (prec == P64) {
+    if (nl == 16) hop_dispatch_nb<P64, 16>(a, nb, grid, blk, red, hasx, st);
+    else hop_dispatch_nb<P64, 8>(a, nb, grid, blk, red, hasx, st);
+  } else if (prec == P32) {
+    if (nl == 16) hop_dispatch_nb<P32, 16>(a, nb, grid, blk, red, hasx, st);
+    else hop_dispatch_nb<P32, 8>(a, nb, grid, blk, red, hasx, st);
+  } else {
+    if (nl == 16) hop_dispatch_nb<P16, 16>(a, nb, grid, blk, red, hasx, st);
+    else hop_dispatch_nb<P16, 8>(a, nb, grid, blk, red, hasx, st);
+  }
+  QOP_CHECK_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// finalize: result[b] = sum_i partial[b*m + i], fixed order
+__global__ void __launch_bounds__(256)
+finalize_kernel(const double *partial, int m, int nval, double *result,
+                const int *status) {
+  if (status && *status != 0) return;
+  __shared__ double sh[4];
+  const int b = blockIdx.x;
+  const double *p = partial + (long)b * m;
+  double s = 0;
+  for (int i = threadIdx.x; i < m; i += 256) s += p[i];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) result[b] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  (void)nval;
+}
+
+void launch_finalize(const double *partial, int m, int n, double *result,
+                     hipStream_t st, const int *status) {
+  hipLaunchKernelGGL(finalize_kernel, dim3(n), dim3(256), 0, st, partial, m, n,
+                     result, status);
+  QOP_CHECK_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// site-wise vector kernels (one thread per site of a parity half, n slots)
+//
+// generic linear combination:
+//   out = c0*v0 + c1*v1 (+ c2*v2), complex coefficients (host values or a
+//   device array coef[slot][3][2]), with an optional reduction of |out|^2.
+template <int PO, int P0, int P1, int P2, int NIN, bool RED>
+__global__ void __launch_bounds__(256)
+lincomb_kernel(const LincombArgs a) {
+  const int Vh = a.Vh;
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (a.status && *a.status != 0) return;
+  const bool act = !a.active || a.active[b];
+  double s = 0;
+  if (h < Vh && act) {
+    double cr[3], ci[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      if (a.coef) {
+        cr[i] = a.coef[(long)b * 6 + 2 * i]; ci[i] = a.coef[(long)b * 6 + 2 * i + 1];
+        if (a.coef_neg & (1 << i)) { cr[i] = -cr[i]; ci[i] = -ci[i]; }
+      } else { cr[i] = a.cr[i]; ci[i] = a.ci[i]; }
+    }
+    double o[6] = {0, 0, 0, 0, 0, 0};
+    auto accum = [&](auto tag, const VecView &v, long stride, double c_r, double c_i) {
+      constexpr int PP = decltype(tag)::value;
+      typename PT<PP>::R t[6];
+      const char *d = (const char *)v.data + (long)b * stride * sizeof(typename PT<PP>::S);
+      const float *nm = v.norm ? v.norm + (long)b * a.nstride : nullptr;
+      vload<PP>(d, nm, h, Vh, t);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        double tr = t[2 * c], ti = t[2 * c + 1];
+        o[2 * c] = fma(c_r, tr, fma(-c_i, ti, o[2 * c]));
+        o[2 * c + 1] = fma(c_r, ti, fma(c_i, tr, o[2 * c + 1]));
+      }
+    };
+    accum(std::integral_constant<int, P0>{}, a.v[0], a.stride[0], cr[0], ci[0]);
+    if constexpr (NIN > 1) accum(std::integral_constant<int, P1>{}, a.v[1], a.stride[1], cr[1], ci[1]);
+    if constexpr (NIN > 2) accum(std::integral_constant<int, P2>{}, a.v[2], a.stride[2], cr[2], ci[2]);
+    typename PT<PO>::R oo[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) oo[i] = (typename PT<PO>::R)o[i];
+    char *d = (char *)a.out.data + (long)b * a.ostride * sizeof(typename PT<PO>::S);
+    float *nm = a.out.norm ? a.out.norm + (long)b * a.nstride : nullptr;
+    vstore<PO>(d, nm, h, Vh, oo);
+    if constexpr (RED) {
+      typename PT<PO>::R q[6];
+      vload<PO>(d, nm, h, Vh, q);
+#pragma unroll
+      for (int i = 0; i < 6; i++) s += (double)q[i] * (double)q[i];
+    }
+  }
+  if constexpr (RED) {
+    const long nchunk = (Vh + 63) / 64;
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0 && (h >> 6) < nchunk) a.partial[(long)b * nchunk + (h >> 6)] = s;
+  }
+}
+
+template <int PO, int P0, int P1, int P2>
+static void lincomb_dispatch2(const LincombArgs &a, int nin, bool red, dim3 g,
+                              dim3 bl, hipStream_t st) {
+#define QOP_L(N, RD) hipLaunchKernelGGL((lincomb_kernel<PO, P0, P1, P2, N, RD>), g, bl, 0, st, a)
+  if (nin == 1) { if (red) QOP_L(1, true); else QOP_L(1, false); }
+  else if (nin == 2) { if (red) QOP_L(2, true); else QOP_L(2, false); }
+  else { if (red) QOP_L(3, true); else QOP_L(3, false); }
+#undef QOP_L
+}
+
+// Supported precision combinations: all same precision, or mixed
+// (out FP64 with FP32/FP16 inputs for accumulations, and conversions).
+void launch_lincomb(const LincombArgs &a, int po, const int *pin, int nin,
+                    bool red, hipStream_t st) {
+  dim3 g((a.Vh + 255) / 256, a.n), bl(256);
+  int p0 = pin[0], p1 = nin > 1 ? pin[1] : p0, p2 = nin > 2 ? pin[2] : p1;
+#define QOP_C(PO_, P0_, P1_, P2_)                                                      \
+  if (po == PO_ && p0 == P0_ && p1 == P1_ && p2 == P2_) {                              \
+    lincomb_dispatch2<PO_, P0_, P1_, P2_>(a, nin, red, g, bl, st);                     \
+    QOP_CHECK_HIP(hipGetLastError());                                                  \
+    return;                                                                            \
+  }
+  QOP_C(P64, P64, P64, P64)
+  QOP_C(P32, P32, P32, P32)
+  QOP_C(P16, P16, P16, P16)
+  // conversions and mixed accumulations
+  QOP_C(P32, P64, P64, P64)
+  QOP_C(P16, P64, P64, P64)
+  QOP_C(P64, P32, P32, P32)
+  QOP_C(P64, P16, P16, P16)
+  QOP_C(P64, P64, P32, P32)
+  QOP_C(P64, P64, P16, P16)
+  QOP_C(P32, P16, P16, P16)
+  QOP_C(P16, P32, P32, P32)
+  QOP_C(P32, P32, P16, P16)
+  QOP_C(P16, P16, P32, P32)
+#undef QOP_C
+  fatal(__FILE__, __LINE__, "lincomb: unsupported precision combination");
+}
+
+// ---------------------------------------------------------------------------
+// dot products: partial[b][chunk] = sum conj(u) v  (2 doubles) or |u|^2
+template <int PU, int PV, int MODE>  // MODE 0: |u|^2, 1: complex <u,v>
+__global__ void __launch_bounds__(256)
+dot_kernel(const DotArgs a) {
+  const int Vh = a.Vh;
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (a.status && *a.status != 0) return;
+  double sr = 0, si = 0;
+  if (h < Vh && (!a.active || a.active[b])) {
+    typename PT<PU>::R u[6];
+    vload<PU>((const char *)a.u.data + (long)b * a.ustride * sizeof(typename PT<PU>::S),
+              a.u.norm ? a.u.norm + (long)b * a.nstride : nullptr, h, Vh, u);
+    if constexpr (MODE == 0) {
+#pragma unroll
+      for (int i = 0; i < 6; i++) sr += (double)u[i] * (double)u[i];
+    } else {
+      typename PT<PV>::R v[6];
+      vload<PV>((const char *)a.v.data + (long)b * a.vstride * sizeof(typename PT<PV>::S),
+                a.v.norm ? a.v.norm + (long)b * a.nstride : nullptr, h, Vh, v);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        double ur = u[2 * c], ui = u[2 * c + 1], vr = v[2 * c], vi = v[2 * c + 1];
+        sr += ur * vr + ui * vi;
+        si += ur * vi - ui * vr;
+      }
+    }
+  }
+  const long nchunk = (Vh + 63) / 64;
+  sr = wave_sum(sr);
+  if constexpr (MODE == 1) si = wave_sum(si);
+  if ((threadIdx.x & 63) == 0 && (h >> 6) < nchunk) {
+    if constexpr (MODE == 0) a.partial[(long)b * nchunk + (h >> 6)] = sr;
+    else {
+      a.partial[(long)(2 * b) * nchunk + (h >> 6)] = sr;
+      a.partial[(long)(2 * b + 1) * nchunk + (h >> 6)] = si;
+    }
+  }
+}
+
+// Fermilab relative residual terms per site: |x|^2 == 0 ? 1 : |r|^2/|x|^2
+// (lib/invert_utilities_p.c:7-26); u = r, v = x.
+template <int P>
+__global__ void __launch_bounds__(256)
+relnorm_kernel(const DotArgs a) {
+  const int Vh = a.Vh;
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (a.status && *a.status != 0) return;
+  double s = 0;
+  if (h < Vh && (!a.active || a.active[b])) {
+    typename PT<P>::R r[6], x[6];
+    vload<P>((const char *)a.u.data + (long)b * a.ustride * sizeof(typename PT<P>::S),
+             a.u.norm ? a.u.norm + (long)b * a.nstride : nullptr, h, Vh, r);
+    vload<P>((const char *)a.v.data + (long)b * a.vstride * sizeof(typename PT<P>::S),
+             a.v.norm ? a.v.norm + (long)b * a.nstride : nullptr, h, Vh, x);
+    double r2 = 0, x2 = 0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) { r2 += (double)r[i] * r[i]; x2 += (double)x[i] * x[i]; }
+    s = x2 == 0.0 ? 1.0 : r2 / x2;
+  }
+  const long nchunk = (Vh + 63) / 64;
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0 && (h >> 6) < nchunk) a.partial[(long)b * nchunk + (h >> 6)] = s;
+}
+
+void launch_dot(const DotArgs &a, int pu, int pv, int mode, hipStream_t st) {
+  if (mode == 2) {
+    dim3 g((a.Vh + 255) / 256, a.n), bl(256);
+    QOP_REQUIRE(pu == pv, "relnorm: precisions must match");
+    if (pu == P64) hipLaunchKernelGGL((relnorm_kernel<P64>), g, bl, 0, st, a);
+    else if (pu == P32) hipLaunchKernelGGL((relnorm_kernel<P32>), g, bl, 0, st, a);
+    else hipLaunchKernelGGL((relnorm_kernel<P16>), g, bl, 0, st, a);
+    QOP_CHECK_HIP(hipGetLastError());
+    return;
+  }
+  dim3 g((a.Vh + 255) / 256, a.n), bl(256);
+#define QOP_D(PU_, PV_, M_) \
+  if (pu == PU_ && pv == PV_ && mode == M_) { hipLaunchKernelGGL((dot_kernel<PU_, PV_, M_>), g, bl, 0, st, a); QOP_CHECK_HIP(hipGetLastError()); return; }
+  QOP_D(P64, P64, 0) QOP_D(P64, P64, 1)
+  QOP_D(P32, P32, 0) QOP_D(P32, P32, 1)
+  QOP_D(P16, P16, 0) QOP_D(P16, P16, 1)
+  if (mode == 0) {
+    if (pu == P32) { hipLaunchKernelGGL((dot_kernel<P32, P32, 0>), g, bl, 0, st, a); QOP_CHECK_HIP(hipGetLastError()); return; }
+  }
+#undef QOP_D
+  fatal(__FILE__, __LINE__, "dot: unsupported precision combination");
+}
+
+// ---------------------------------------------------------------------------
+// raw (site-major [V][3][2]) <-> SoA vector, both precisions of raw
+template <typename T, int P>
+__global__ void raw_to_vec_kernel(const T *raw, void *data, float *norm, long Vh,
+                                  long begin, long end) {
+  long i = begin + blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= end) return;
+  int p = i >= Vh;
+  long h = i - p * Vh;
+  typename PT<P>::R v[6];
+#pragma unroll
+  for (int c = 0; c < 6; c++) v[c] = (typename PT<P>::R)raw[6 * i + c];
+  char *d = (char *)data + (size_t)p * 3 * Vh * sizeof(typename PT<P>::S);
+  vstore<P>(d, norm ? norm + p * Vh : nullptr, h, Vh, v);
+}
+
+template <typename T, int P>
+__global__ void vec_to_raw_kernel(T *raw, const void *data, const float *norm,
+                                  long Vh, long begin, long end) {
+  long i = begin + blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= end) return;
+  int p = i >= Vh;
+  long h = i - p * Vh;
+  typename PT<P>::R v[6];
+  const char *d = (const char *)data + (size_t)p * 3 * Vh * sizeof(typename PT<P>::S);
+  vload<P>(d, norm ? norm + p * Vh : nullptr, h, Vh, v);
+#pragma unroll
+  for (int c = 0; c < 6; c++) raw[6 * i + c] = (T)v[c];
+}
+
+static void eo_range(const Lattice *lat, int eo, long &b, long &e) {
+  b = eo == 1 ? lat->Vh : 0;
+  e = eo == 0 ? lat->Vh : lat->V;
+}
+
+template <typename T>
+static void raw_to_vec_t(const Lattice *lat, const T *raw, DevVec &v, int eo, hipStream_t st) {
+  long b, e;
+  eo_range(lat, eo, b, e);
+  dim3 g((e - b + 255) / 256), bl(256);
+  if (v.prec == P64) hipLaunchKernelGGL((raw_to_vec_kernel<T, P64>), g, bl, 0, st, raw, v.data, v.norm, lat->Vh, b, e);
+  else if (v.prec == P32) hipLaunchKernelGGL((raw_to_vec_kernel<T, P32>), g, bl, 0, st, raw, v.data, v.norm, lat->Vh, b, e);
+  else hipLaunchKernelGGL((raw_to_vec_kernel<T, P16>), g, bl, 0, st, raw, v.data, v.norm, lat->Vh, b, e);
+  QOP_CHECK_HIP(hipGetLastError());
+}
+template <typename T>
+static void vec_to_raw_t(const Lattice *lat, const DevVec &v, T *raw, int eo, hipStream_t st) {
+  long b, e;
+  eo_range(lat, eo, b, e);
+  dim3 g((e - b + 255) / 256), bl(256);
+  if (v.prec == P64) hipLaunchKernelGGL((vec_to_raw_kernel<T, P64>), g, bl, 0, st, raw, v.data, v.norm, lat->Vh, b, e);
+  else if (v.prec == P32) hipLaunchKernelGGL((vec_to_raw_kernel<T, P32>), g, bl, 0, st, raw, v.data, v.norm, lat->Vh, b, e);
+  else hipLaunchKernelGGL((vec_to_raw_kernel<T, P16>), g, bl, 0, st, raw, v.data, v.norm, lat->Vh, b, e);
+  QOP_CHECK_HIP(hipGetLastError());
+}
+
+void launch_raw_to_vec(const Lattice *lat, const double *raw, DevVec &v, int eo, hipStream_t st) { raw_to_vec_t(lat, raw, v, eo, st); }
+void launch_vec_to_raw(const Lattice *lat, const DevVec &v, double *raw, int eo, hipStream_t st) { vec_to_raw_t(lat, v, raw, eo, st); }
+void launch_rawf_to_vec(const Lattice *lat, const float *raw, DevVec &v, int eo, hipStream_t st) { raw_to_vec_t(lat, raw, v, eo, st); }
+void launch_vec_to_rawf(const Lattice *lat, const DevVec &v, float *raw, int eo, hipStream_t st) { vec_to_raw_t(lat, v, raw, eo, st); }
+
+}  // namespace qop

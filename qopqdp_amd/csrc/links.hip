@@ -1,0 +1,369 @@
+// Link preparation kernels on gfx950:
+//  * double storage of user-normalised fat/long links with the x1/2 factor
+//    (lib/asqtad_dslash_p.c:79-133, 152-223) into the hop's SoA layout;
+//  * precision copies (FP64 -> FP32, FP64 -> FP16 + per-link float scale);
+//  * extraction back to raw (x2, lib/asqtad_dslash_p.c:915-938);
+//  * staggered phases and boundary phases (lib/common_p.c:464-526,
+//    lib/asqtad_dslash_p.c:1385-1482);
+//  * asqtad fattening (lib/asqtad_dslash_p.c:391-633) on raw-layout matrices.
+#include "common.h"
+#include "kernels.h"
+
+namespace qop {
+
+__device__ __forceinline__ void raw_coords_d(const Geom &g, long i, int x[4]) {
+  const long Vh = g.Vh;
+  int p = i >= Vh;
+  long h = i - p * Vh;
+  long l = 2 * h;
+  long t = l;
+  x[3] = (int)(t % g.L[3]); t /= g.L[3];
+  x[2] = (int)(t % g.L[2]); t /= g.L[2];
+  x[1] = (int)(t % g.L[1]); t /= g.L[1];
+  x[0] = (int)t;
+  if (((x[0] + x[1] + x[2] + x[3]) & 1) != p) x[3] += 1;
+}
+
+__device__ __forceinline__ long raw_index_d(const Geom &g, const int x[4]) {
+  long l = ((long)(x[0] * g.L[1] + x[1]) * g.L[2] + x[2]) * g.L[3] + x[3];
+  int p = (x[0] + x[1] + x[2] + x[3]) & 1;
+  return (l >> 1) + (p ? (long)g.Vh : 0);
+}
+
+__device__ __forceinline__ long shift_d(const Geom &g, const int x[4], int mu, int d) {
+  int y[4] = {x[0], x[1], x[2], x[3]};
+  y[mu] = ((y[mu] + d) % g.L[mu] + g.L[mu]) % g.L[mu];
+  return raw_index_d(g, y);
+}
+
+// ---------------------------------------------------------------------------
+// double store: out[p][k][e][h]
+__global__ void dblstore_kernel(Geom g, const double *const *fat,
+                                const double *const *lng, int nl, double2 *out) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long V = 2L * g.Vh;
+  if (i >= V) return;
+  int x[4];
+  raw_coords_d(g, i, x);
+  const int p = i >= g.Vh;
+  const long h = i - p * g.Vh;
+  const int per = nl / 4;
+  for (int mu = 0; mu < 4; mu++) {
+    for (int j = 0; j < per; j++) {
+      const bool fwd = per == 4 ? (j < 2) : (j == 0);
+      const bool isl = per == 4 && (j & 1);
+      const double *src = isl ? lng[mu] : fat[mu];
+      const int d = isl ? 3 : 1;
+      const int k = per * mu + j;
+      double2 *o = out + ((long)p * nl + k) * 9 * g.Vh + h;
+      if (fwd) {
+        const double *m = src + 18 * i;
+        for (int e = 0; e < 9; e++) o[(long)e * g.Vh] = make_double2(0.5 * m[2 * e], 0.5 * m[2 * e + 1]);
+      } else {
+        const double *m = src + 18 * shift_d(g, x, mu, -d);
+        // -(1/2 m)^dagger: element (r,c) = -conj(m[c][r]) / 2
+        for (int r = 0; r < 3; r++)
+          for (int c = 0; c < 3; c++)
+            o[(long)(3 * r + c) * g.Vh] =
+                make_double2(-(0.5 * m[6 * c + 2 * r]), 0.5 * m[6 * c + 2 * r + 1]);
+      }
+    }
+  }
+}
+
+void build_links64(Links &L, const double *fat_dev[4], const double *lng_dev[4],
+                   hipStream_t st) {
+  const Lattice *lat = L.lat;
+  L.nl = lng_dev ? 16 : 8;
+  LinkSet &s = L.l64;
+  linkset_free(s);
+  s.prec = P64;
+  s.nl = L.nl;
+  QOP_CHECK_HIP(hipMalloc(&s.data, (size_t)2 * s.nl * 9 * lat->Vh * sizeof(double2)));
+  const double **ptrs;
+  QOP_CHECK_HIP(hipMalloc(&ptrs, 8 * sizeof(double *)));
+  const double *hp[8];
+  for (int i = 0; i < 4; i++) { hp[i] = fat_dev[i]; hp[4 + i] = lng_dev ? lng_dev[i] : nullptr; }
+  QOP_CHECK_HIP(hipMemcpyAsync(ptrs, hp, sizeof hp, hipMemcpyHostToDevice, st));
+  dim3 g((lat->V + 255) / 256), b(256);
+  hipLaunchKernelGGL(dblstore_kernel, g, b, 0, st, lat->g, ptrs, ptrs + 4, s.nl, (double2 *)s.data);
+  QOP_CHECK_HIP(hipGetLastError());
+  QOP_CHECK_HIP(hipStreamSynchronize(st));
+  QOP_CHECK_HIP(hipFree(ptrs));
+  linkset_free(L.l32);
+  linkset_free(L.l16);
+}
+
+// ---------------------------------------------------------------------------
+// precision copies: one thread per (p,k,h) link
+__global__ void to32_kernel(const double2 *in, float2 *out, long nlinks, long Vh) {
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= nlinks) return;
+  long pk = t / Vh, h = t - pk * Vh;
+  const double2 *a = in + pk * 9 * Vh + h;
+  float2 *o = out + pk * 9 * Vh + h;
+  for (int e = 0; e < 9; e++) {
+    double2 v = a[e * Vh];
+    o[e * Vh] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+__global__ void to16_kernel(const double2 *in, __half2 *out, float *scale,
+                            long nlinks, long Vh) {
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= nlinks) return;
+  long pk = t / Vh, h = t - pk * Vh;
+  const double2 *a = in + pk * 9 * Vh + h;
+  __half2 *o = out + pk * 9 * Vh + h;
+  double m = 0;
+  for (int e = 0; e < 9; e++) {
+    double2 v = a[e * Vh];
+    m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
+  }
+  float s = (float)m;
+  float inv = s > 0 ? 1.0f / s : 0.0f;
+  for (int e = 0; e < 9; e++) {
+    double2 v = a[e * Vh];
+    o[e * Vh] = __floats2half2_rn((float)v.x * inv, (float)v.y * inv);
+  }
+  scale[t] = s;
+}
+
+void convert_links(const Lattice *lat, const LinkSet &src, LinkSet &dst, int prec,
+                   hipStream_t st) {
+  QOP_REQUIRE(src.prec == P64, "convert_links: source must be FP64");
+  linkset_free(dst);
+  dst.prec = prec;
+  dst.nl = src.nl;
+  long nlinks = 2L * src.nl * lat->Vh;
+  QOP_CHECK_HIP(hipMalloc(&dst.data, (size_t)nlinks * 9 * prec_cbytes(prec)));
+  dim3 g((nlinks + 255) / 256), b(256);
+  if (prec == P32) {
+    hipLaunchKernelGGL(to32_kernel, g, b, 0, st, (const double2 *)src.data, (float2 *)dst.data, nlinks, lat->Vh);
+  } else if (prec == P16) {
+    QOP_CHECK_HIP(hipMalloc(&dst.scale, (size_t)nlinks * sizeof(float)));
+    hipLaunchKernelGGL(to16_kernel, g, b, 0, st, (const double2 *)src.data, (__half2 *)dst.data, dst.scale, nlinks, lat->Vh);
+  } else {
+    fatal(__FILE__, __LINE__, "convert_links: bad precision");
+  }
+  QOP_CHECK_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// extract: raw fat/long (user normalisation) from the forward FP64 links
+__global__ void extract_kernel(Geom g, const double2 *links, int nl, double *const *fat,
+                               double *const *lng) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long V = 2L * g.Vh;
+  if (i >= V) return;
+  const int p = i >= g.Vh;
+  const long h = i - p * g.Vh;
+  const int per = nl / 4;
+  for (int mu = 0; mu < 4; mu++) {
+    for (int j = 0; j < (per == 4 ? 2 : 1); j++) {
+      double *dst = j == 0 ? fat[mu] : lng[mu];
+      if (!dst) continue;
+      const int k = per * mu + j;
+      const double2 *w = links + ((long)p * nl + k) * 9 * g.Vh + h;
+      for (int e = 0; e < 9; e++) {
+        double2 v = w[(long)e * g.Vh];
+        dst[18 * i + 2 * e] = 2.0 * v.x;
+        dst[18 * i + 2 * e + 1] = 2.0 * v.y;
+      }
+    }
+  }
+}
+
+void extract_links64(const Links &L, double *fat_dev[4], double *lng_dev[4], hipStream_t st) {
+  const Lattice *lat = L.lat;
+  double **ptrs;
+  QOP_CHECK_HIP(hipMalloc(&ptrs, 8 * sizeof(double *)));
+  double *hp[8];
+  for (int i = 0; i < 4; i++) { hp[i] = fat_dev[i]; hp[4 + i] = lng_dev ? lng_dev[i] : nullptr; }
+  QOP_CHECK_HIP(hipMemcpyAsync(ptrs, hp, sizeof hp, hipMemcpyHostToDevice, st));
+  dim3 g((lat->V + 255) / 256), b(256);
+  hipLaunchKernelGGL(extract_kernel, g, b, 0, st, lat->g, (const double2 *)L.l64.data, L.nl, ptrs, ptrs + 4);
+  QOP_CHECK_HIP(hipGetLastError());
+  QOP_CHECK_HIP(hipStreamSynchronize(st));
+  QOP_CHECK_HIP(hipFree(ptrs));
+}
+
+// ---------------------------------------------------------------------------
+// phases on one direction's raw matrices.  bcdir: apply bc on the last
+// `nslice` slices of direction mu (relative to r0); sign bits use (x - r0) when
+// sign_rel, else absolute x (rephase_G uses absolute coordinates).
+__global__ void rephase_kernel(Geom g, double *U, int mu, int r00, int r01, int r02,
+                               int r03, double bcre, double bcim, int bcdir,
+                               int signbits, int nslice, int sign_rel) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long V = 2L * g.Vh;
+  if (i >= V) return;
+  int x[4];
+  raw_coords_d(g, i, x);
+  const int r0[4] = {r00, r01, r02, r03};
+  double *m = U + 18 * i;
+  if (bcdir) {
+    int rs = (x[mu] + g.L[mu] - r0[mu]) % g.L[mu];
+    if (rs >= g.L[mu] - nslice) {
+      for (int e = 0; e < 9; e++) {
+        double a = m[2 * e], b = m[2 * e + 1];
+        m[2 * e] = bcre * a - bcim * b;
+        m[2 * e + 1] = bcre * b + bcim * a;
+      }
+    }
+  }
+  if (signbits) {
+    int s = 0;
+    for (int j = 0; j < 4; j++) {
+      int c = sign_rel ? (x[j] + g.L[j] - r0[j]) % g.L[j] : x[j];
+      s += ((signbits >> j) & 1) * c;
+    }
+    if (s & 1)
+      for (int e = 0; e < 18; e++) m[e] = -m[e];
+  }
+}
+
+void launch_rephase(const Lattice *lat, double *U_dev, int mu, const int r0[4],
+                    double bcre, double bcim, int bcdir, int signbits, int nslice,
+                    int sign_rel, hipStream_t st) {
+  dim3 g((lat->V + 255) / 256), b(256);
+  hipLaunchKernelGGL(rephase_kernel, g, b, 0, st, lat->g, U_dev, mu, r0[0], r0[1], r0[2],
+                     r0[3], bcre, bcim, bcdir, signbits, nslice, sign_rel);
+  QOP_CHECK_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// fattening (compute_gen_staple / make_imp_links)
+__device__ __forceinline__ void mmul(double *c, const double *a, const double *b) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double re = 0, im = 0;
+      for (int k = 0; k < 3; k++) {
+        re += a[6 * i + 2 * k] * b[6 * k + 2 * j] - a[6 * i + 2 * k + 1] * b[6 * k + 2 * j + 1];
+        im += a[6 * i + 2 * k] * b[6 * k + 2 * j + 1] + a[6 * i + 2 * k + 1] * b[6 * k + 2 * j];
+      }
+      c[6 * i + 2 * j] = re; c[6 * i + 2 * j + 1] = im;
+    }
+}
+__device__ __forceinline__ void mmul_na(double *c, const double *a, const double *b) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double re = 0, im = 0;
+      for (int k = 0; k < 3; k++) {
+        re += a[6 * i + 2 * k] * b[6 * j + 2 * k] + a[6 * i + 2 * k + 1] * b[6 * j + 2 * k + 1];
+        im += -a[6 * i + 2 * k] * b[6 * j + 2 * k + 1] + a[6 * i + 2 * k + 1] * b[6 * j + 2 * k];
+      }
+      c[6 * i + 2 * j] = re; c[6 * i + 2 * j + 1] = im;
+    }
+}
+__device__ __forceinline__ void mmul_an(double *c, const double *a, const double *b) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double re = 0, im = 0;
+      for (int k = 0; k < 3; k++) {
+        re += a[6 * k + 2 * i] * b[6 * k + 2 * j] + a[6 * k + 2 * i + 1] * b[6 * k + 2 * j + 1];
+        im += a[6 * k + 2 * i] * b[6 * k + 2 * j + 1] - a[6 * k + 2 * i + 1] * b[6 * k + 2 * j];
+      }
+      c[6 * i + 2 * j] = re; c[6 * i + 2 * j + 1] = im;
+    }
+}
+
+__global__ void scale_kernel(double *o, const double *a, double s, long n) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) o[i] = s * a[i];
+}
+
+// upper(x) = U_nu(x) link(x+nu) U_nu(x+mu)^+, lower(x) = U_nu(x-nu)^+ link(x-nu) U_nu(x-nu+mu)
+__global__ void staple_kernel(Geom g, double *staple, int mu, int nu, const double *link,
+                              double coef, const double *gnu, double *fl) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long V = 2L * g.Vh;
+  if (i >= V) return;
+  int x[4], y[4];
+  raw_coords_d(g, i, x);
+  long xpnu = shift_d(g, x, nu, 1), xpmu = shift_d(g, x, mu, 1), xmnu = shift_d(g, x, nu, -1);
+  raw_coords_d(g, xmnu, y);
+  long xmnupmu = shift_d(g, y, mu, 1);
+  double t1[18], up[18], lo[18], a[18], b[18];
+  for (int e = 0; e < 18; e++) { a[e] = link[18 * xpnu + e]; b[e] = gnu[18 * xpmu + e]; }
+  mmul_na(t1, a, b);
+  for (int e = 0; e < 18; e++) a[e] = gnu[18 * i + e];
+  mmul(up, a, t1);
+  for (int e = 0; e < 18; e++) { a[e] = gnu[18 * xmnu + e]; b[e] = link[18 * xmnu + e]; }
+  mmul_an(t1, a, b);
+  for (int e = 0; e < 18; e++) a[e] = gnu[18 * xmnupmu + e];
+  mmul(lo, t1, a);
+  double *f = fl + 18 * i;
+  if (staple) {
+    double *s = staple + 18 * i;
+    for (int e = 0; e < 18; e++) {
+      double v = up[e] + lo[e];
+      s[e] = v;
+      f[e] += coef * v;
+    }
+  } else {
+    for (int e = 0; e < 18; e++) f[e] += coef * up[e];
+    for (int e = 0; e < 18; e++) f[e] += coef * lo[e];
+  }
+}
+
+// L(x) = naik * U(x) [U(x+mu) U(x+2mu)]
+__global__ void naik_kernel(Geom g, int mu, const double *U, double naik, double *out) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long V = 2L * g.Vh;
+  if (i >= V) return;
+  int x[4];
+  raw_coords_d(g, i, x);
+  long x1 = shift_d(g, x, mu, 1), x2 = shift_d(g, x, mu, 2);
+  double a[18], b[18], t[18], o[18];
+  for (int e = 0; e < 18; e++) { a[e] = U[18 * x1 + e]; b[e] = U[18 * x2 + e]; }
+  mmul(t, a, b);
+  for (int e = 0; e < 18; e++) a[e] = U[18 * i + e];
+  mmul(o, a, t);
+  for (int e = 0; e < 18; e++) out[18 * i + e] = naik * o[e];
+}
+
+void launch_fatten(const Lattice *lat, const double *gf[4], const double *gl[4],
+                   const double c[6], double *fat[4], double *lng[4], hipStream_t st) {
+  const long V = lat->V;
+  dim3 g((V + 255) / 256), b(256);
+  double coef1 = c[0], coef3 = c[1], coef5 = c[2], coef7 = c[3], coefL = c[4];
+  coef1 -= 6 * coefL;  // lib/asqtad_dslash_p.c:531
+  bool have5 = coef5 != 0 || coef7 != 0 || coefL != 0;
+  bool have3 = coef3 != 0 || have5;
+  double *staple = nullptr, *temp = nullptr;
+  if (have3) {
+    QOP_CHECK_HIP(hipMalloc(&staple, V * 18 * sizeof(double)));
+    QOP_CHECK_HIP(hipMalloc(&temp, V * 18 * sizeof(double)));
+  }
+  for (int dir = 0; dir < 4; dir++) {
+    dim3 gs((18 * V + 255) / 256);
+    hipLaunchKernelGGL(scale_kernel, gs, b, 0, st, fat[dir], gf[dir], coef1, 18 * V);
+    if (!have3) continue;
+    for (int nu = 0; nu < 4; nu++) {
+      if (nu == dir) continue;
+      hipLaunchKernelGGL(staple_kernel, g, b, 0, st, lat->g, staple, dir, nu, gf[dir], coef3, gf[nu], fat[dir]);
+      if (coefL != 0)
+        hipLaunchKernelGGL(staple_kernel, g, b, 0, st, lat->g, (double *)nullptr, dir, nu, (const double *)staple, coefL, gf[nu], fat[dir]);
+      if (coef5 != 0 || coef7 != 0) {
+        for (int rho = 0; rho < 4; rho++) {
+          if (rho == dir || rho == nu) continue;
+          hipLaunchKernelGGL(staple_kernel, g, b, 0, st, lat->g, temp, dir, rho, (const double *)staple, coef5, gf[rho], fat[dir]);
+          if (coef7 != 0)
+            for (int sig = 0; sig < 4; sig++)
+              if (sig != dir && sig != nu && sig != rho)
+                hipLaunchKernelGGL(staple_kernel, g, b, 0, st, lat->g, (double *)nullptr, dir, sig, (const double *)temp, coef7, gf[sig], fat[dir]);
+        }
+      }
+    }
+  }
+  if (lng)
+    for (int dir = 0; dir < 4; dir++)
+      hipLaunchKernelGGL(naik_kernel, g, b, 0, st, lat->g, dir, gl[dir], c[5], lng[dir]);
+  QOP_CHECK_HIP(hipGetLastError());
+  QOP_CHECK_HIP(hipStreamSynchronize(st));
+  if (staple) QOP_CHECK_HIP(hipFree(staple));
+  if (temp) QOP_CHECK_HIP(hipFree(temp));
+}
+
+}  // namespace qop

@@ -1,0 +1,130 @@
+// Runtime pieces of the MI355X asqtad library: error handling, lattice
+// cache, device allocation, the library stream and reduction scratch.
+#include <map>
+#include <mutex>
+
+#include "common.h"
+
+namespace qop {
+
+[[noreturn]] void fatal(const char *file, int line, const char *msg) {
+  fprintf(stderr, "QOP-MI355X fatal error at %s:%d: %s\n", file, line, msg);
+  fflush(stderr);
+  abort();
+}
+
+static std::mutex g_mu;
+
+Lattice *lattice_get(const int L[4]) {
+  static std::map<std::vector<int>, Lattice *> cache;
+  std::lock_guard<std::mutex> lk(g_mu);
+  std::vector<int> key(L, L + 4);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  for (int i = 0; i < 4; i++)
+    QOP_REQUIRE(L[i] >= 4 && L[i] % 2 == 0, "lattice extents must be even and >= 4");
+  Lattice *lat = new Lattice;
+  for (int i = 0; i < 4; i++) lat->L[i] = L[i];
+  lat->V = (long)L[0] * L[1] * L[2] * L[3];
+  QOP_REQUIRE(lat->V < (1L << 31), "lattice volume must fit in a 32-bit index");
+  lat->Vh = lat->V / 2;
+  Geom &g = lat->g;
+  for (int i = 0; i < 4; i++) g.L[i] = L[i];
+  g.Lh3 = L[3] / 2;
+  g.Vh = (int)lat->Vh;
+  g.stride[3] = 1;
+  g.stride[2] = L[3];
+  g.stride[1] = L[3] * L[2];
+  g.stride[0] = L[3] * L[2] * L[1];
+  cache[key] = lat;
+  return lat;
+}
+
+void devvec_alloc(DevVec &v, Lattice *lat, int prec) {
+  v.lat = lat;
+  v.prec = prec;
+  QOP_CHECK_HIP(hipMalloc(&v.data, (size_t)2 * 3 * lat->Vh * prec_cbytes(prec)));
+  QOP_CHECK_HIP(hipMemsetAsync(v.data, 0, (size_t)2 * 3 * lat->Vh * prec_cbytes(prec), stream()));
+  v.norm = nullptr;
+  if (prec == P16) {
+    QOP_CHECK_HIP(hipMalloc(&v.norm, (size_t)2 * lat->Vh * sizeof(float)));
+    QOP_CHECK_HIP(hipMemsetAsync(v.norm, 0, (size_t)2 * lat->Vh * sizeof(float), stream()));
+  }
+}
+
+void devvec_free(DevVec &v) {
+  if (v.data) QOP_CHECK_HIP(hipFree(v.data));
+  if (v.norm) QOP_CHECK_HIP(hipFree(v.norm));
+  v.data = nullptr;
+  v.norm = nullptr;
+}
+
+void devbatch_alloc(DevBatch &v, Lattice *lat, int prec, int n) {
+  v.lat = lat;
+  v.prec = prec;
+  v.n = n;
+  size_t bytes = (size_t)n * 3 * lat->Vh * prec_cbytes(prec);
+  QOP_CHECK_HIP(hipMalloc(&v.data, bytes));
+  QOP_CHECK_HIP(hipMemsetAsync(v.data, 0, bytes, stream()));
+  v.norm = nullptr;
+  if (prec == P16) {
+    QOP_CHECK_HIP(hipMalloc(&v.norm, (size_t)n * lat->Vh * sizeof(float)));
+    QOP_CHECK_HIP(hipMemsetAsync(v.norm, 0, (size_t)n * lat->Vh * sizeof(float), stream()));
+  }
+}
+
+void devbatch_free(DevBatch &v) {
+  if (v.data) QOP_CHECK_HIP(hipFree(v.data));
+  if (v.norm) QOP_CHECK_HIP(hipFree(v.norm));
+  v.data = nullptr;
+  v.norm = nullptr;
+  v.n = 0;
+}
+
+void linkset_free(LinkSet &l) {
+  if (l.data) QOP_CHECK_HIP(hipFree(l.data));
+  if (l.scale) QOP_CHECK_HIP(hipFree(l.scale));
+  l.data = nullptr;
+  l.scale = nullptr;
+  l.prec = -1;
+}
+
+LinkSet &Links::get(int prec) {
+  if (prec == P64) return l64;
+  LinkSet &d = prec == P32 ? l32 : l16;
+  if (!d.data) convert_links(lat, l64, d, prec, stream());
+  return d;
+}
+
+static hipStream_t g_stream = nullptr;
+hipStream_t stream() {
+  if (!g_stream) QOP_CHECK_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  return g_stream;
+}
+void sync() { QOP_CHECK_HIP(hipStreamSynchronize(stream())); }
+
+namespace {
+struct Scratch {
+  void *p = nullptr;
+  size_t n = 0;
+  bool host = false;
+  void *get(size_t bytes) {
+    if (bytes <= n) return p;
+    if (p) {
+      QOP_CHECK_HIP(hipStreamSynchronize(stream()));
+      if (host) QOP_CHECK_HIP(hipHostFree(p)); else QOP_CHECK_HIP(hipFree(p));
+    }
+    n = bytes < (1u << 20) ? (1u << 20) : bytes * 2;
+    if (host) QOP_CHECK_HIP(hipHostMalloc(&p, n, hipHostMallocDefault));
+    else QOP_CHECK_HIP(hipMalloc(&p, n));
+    return p;
+  }
+};
+Scratch s_partial, s_result, s_host{nullptr, 0, true};
+}  // namespace
+
+double *red_partial(size_t nd) { return (double *)s_partial.get(nd * sizeof(double)); }
+double *red_result(size_t nd) { return (double *)s_result.get(nd * sizeof(double)); }
+double *red_host(size_t nd) { return (double *)s_host.get(nd * sizeof(double)); }
+
+}  // namespace qop

@@ -1,0 +1,80 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol
+include/qop_mi355x.h declares, and the host-side input generator follows its
+recipe (SU(3) links, unit-variance Gaussian sources, seed determinism)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import qopqdp_amd as q
+from conftest import from_raw
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                      "qop_mi355x.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = set(re.findall(r"\b(QOPX?_\w+)\s*\(", src))
+    return {n for n in names if not n.endswith("_DEFAULT") and not n.endswith("_ZERO")}
+
+
+def test_library_exports_every_declared_symbol():
+    lib = q.lib()
+    decl = declared_functions()
+    assert len(decl) > 40
+    missing = [n for n in sorted(decl) if not hasattr(lib, n)]
+    assert not missing, missing
+    bound = {s[0] for s in q.SIGNATURES}
+    assert decl == bound, (decl - bound, bound - decl)
+
+
+def test_version_and_flags():
+    lib = q.lib()
+    assert lib.QOP_version_int() == 21001
+    assert lib.QOP_is_initialized() == 0
+
+
+def test_gen_gauge_is_su3_and_deterministic():
+    L = [4, 4, 4, 4]
+    U = q.gen_gauge(L, 1234, 0.3)
+    U2 = q.gen_gauge(L, 1234, 0.3)
+    U3 = q.gen_gauge(L, 1235, 0.3)
+    for mu in range(4):
+        assert np.array_equal(U[mu], U2[mu])
+        assert not np.array_equal(U[mu], U3[mu])
+        u = from_raw(U[mu])
+        uu = u @ np.conj(np.transpose(u, (0, 2, 1)))
+        assert np.abs(uu - np.eye(3)).max() < 1e-13
+        assert np.abs(np.linalg.det(u) - 1).max() < 1e-13
+        # exp(i eps H) with eps = 0.3 stays close to the identity
+        assert np.abs(u - np.eye(3)).max() < 1.5
+
+
+def test_gen_gauge_plaquette(oracle):
+    L = [4, 4, 4, 4]
+    U = q.gen_gauge(L, 1000, 0.3)
+    p = oracle.plaquette(L, U)
+    assert 0.6 < p < 0.99
+
+
+def test_gen_source_statistics():
+    L = [8, 8, 8, 8]
+    V = int(np.prod(L))
+    v = q.gen_source(L, 2000, q.QOP_EVEN)
+    assert np.all(v[V // 2:] == 0)
+    z = from_raw(v[: V // 2])
+    assert np.mean(np.abs(z) ** 2) == pytest.approx(1.0, rel=0.03)
+    assert abs(np.mean(z)) < 0.03
+    w = q.gen_source(L, 2000, q.QOP_EVENODD)
+    assert np.array_equal(w[: V // 2], v[: V // 2])
+
+
+def test_raw_coords_helper_matches_oracle(oracle):
+    L = [4, 6, 4, 8]
+    x = q.raw_coords_all(L)
+    for i in range(0, len(x), 41):
+        assert list(x[i]) == oracle.raw_coords(L, i)
+        assert q.raw_index(L, x[i]) == i
