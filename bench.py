@@ -1,0 +1,271 @@
+"""Benchmark of the asqtad hot path on one MI355X.
+
+Workload (32^3 x 64, the single-GPU configuration the metric is quoted on):
+seeded SU(3) gauge links U = reunit(exp(0.3 i H)), MILC phases with
+antiperiodic time, asqtad fattening with the MILC coefficients on the GPU,
+unit-variance Gaussian even-site sources, m = 0.01, FP64 true relative
+residual 1e-10 per solve.
+
+A step is one batch of B sources solved by the batched mixed-precision CG
+(sloppy FP16 vectors with per-site norms and FP16 links with per-link scales,
+FP32 arithmetic, FP64 reductions, reliable updates and true residuals).
+value = whole-job solves/s over the K timed steps; p50/p99 are the per-solve
+latencies (step start to that solve's completion).  Beside it the plain FP64
+GPU CG with the reference algorithm and stopping rule is timed on the same
+lattice and sources, and the FP64 true residual of every solve is checked.
+
+roofline: the dominant kernel of the step, the multi-RHS FP16 hop of the CG
+iteration (first hop, fused |D p|^2), timed with HIP events on the library
+stream; algorithmic bytes per launch = (16*40 + 2*B*16) * Vh.
+cpu_baseline: the C oracle's FP64 CG (the reference algorithm) on the host
+cores for a bounded number of iterations, extrapolated to the GPU FP64
+solve's iteration count.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "solves/s + p99 latency, mixed-prec batched vs FP64 CG at 1e-10 true resid"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--prec", type=int, default=16, choices=[16, 32])
+    ap.add_argument("--lattice", default="32,32,32,64")
+    ap.add_argument("--mass", type=float, default=0.01)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--fp64-solves", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the build has no "
+              "multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+
+    import qopqdp_amd as q
+
+    L = [int(x) for x in args.lattice.split(",")]
+    V = int(np.prod(L))
+    Vh = V // 2
+    B = args.batch
+    t0 = time.time()
+    U = q.gen_gauge(L, 1000 + 1 * 10 + 0, 0.3)  # gauge seed 1000 + cfg*10 + k
+    s = q.Session(L)
+    g = s.gauge(U)
+    del U
+    r0, bc, sign, keep = q.milc_rephase_args(L)
+    s.lib.QOP_D3_rephase_G(g, r0, C.byref(bc), C.byref(sign))
+    fla, finfo = s.links_from_gauge(g)
+    s.lib.QOP_D3_destroy_G(g)
+    log(f"setup {time.time() - t0:.1f}s (fattening {finfo.final_sec * 1e3:.1f} ms on GPU)")
+
+    srcs = [q.gen_source(L, 2000 + 100 * 1 + j, q.QOP_EVEN) for j in range(B)]
+    vins = [s.vec(x) for x in srcs]
+    vouts = [s.zeros() for _ in range(B)]
+    P = C.c_void_p
+
+    def batch_step():
+        ras = [q.resid_arg_default() for _ in range(B)]
+        for r in ras:
+            r.rsqmin = args.tol * args.tol
+        ia = q.invert_arg_default()
+        ia.evenodd = q.QOP_EVEN
+        ia.max_iter = 100000
+        lat = (C.c_double * B)()
+        masses = (C.c_double * B)(*([args.mass] * B))
+        info = q.QOPInfo()
+        ok = s.lib.QOPX_D3_asqtad_invert_batch(
+            C.byref(info), fla, C.byref(ia), (P * B)(*[C.cast(C.pointer(r), P) for r in ras]),
+            masses, (P * B)(*vouts), (P * B)(*vins), B, B, args.prec, lat)
+        return ok, ras, list(lat), info
+
+    # warmup (also autotunes the hop launch shapes)
+    for w in range(args.warmup):
+        batch_step()
+        log(f"warmup step {w} done")
+    lats, iters, steps_ms, flops = [], [], [], 0.0
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        ok, ras, lat, info = batch_step()
+        steps_ms.append((time.perf_counter() - ts) * 1e3)
+        assert ok == B, f"only {ok}/{B} systems converged"
+        lats += lat
+        iters += [r.final_iter for r in ras]
+        flops += info.final_flop
+        log(f"step {k}: {steps_ms[-1]:.0f} ms, max final_rsq {max(r.final_rsq for r in ras):.2e}")
+    elapsed = time.perf_counter() - t_start
+    value = B * args.steps / elapsed
+
+    # FP64 true residual of the last step's solutions, with the FP64 GPU operator
+    true_res = []
+    info = q.QOPInfo()
+    tmp, Ax = s.zeros(), s.zeros()
+    for j in range(B):
+        x = s.get(vouts[j]) / args.mass
+        vx = s.vec(x)
+        s.lib.QOP_D3_asqtad_dslash(info, fla, 0.0, tmp, vx, q.QOP_ODD, q.QOP_EVEN)
+        s.lib.QOP_D3_asqtad_dslash(info, fla, 0.0, Ax, tmp, q.QOP_EVEN, q.QOP_ODD)
+        ax = args.mass ** 2 * x - s.get(Ax)
+        r = srcs[j][:Vh] - ax[:Vh]
+        true_res.append(float(np.linalg.norm(r) / np.linalg.norm(srcs[j][:Vh])))
+        s.free(vx)
+    s.free(tmp)
+    s.free(Ax)
+
+    # plain FP64 GPU CG, reference algorithm and stopping rule
+    s.set_opts(qmx_solver=0)
+    fp64_t, fp64_it = [], []
+    for j in range(args.fp64_solves):
+        ia = q.invert_arg_default()
+        ia.evenodd = q.QOP_EVEN
+        ia.max_iter = 100000
+        ia.max_restarts = 50
+        ra = q.resid_arg_default()
+        ra.rsqmin = args.tol * args.tol
+        vo = s.zeros()
+        ts = time.perf_counter()
+        s.lib.QOP_D3_asqtad_invert(C.byref(info), fla, C.byref(ia), C.byref(ra), args.mass, vo, vins[j])
+        fp64_t.append(time.perf_counter() - ts)
+        fp64_it.append(ra.final_iter)
+        s.free(vo)
+        log(f"FP64 CG solve {j}: {fp64_t[-1]:.3f}s, {ra.final_iter} iterations, rsq {ra.final_rsq:.2e}")
+    fp64_rate = len(fp64_t) / sum(fp64_t)
+
+    # dominant kernel roofline: multi-RHS hop (first hop of the iteration)
+    shape = (C.c_int * 3)()
+    nrep = 50
+    hop_ms = s.lib.QOPX_D3_asqtad_hop_time(fla, args.prec, B, nrep, 1, shape)
+    s_link, s_vec = (40, 16) if args.prec == 16 else (72, 24)
+    hop_bytes = (16 * s_link + 2 * B * s_vec) * Vh
+    achieved = hop_bytes / (hop_ms * 1e-3) / 1e9
+    # single-RHS bandwidth baseline (FP64 / FP32 / FP16 even->odd Dslash)
+    single = {}
+    for prec, sl, sv in ((64, 144, 48), (32, 72, 24), (16, 40, 16)):
+        ms = s.lib.QOPX_D3_asqtad_hop_time(fla, prec, 1, 100, 0, None)
+        single[f"fp{prec}"] = dict(us=round(ms * 1e3, 1),
+                                    GBs=round((16 * sl + 2 * sv) * Vh / (ms * 1e-3) / 1e9, 1))
+    log("single-RHS hop:", single)
+
+    # CPU baseline: the oracle's FP64 CG, bounded sample
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(s, fla, L, srcs[0], args, float(np.mean(fp64_it)))
+    for v in vins + vouts:
+        s.free(v)
+    s.lib.QOP_D3_asqtad_destroy_L(fla)
+    s.close()
+
+    lat_ms = np.array(lats) * 1e3
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(float(np.mean(steps_ms)), 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"fp{args.prec} storage, fp32 arithmetic, fp64 reductions/true residual",
+        "data": "synthetic",
+        "config": {
+            "workload": f"asqtad even-subset solves {'x'.join(map(str, L))}, m={args.mass}, "
+                        f"{B} Gaussian sources per step, FP64 true residual {args.tol}",
+            "lattice": L, "batch": B, "mass": args.mass, "tol": args.tol,
+            "inner_prec": args.prec, "gauge": "reunit(exp(0.3iH)), MILC phases, asqtad smearing",
+        },
+        "p50_latency_ms": round(float(np.percentile(lat_ms, 50)), 1),
+        "p99_latency_ms": round(float(np.percentile(lat_ms, 99)), 1),
+        "fp64_cg_solves_per_s": round(fp64_rate, 3),
+        "speedup_vs_fp64_cg": round(value / fp64_rate, 2),
+        "fp64_cg_iters": int(np.mean(fp64_it)),
+        "mixed_iters_mean": round(float(np.mean(iters)), 1),
+        "max_fp64_true_resid": max(true_res),
+        "gflops_reference_model": round(flops / elapsed / 1e9, 1),
+        "single_rhs_dslash": single,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"hop_kernel fp{args.prec} x{B} RHS (fused |Dp|^2)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "launch_us": round(hop_ms * 1e3, 1),
+            "algorithmic_bytes": hop_bytes,
+            "shape": list(shape),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(s, fla, L, src, args, gpu_iters):
+    """Oracle FP64 CG on the host, bounded to about args.cpu_seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _oracle import Oracle
+    import qopqdp_amd as q
+    orc = Oracle()
+    fat, lng = s.extract_links(fla)
+    h = orc.links(L, fat, lng)
+    del fat, lng
+    # time a growing number of iterations until the sample is long enough
+    n, per_it = 2, None
+    while True:
+        t = time.perf_counter()
+        orc.invert(h, args.mass, src, q.QOP_EVEN, 1e-30, max_iter=n, restart=n + 10, max_restarts=0)
+        dt = time.perf_counter() - t
+        per_it = dt / n
+        if dt > args.cpu_seconds / 4 or n >= 64:
+            break
+        n *= 2
+    orc.free_links(h)
+    solve_s = per_it * gpu_iters
+    cores = os.cpu_count()
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    log(f"cpu baseline: {per_it * 1e3:.1f} ms/iteration over {n} iterations on {orc.threads()} threads")
+    return {
+        "value": round(1.0 / solve_s, 5),
+        "unit": "solves/s",
+        "cores": orc.threads(),
+        "kind": "port",
+        "sample": f"{n} FP64 CG iterations of the same solve on {orc.threads()} OpenMP threads "
+                  f"({cores} cores available, {model}); solves/s extrapolated to the GPU FP64 "
+                  f"solve's {int(gpu_iters)} iterations",
+        "ms_per_iteration": round(per_it * 1e3, 2),
+    }
+
+
+if __name__ == "__main__":
+    main()

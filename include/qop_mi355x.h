@@ -224,6 +224,24 @@ void QOP_F3_asqtad_invert_multi(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *asq
 void QOPX_gen_gauge(const int L[4], unsigned long long seed, double eps, double *U[4]);
 void QOPX_gen_gaussian_V(const int L[4], unsigned long long seed, int evenodd, double *v);
 
+/* Batched mixed-precision solves: out[i] = m_i A_i^-1 in[i] on the EVEN or
+ * ODD subset (inv_arg->evenodd), each to FP64 true relative residual
+ * sqrt(res[i]->rsqmin).  Up to `capacity` systems iterate together and a
+ * finished slot takes the next system (continuous batching).  prec: sloppy
+ * precision 32 or 16 (FP16 vectors with per-site norms, FP16 links with
+ * per-link scales).  latency[i] (optional): seconds from the call's start to
+ * system i's completion.  Returns the number of systems that converged. */
+int QOPX_D3_asqtad_invert_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla,
+                                QOP_invert_arg_t *inv_arg, QOP_resid_arg_t *res[], double masses[],
+                                QOP_D3_ColorVector *out[], QOP_D3_ColorVector *in[], int n,
+                                int capacity, int prec, double *latency);
+
+/* Mean ms per launch of the even->odd hop kernel (precision 64/32/16, nrhs
+ * right-hand sides, autotuned shape returned in shape[3]; red != 0 times the
+ * instance with the fused |out|^2 reduction), from HIP events. */
+double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, int nrep,
+                               int red, int *shape);
+
 /* Library status: 1 when a HIP device is usable. */
 int QOPX_device_ok(void);
 /* Last error message (empty when none). */

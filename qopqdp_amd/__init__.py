@@ -146,6 +146,10 @@ SIGNATURES = [
                                           C.POINTER(C.c_int), _PP, _PP, C.c_int]),
     ("QOPX_gen_gauge", None, [C.POINTER(C.c_int), C.c_ulonglong, C.c_double, _PP]),
     ("QOPX_gen_gaussian_V", None, [C.POINTER(C.c_int), C.c_ulonglong, C.c_int, _DP]),
+    ("QOPX_D3_asqtad_invert_batch", C.c_int, [C.POINTER(QOPInfo), _VP, C.POINTER(QOPInvertArg), _PP,
+                                              C.POINTER(C.c_double), _PP, _PP, C.c_int, C.c_int, C.c_int,
+                                              C.POINTER(C.c_double)]),
+    ("QOPX_D3_asqtad_hop_time", C.c_double, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     ("QOPX_device_ok", C.c_int, []),
     ("QOPX_last_error", C.c_char_p, []),
 ]

@@ -4,35 +4,10 @@
 #include <cstring>
 #include <string>
 
-#include "../../include/qop_mi355x.h"
+#include "handles.h"
 #include "solver.h"
 
 using namespace qop;
-
-struct QOP_D3_ColorVector_struct {
-  DevVec v;
-  double *raw = nullptr;  // set by convert_V_from_raw
-};
-struct QOP_F3_ColorVector_struct {
-  DevVec v;
-  float *raw = nullptr;
-};
-// Gauge fields are kept on the device in the raw layout (FP64), one array
-// per direction.
-struct QOP_D3_GaugeField_struct {
-  Lattice *lat = nullptr;
-  double *U[4] = {nullptr, nullptr, nullptr, nullptr};
-};
-struct QOP_F3_GaugeField_struct {
-  Lattice *lat = nullptr;
-  double *U[4] = {nullptr, nullptr, nullptr, nullptr};
-};
-struct QOP_D3_FermionLinksAsqtad_struct {
-  Links L;
-};
-struct QOP_F3_FermionLinksAsqtad_struct {
-  Links L;
-};
 
 namespace {
 

@@ -1,0 +1,143 @@
+// Extension entry points (QOPX_*): batched mixed-precision solves, hop
+// timing for roofline measurements, and link-precision control.  None of
+// these has a counterpart in the reference; they sit beside the QOP ABI.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "batch.h"
+#include "handles.h"
+
+using namespace qop;
+
+extern "C" {
+
+// Time `nrep` launches of the even->odd hop kernel in precision prec
+// (64/32/16) on nrhs right-hand sides with the autotuned launch shape.
+// red != 0 selects the instance with the fused |out|^2 reduction (the first
+// hop of a CG iteration).  Returns the mean ms per launch from HIP events on
+// the library stream; shape[3] receives (rhs per thread, block, XCD remap).
+double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, int nrep,
+                               int red, int *shape) {
+  Links &L = fla->L;
+  Lattice *lat = L.lat;
+  int p = prec == 64 ? P64 : prec == 32 ? P32 : P16;
+  QOP_REQUIRE(nrhs >= 1 && nrhs <= 64 && nrep >= 1, "hop_time: bad arguments");
+  DevBatch src, dst;
+  devbatch_alloc(src, lat, p, nrhs);
+  devbatch_alloc(dst, lat, p, nrhs);
+  // fill the source with a constant so FP16 norms are nonzero
+  {
+    std::vector<double> one((size_t)lat->V * 6, 0.5);
+    DevVec tmp;
+    devvec_alloc(tmp, lat, P64);
+    double *d;
+    QOP_CHECK_HIP(hipMalloc(&d, one.size() * sizeof(double)));
+    QOP_CHECK_HIP(hipMemcpy(d, one.data(), one.size() * sizeof(double), hipMemcpyHostToDevice));
+    launch_raw_to_vec(lat, d, tmp, 2, stream());
+    for (int b = 0; b < nrhs; b++) {
+      LincombArgs a;
+      a.Vh = (int)lat->Vh;
+      a.out = src.slot(b);
+      a.v[0] = tmp.half(0);
+      a.cr[0] = 1.0;
+      int pin[1] = {P64};
+      launch_lincomb(a, p, pin, 1, false, stream());
+    }
+    sync();
+    QOP_CHECK_HIP(hipFree(d));
+    devvec_free(tmp);
+  }
+  HopShape hs = tune_hop(L, p, nrhs, src, dst, 1);
+  LinkSet &ls = L.get(p);
+  HopArgs a;
+  a.g = lat->g;
+  a.parity = 1;
+  a.links = ls.parity_data(lat, 1);
+  a.lscale = ls.parity_scale(lat, 1);
+  a.src = src.slot(0);
+  a.dst = dst.slot(0);
+  a.sstride = a.dstride = 3 * lat->Vh;
+  a.nstride = lat->Vh;
+  a.n = nrhs;
+  a.remap = hs.remap;
+  double *part = nullptr;
+  if (red) {
+    QOP_CHECK_HIP(hipMalloc(&part, (size_t)nrhs * ((lat->Vh + 63) / 64) * sizeof(double)));
+    a.partial = part;
+  }
+  hipStream_t st = stream();
+  for (int w = 0; w < 3; w++) launch_hop_args(a, p, ls.nl, hs.nb, hs.block, red ? RED_OUT2 : RED_NONE, false, st);
+  hipEvent_t e0, e1;
+  QOP_CHECK_HIP(hipEventCreate(&e0));
+  QOP_CHECK_HIP(hipEventCreate(&e1));
+  QOP_CHECK_HIP(hipEventRecord(e0, st));
+  for (int r = 0; r < nrep; r++) launch_hop_args(a, p, ls.nl, hs.nb, hs.block, red ? RED_OUT2 : RED_NONE, false, st);
+  QOP_CHECK_HIP(hipEventRecord(e1, st));
+  QOP_CHECK_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  QOP_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+  QOP_CHECK_HIP(hipEventDestroy(e0));
+  QOP_CHECK_HIP(hipEventDestroy(e1));
+  if (part) QOP_CHECK_HIP(hipFree(part));
+  devbatch_free(src);
+  devbatch_free(dst);
+  if (shape) { shape[0] = hs.nb; shape[1] = hs.block; shape[2] = hs.remap; }
+  return ms / nrep;
+}
+
+// Batched mixed-precision solves of n EVEN (or ODD) systems
+// out[i] = m_i A_i^-1 in[i] (the QOP m/A normalisation), each to its own
+// FP64 true relative residual sqrt(res[i]->rsqmin).  Up to `capacity`
+// systems iterate together; a finished slot is refilled with the next system
+// (continuous batching).  prec: sloppy precision 32 or 16.  latency[i]
+// (optional) receives the seconds from the call's start to system i's
+// completion.  Returns the number of systems that reached their tolerance.
+int QOPX_D3_asqtad_invert_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla,
+                                QOP_invert_arg_t *inv_arg, QOP_resid_arg_t *res[], double masses[],
+                                QOP_D3_ColorVector *out[], QOP_D3_ColorVector *in[], int n,
+                                int capacity, int prec, double *latency) {
+  Links &L = fla->L;
+  QOP_REQUIRE(inv_arg->evenodd == QOP_EVEN || inv_arg->evenodd == QOP_ODD,
+              "invert_batch: subset must be EVEN or ODD");
+  QOP_REQUIRE(n >= 0 && capacity >= 1, "invert_batch: bad sizes");
+  if (n == 0) return 0;
+  const int sp = prec == 16 ? P16 : prec == 64 ? P64 : P32;
+  double t0 = wall_time();
+  const int cap = std::min(capacity, n);
+  BatchSolver bs(L, inv_arg->evenodd, cap, sp, options().delta);
+  std::vector<int> slot_of(cap, -1);
+  int next = 0, finished = 0, ok = 0;
+  long iters = 0;
+  while (finished < n) {
+    int s;
+    while (next < n && (s = bs.free_slot()) >= 0) {
+      bs.load(s, in[next]->v, nullptr, masses[next], std::sqrt(res[next]->rsqmin), inv_arg->max_iter);
+      slot_of[s] = next++;
+    }
+    std::vector<int> done;
+    bs.step(done);
+    for (int d : done) {
+      int k = slot_of[d];
+      SlotResult r;
+      bs.result(d, out[k]->v, true, &r);
+      res[k]->final_iter = r.iter;
+      res[k]->final_rsq = r.true_rsq;
+      res[k]->final_rel = 0;
+      res[k]->final_restart = r.nupdate;
+      ok += r.reason == 1;
+      iters += r.iter;
+      if (latency) latency[k] = wall_time() - t0;
+      bs.release(d);
+      finished++;
+    }
+  }
+  if (info) {
+    info->final_sec = wall_time() - t0;
+    info->final_flop = 0.5 * (20 * 3 + 2 * 8 * 9 * L.nl) * (double)iters * (double)L.lat->V;
+    info->status = QOP_SUCCESS;
+  }
+  return ok;
+}
+
+}  // extern "C"
