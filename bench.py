@@ -54,6 +54,8 @@ def main():
     ap.add_argument("--fp64-solves", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only time the dominant hop kernel (for the rocprofv3 cross-check)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the build has no "
@@ -77,6 +79,20 @@ def main():
     s.lib.QOP_D3_destroy_G(g)
     log(f"setup {time.time() - t0:.1f}s (fattening {finfo.final_sec * 1e3:.1f} ms on GPU)")
 
+    if args.roofline_only:
+        shape = (C.c_int * 5)()
+        ms = s.lib.QOPX_D3_asqtad_hop_time(fla, args.prec, B, 50, 1, shape)
+        s_link, s_vec = (40, 16) if args.prec == 16 else (72, 24)
+        byts = (16 * s_link + 2 * B * s_vec) * Vh
+        cbytes = 1 << 30
+        cms = s.lib.QOPX_copy_time(cbytes, 20)
+        print(json.dumps({"kernel_us": round(ms * 1e3, 1), "GBs": round(byts / ms / 1e6, 1),
+                          "shape": list(shape), "algorithmic_bytes": byts,
+                          "copy_bytes_read": cbytes, "copy_us": round(cms * 1e3, 1),
+                          "copy_GBs_rw": round(2 * cbytes / cms / 1e6, 1)}))
+        s.lib.QOP_D3_asqtad_destroy_L(fla)
+        s.close()
+        return
     srcs = [q.gen_source(L, 2000 + 100 * 1 + j, q.QOP_EVEN) for j in range(B)]
     vins = [s.vec(x) for x in srcs]
     vouts = [s.zeros() for _ in range(B)]
@@ -151,7 +167,7 @@ def main():
     fp64_rate = len(fp64_t) / sum(fp64_t)
 
     # dominant kernel roofline: multi-RHS hop (first hop of the iteration)
-    shape = (C.c_int * 3)()
+    shape = (C.c_int * 5)()
     nrep = 50
     hop_ms = s.lib.QOPX_D3_asqtad_hop_time(fla, args.prec, B, nrep, 1, shape)
     s_link, s_vec = (40, 16) if args.prec == 16 else (72, 24)

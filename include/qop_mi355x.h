@@ -237,10 +237,15 @@ int QOPX_D3_asqtad_invert_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla
                                 int capacity, int prec, double *latency);
 
 /* Mean ms per launch of the even->odd hop kernel (precision 64/32/16, nrhs
- * right-hand sides, autotuned shape returned in shape[3]; red != 0 times the
+ * right-hand sides, autotuned shape (rhs per thread, block, XCD remap,
+ * non-temporal links, RHS-innermost order) returned in shape[5]; red != 0 times the
  * instance with the fused |out|^2 reduction), from HIP events. */
 double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, int nrep,
                                int red, int *shape);
+
+/* Mean ms per launch of a plain device copy of `bytes` (multiple of 16):
+ * the measured copy bandwidth and the calibration kernel for HBM counters. */
+double QOPX_copy_time(unsigned long long bytes, int nrep);
 
 /* Library status: 1 when a HIP device is usable. */
 int QOPX_device_ok(void);

@@ -150,6 +150,7 @@ SIGNATURES = [
                                               C.POINTER(C.c_double), _PP, _PP, C.c_int, C.c_int, C.c_int,
                                               C.POINTER(C.c_double)]),
     ("QOPX_D3_asqtad_hop_time", C.c_double, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    ("QOPX_copy_time", C.c_double, [C.c_ulonglong, C.c_int]),
     ("QOPX_device_ok", C.c_int, []),
     ("QOPX_last_error", C.c_char_p, []),
 ]

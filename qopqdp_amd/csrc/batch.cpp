@@ -2,6 +2,7 @@
 #include "batch.h"
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -34,14 +35,20 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
   QOP_CHECK_HIP(hipEventCreate(&e1));
   HopShape best;
   float best_ms = 1e30f;
-  const int nbs[] = {1, 2, 4, 8};
+  const int nbs[] = {1, 2, 4, 8, 16};
   const int blocks[] = {64, 128, 256};
-  const int reps = lat->Vh > 200000 ? 3 : 10;
+  const int reps = lat->Vh > 200000 ? 4 : 10;
   for (int nb : nbs) {
     if (nb > 1 && nb > nrhs) continue;
+    if (nb == 16 && (prec == P64 || L.nl != 16)) continue;
     for (int blk : blocks)
-      for (int rm = 0; rm < 2; rm++) {
+      for (int rm = 0; rm < 2; rm++)
+      for (int nt = 0; nt < 2; nt++)
+      for (int ri = 0; ri < 2; ri++) {
+        if (ri && nrhs <= nb) continue;
         a.remap = rm;
+        a.nt = nt;
+        a.rhs_inner = ri;
         launch_hop_args(a, prec, ls.nl, nb, blk, RED_NONE, false, st);
         QOP_CHECK_HIP(hipEventRecord(e0, st));
         for (int r = 0; r < reps; r++) launch_hop_args(a, prec, ls.nl, nb, blk, RED_NONE, false, st);
@@ -49,14 +56,14 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
         QOP_CHECK_HIP(hipEventSynchronize(e1));
         float ms = 0;
         QOP_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best_ms) { best_ms = ms; best = HopShape{nb, blk, rm}; }
+        if (ms < best_ms) { best_ms = ms; best = HopShape{nb, blk, rm, nt, ri}; }
       }
   }
   QOP_CHECK_HIP(hipEventDestroy(e0));
   QOP_CHECK_HIP(hipEventDestroy(e1));
   if (options().verbose)
-    printf("TUNE: prec %d nrhs %d -> nb %d block %d remap %d (%.3f ms/hop)\n", prec, nrhs, best.nb,
-           best.block, best.remap, best_ms / reps);
+    printf("TUNE: prec %d nrhs %d -> nb %d block %d remap %d nt %d inner %d (%.3f ms/hop)\n", prec,
+           nrhs, best.nb, best.block, best.remap, best.nt, best.rhs_inner, best_ms / reps);
   cache[key] = best;
   return best;
 }
@@ -66,7 +73,7 @@ BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double
     : L_(L), lat_(L.lat), eo_(eo), cap_(capacity), sp_(sloppy_prec), delta_(delta) {
   QOP_REQUIRE(eo == 0 || eo == 1, "batch solver: subset must be even or odd");
   QOP_REQUIRE(capacity >= 1 && capacity <= 256, "batch solver: capacity 1..256");
-  QOP_REQUIRE(sp_ == P32 || sp_ == P16 || sp_ == P64, "batch solver: bad precision");
+  QOP_REQUIRE(sp_ == P32 || sp_ == P16, "batch solver: sloppy precision must be FP32 or FP16");
   nch_ = (int)((lat_->Vh + 63) / 64);
   devbatch_alloc(x64_, lat_, P64, cap_);
   devbatch_alloc(r64_, lat_, P64, cap_);
@@ -75,7 +82,6 @@ BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double
   devbatch_alloc(t64_, lat_, P64, cap_);
   devbatch_alloc(rs_, lat_, sp_, cap_);
   devbatch_alloc(ps_, lat_, sp_, cap_);
-  devbatch_alloc(aps_, lat_, sp_, cap_);
   devbatch_alloc(ts_, lat_, sp_, cap_);
   devbatch_alloc(xs_, lat_, P32, cap_);
   QOP_CHECK_HIP(hipMalloc(&rhs_, cap_ * sizeof(BsRhs)));
@@ -84,6 +90,9 @@ BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double
   sel_ = active_ + cap_;
   QOP_CHECK_HIP(hipHostMalloc(&active_h_, 2 * cap_ * sizeof(int), hipHostMallocDefault));
   sel_h_ = active_h_ + cap_;
+  QOP_CHECK_HIP(hipMalloc(&ru_, 2 * cap_ * sizeof(int)));
+  ru_cont_ = ru_ + cap_;
+  QOP_CHECK_HIP(hipMemsetAsync(ru_, 0, 2 * cap_ * sizeof(int), stream()));
   QOP_CHECK_HIP(hipMalloc(&glob_, sizeof(BsGlobal)));
   QOP_CHECK_HIP(hipHostMalloc(&glob_h_, sizeof(BsGlobal), hipHostMallocDefault));
   QOP_CHECK_HIP(hipMalloc(&partA_, 3 * (size_t)cap_ * nch_ * sizeof(double)));
@@ -96,7 +105,7 @@ BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double
   memset(rhs_h_, 0, cap_ * sizeof(BsRhs));
   memset(active_h_, 0, 2 * cap_ * sizeof(int));
   glob_h_->status = 0;
-  glob_h_->pending = 0;
+  glob_h_->pending = 0;  // the status word stays 0: slots never pause the batch
   hipStream_t st = stream();
   QOP_CHECK_HIP(hipMemcpyAsync(rhs_, rhs_h_, cap_ * sizeof(BsRhs), hipMemcpyHostToDevice, st));
   QOP_CHECK_HIP(hipMemcpyAsync(active_, active_h_, 2 * cap_ * sizeof(int), hipMemcpyHostToDevice, st));
@@ -107,16 +116,19 @@ BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double
   nb_ = hs.nb;
   block_ = hs.block;
   remap_ = hs.remap;
+  nt_ = hs.nt;
+  inner_ = hs.rhs_inner;
   sync();
 }
 
 BatchSolver::~BatchSolver() {
   sync();
   if (graph_) (void)hipGraphExecDestroy(graph_);
-  for (DevBatch *b : {&x64_, &r64_, &b64_, &ax64_, &t64_, &rs_, &ps_, &aps_, &ts_, &xs_}) devbatch_free(*b);
+  for (DevBatch *b : {&x64_, &r64_, &b64_, &ax64_, &t64_, &rs_, &ps_, &ts_, &xs_}) devbatch_free(*b);
   (void)hipFree(rhs_);
   (void)hipHostFree(rhs_h_);
   (void)hipFree(active_);
+  (void)hipFree(ru_);
   (void)hipHostFree(active_h_);
   (void)hipFree(glob_);
   (void)hipHostFree(glob_h_);
@@ -161,8 +173,7 @@ static void batch_lincomb(Lattice *lat, int cap, const DevBatch &out, int nin, c
   launch_lincomb(a, out.prec, pin, nin, partial != nullptr, st);
 }
 
-void BatchSolver::apply_A64(const std::vector<int> &slots) {
-  (void)slots;  // sel_ already uploaded
+void BatchSolver::apply_A64(const int *mask) {
   hipStream_t st = stream();
   LinkSet &ls = L_.get(P64);
   const int opp = 1 - eo_;
@@ -176,8 +187,11 @@ void BatchSolver::apply_A64(const std::vector<int> &slots) {
   a.sstride = a.dstride = a.xstride = 3 * lat_->Vh;
   a.nstride = lat_->Vh;
   a.n = cap_;
-  a.active = sel_;
-  launch_hop_args(a, P64, ls.nl, std::min(nb_, 4), 256, RED_NONE, false, st);
+  a.active = mask;
+  // four slots per thread: slots that start together flag their updates in
+  // the same iteration, so a group shares one read of the FP64 links; a group
+  // with no flagged slot reads nothing
+  launch_hop_args(a, P64, ls.nl, 4, 256, RED_NONE, false, st);
   HopArgs b = a;
   b.parity = eo_;
   b.links = ls.parity_data(lat_, eo_);
@@ -186,7 +200,7 @@ void BatchSolver::apply_A64(const std::vector<int> &slots) {
   b.dst = ax64_.slot(0);
   b.x = x64_.slot(0);
   b.coef = coef_;
-  launch_hop_args(b, P64, ls.nl, std::min(nb_, 4), 256, RED_NONE, true, st);
+  launch_hop_args(b, P64, ls.nl, 4, 256, RED_NONE, true, st);
 }
 
 void BatchSolver::load(int slot, const DevVec &src, const DevVec *x0, double mass, double tol,
@@ -215,8 +229,7 @@ void BatchSolver::load(int slot, const DevVec &src, const DevVec *x0, double mas
   for (int i = 0; i < cap_; i++) sel_h_[i] = i == slot;
   QOP_CHECK_HIP(hipMemcpyAsync(sel_, sel_h_, cap_ * sizeof(int), hipMemcpyHostToDevice, st));
   if (x0) {
-    std::vector<int> s{slot};
-    apply_A64(s);
+    apply_A64(sel_);
     const DevBatch *in[2] = {&b64_, &ax64_};
     double cr[2] = {1.0, -1.0};
     batch_lincomb(lat_, cap_, r64_, 2, in, cr, nullptr, nullptr, sel_, partC_, st);
@@ -246,6 +259,16 @@ void BatchSolver::load(int slot, const DevVec &src, const DevVec *x0, double mas
     batch_lincomb(lat_, cap_, rs_, 1, in, cr, nullptr, nullptr, sel_, nullptr, st);
     const DevBatch *in2[1] = {&rs_};
     batch_lincomb(lat_, cap_, ps_, 1, in2, cr, nullptr, nullptr, sel_, nullptr, st);
+    // |ps|^2 partials of this slot, consumed by the first iteration's p.Ap
+    DotArgs d;
+    d.Vh = (int)Vh;
+    d.n = cap_;
+    d.u = ps_.slot(0);
+    d.ustride = 3 * Vh;
+    d.nstride = Vh;
+    d.partial = partB_;
+    d.active = sel_;
+    launch_dot(d, sp_, sp_, 0, st);
     QOP_CHECK_HIP(hipMemsetAsync(xs_.slot(slot).data, 0, 3 * Vh * sizeof(float2), st));
   }
   QOP_CHECK_HIP(hipMemcpyAsync(sums_h_, sums_, 2 * cap_ * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -288,23 +311,38 @@ void BatchSolver::enqueue_iteration() {
   a.status = &glob_->status;
   a.partial = partA_;
   a.remap = remap_;
+  a.nt = nt_;
+  a.rhs_inner = inner_;
   launch_hop_args(a, sp_, ls.nl, nb_, block_, RED_OUT2, false, st);
+  // second hop with the CG update fused: r -= alpha (m^2 p - D t), x += alpha p
   HopArgs b = a;
   b.parity = eo_;
   b.links = ls.parity_data(lat_, eo_);
   b.lscale = ls.parity_scale(lat_, eo_);
   b.src = ts_.slot(0);
-  b.dst = aps_.slot(0);
+  b.dst = rs_.slot(0);
   b.x = ps_.slot(0);
+  b.xs = xs_.slot(0);
   b.coef = coef_;
-  b.partial = partB_;
-  launch_hop_args(b, sp_, ls.nl, nb_, block_, RED_X2, true, st);
-  bs_launch_scalar1(rhs_, active_, &glob_->status, partA_, partB_, nch_, cap_, st);
-  bs_launch_update(rhs_, active_, &glob_->status, sp_, (int)Vh, rs_.slot(0), aps_.slot(0), ps_.slot(0),
-                   xs_.slot(0), 3 * Vh, partC_, cap_, st);
-  bs_launch_scalar2(rhs_, active_, glob_, partC_, nch_, cap_, st);
-  bs_launch_pupdate(rhs_, active_, &glob_->status, sp_, (int)Vh, rs_.slot(0), ps_.slot(0), 3 * Vh, cap_, 1, st);
-  bs_launch_commit(glob_, st);
+  b.cg_alpha = (const double *)((const char *)rhs_ + offsetof(BsRhs, alpha));
+  b.cg_stride = (int)(sizeof(BsRhs) / sizeof(double));
+  b.partial = partC_;
+  bs_launch_scalar1(rhs_, active_, &glob_->status, partA_, partB_, nch_, cap_, ru_cont_, st);
+  launch_hop_args(b, sp_, ls.nl, nb_, block_, RED_CG, true, st);
+  bs_launch_scalar2(rhs_, active_, glob_, partC_, nch_, cap_, ru_, st);
+  bs_launch_pupdate(rhs_, active_, &glob_->status, sp_, (int)Vh, rs_.slot(0), ps_.slot(0), 3 * Vh, cap_, 1,
+                    partB_, st);
+  // reliable update of the flagged slots, all on the device
+  const long stride = 3 * Vh;
+  bs_launch_ru(rhs_, active_, ru_, ru_cont_, lcoef_, sp_, (int)Vh, x64_.slot(0), xs_.slot(0), b64_.slot(0),
+               ax64_.slot(0), rs_.slot(0), ps_.slot(0), stride, partA_, partB_, partC_, nch_, cap_, 0, partB_, st);
+  apply_A64(ru_);
+  bs_launch_ru(rhs_, active_, ru_, ru_cont_, lcoef_, sp_, (int)Vh, x64_.slot(0), xs_.slot(0), b64_.slot(0),
+               ax64_.slot(0), rs_.slot(0), ps_.slot(0), stride, partA_, partB_, partC_, nch_, cap_, 1, partB_, st);
+  bs_launch_ru(rhs_, active_, ru_, ru_cont_, lcoef_, sp_, (int)Vh, x64_.slot(0), xs_.slot(0), b64_.slot(0),
+               ax64_.slot(0), rs_.slot(0), ps_.slot(0), stride, partA_, partB_, partC_, nch_, cap_, 2, partB_, st);
+  bs_launch_ru(rhs_, active_, ru_, ru_cont_, lcoef_, sp_, (int)Vh, x64_.slot(0), xs_.slot(0), b64_.slot(0),
+               ax64_.slot(0), rs_.slot(0), ps_.slot(0), stride, partA_, partB_, partC_, nch_, cap_, 3, partB_, st);
 }
 
 void BatchSolver::build_graph() {
@@ -321,105 +359,6 @@ void BatchSolver::build_graph() {
   graph_chunk_ = chunk;
 }
 
-void BatchSolver::reliable_update(const std::vector<int> &slots) {
-  hipStream_t st = stream();
-  const long Vh = lat_->Vh;
-  for (int i = 0; i < cap_; i++) sel_h_[i] = 0;
-  for (int s : slots) sel_h_[s] = 1;
-  QOP_CHECK_HIP(hipMemcpyAsync(sel_, sel_h_, cap_ * sizeof(int), hipMemcpyHostToDevice, st));
-  // x += xs
-  {
-    const DevBatch *in[2] = {&x64_, &xs_};
-    double cr[2] = {1.0, 1.0};
-    batch_lincomb(lat_, cap_, x64_, 2, in, cr, nullptr, nullptr, sel_, nullptr, st);
-  }
-  // r = b - A x, |r|^2
-  apply_A64(slots);
-  {
-    const DevBatch *in[2] = {&b64_, &ax64_};
-    double cr[2] = {1.0, -1.0};
-    batch_lincomb(lat_, cap_, r64_, 2, in, cr, nullptr, nullptr, sel_, partC_, st);
-    bs_launch_sum(partC_, nch_, cap_, sel_, sums_, st);
-  }
-  // rs = r ; xs = 0
-  {
-    const DevBatch *in[1] = {&r64_};
-    double cr[1] = {1.0};
-    batch_lincomb(lat_, cap_, rs_, 1, in, cr, nullptr, nullptr, sel_, nullptr, st);
-    for (int s : slots) QOP_CHECK_HIP(hipMemsetAsync(xs_.slot(s).data, 0, 3 * Vh * sizeof(float2), st));
-  }
-  // <rs, ps> for the re-orthogonalisation
-  {
-    DotArgs d;
-    d.Vh = (int)Vh;
-    d.n = cap_;
-    d.u = rs_.slot(0);
-    d.v = ps_.slot(0);
-    d.ustride = d.vstride = 3 * Vh;
-    d.nstride = Vh;
-    d.partial = partA_;  // [2*cap][nch]
-    d.active = sel_;
-    launch_dot(d, sp_, sp_, 1, st);
-  }
-  // partA_ has room for cap*nch doubles per buffer; the complex dot writes 2*cap
-  // rows, i.e. partA_ and partB_ together.
-  std::vector<int> sel2(2 * cap_, 0);
-  for (int s : slots) { sel2[2 * s] = 1; sel2[2 * s + 1] = 1; }
-  QOP_CHECK_HIP(hipMemcpyAsync(rhs_h_, rhs_, cap_ * sizeof(BsRhs), hipMemcpyDeviceToHost, st));
-  QOP_CHECK_HIP(hipMemcpyAsync(sums_h_, sums_, cap_ * sizeof(double), hipMemcpyDeviceToHost, st));
-  sync();
-  std::vector<double> rp_re(cap_, 0), rp_im(cap_, 0);
-  {
-    // reduce the dot partials for selected slots (2 rows per slot)
-    double *dsum;
-    QOP_CHECK_HIP(hipMalloc(&dsum, 2 * cap_ * sizeof(double)));
-    int *dsel;
-    QOP_CHECK_HIP(hipMalloc(&dsel, 2 * cap_ * sizeof(int)));
-    QOP_CHECK_HIP(hipMemcpyAsync(dsel, sel2.data(), 2 * cap_ * sizeof(int), hipMemcpyHostToDevice, st));
-    bs_launch_sum(partA_, nch_, 2 * cap_, dsel, dsum, st);
-    std::vector<double> h(2 * cap_);
-    QOP_CHECK_HIP(hipMemcpyAsync(h.data(), dsum, 2 * cap_ * sizeof(double), hipMemcpyDeviceToHost, st));
-    sync();
-    for (int s : slots) { rp_re[s] = h[2 * s]; rp_im[s] = h[2 * s + 1]; }
-    QOP_CHECK_HIP(hipFree(dsum));
-    QOP_CHECK_HIP(hipFree(dsel));
-  }
-  // decide per slot; continuing slots get ps = (1 - beta rp) rs + beta ps
-  std::vector<double> lc(6 * cap_, 0.0);
-  for (int i = 0; i < cap_; i++) sel_h_[i] = 0;
-  bool any = false;
-  for (int s : slots) {
-    BsRhs &R = rhs_h_[s];
-    double r2t = sums_h_[s];
-    R.nupdate++;
-    R.need_update = 0;
-    if (r2t <= R.stop2) { R.reason = 1; active_h_[s] = 0; R.r2 = r2t; R.r2_rel = r2t; continue; }
-    if (R.iter >= R.max_iter) { R.reason = 2; active_h_[s] = 0; R.r2 = r2t; R.r2_rel = r2t; continue; }
-    if (!(R.pAp > 0)) { R.reason = 3; active_h_[s] = 0; R.r2 = r2t; R.r2_rel = r2t; continue; }
-    double rpr = rp_re[s] / r2t, rpi = rp_im[s] / r2t;
-    double beta = r2t / R.r2_old;
-    R.beta = beta;
-    R.r2 = r2t;
-    R.r2_rel = r2t;
-    // ps <- rs + beta (ps - rp rs) = (1 - beta rp) rs + beta ps
-    lc[6 * s + 0] = 1.0 - beta * rpr;
-    lc[6 * s + 1] = -beta * rpi;
-    lc[6 * s + 2] = beta;
-    lc[6 * s + 3] = 0.0;
-    sel_h_[s] = 1;
-    any = true;
-  }
-  QOP_CHECK_HIP(hipMemcpyAsync(rhs_, rhs_h_, cap_ * sizeof(BsRhs), hipMemcpyHostToDevice, st));
-  QOP_CHECK_HIP(hipMemcpyAsync(active_, active_h_, cap_ * sizeof(int), hipMemcpyHostToDevice, st));
-  if (any) {
-    QOP_CHECK_HIP(hipMemcpyAsync(lcoef_, lc.data(), 6 * cap_ * sizeof(double), hipMemcpyHostToDevice, st));
-    QOP_CHECK_HIP(hipMemcpyAsync(sel_, sel_h_, cap_ * sizeof(int), hipMemcpyHostToDevice, st));
-    const DevBatch *in[2] = {&rs_, &ps_};
-    batch_lincomb(lat_, cap_, ps_, 2, in, nullptr, nullptr, lcoef_, sel_, nullptr, st);
-  }
-  sync();
-}
-
 void BatchSolver::step(std::vector<int> &done) {
   hipStream_t st = stream();
   bool any = false;
@@ -432,22 +371,12 @@ void BatchSolver::step(std::vector<int> &done) {
       const int chunk = options().chunk > 0 ? options().chunk : 16;
       for (int c = 0; c < chunk; c++) enqueue_iteration();
     }
-    QOP_CHECK_HIP(hipMemcpyAsync(glob_h_, glob_, sizeof(BsGlobal), hipMemcpyDeviceToHost, st));
+    // slots finish on the device (bs_ru_scalar clears active[b])
+    QOP_CHECK_HIP(hipMemcpyAsync(active_h_, active_, cap_ * sizeof(int), hipMemcpyDeviceToHost, st));
     sync();
-    if (glob_h_->status != 0) {
-      QOP_CHECK_HIP(hipMemcpyAsync(rhs_h_, rhs_, cap_ * sizeof(BsRhs), hipMemcpyDeviceToHost, st));
-      sync();
-      std::vector<int> flagged;
-      for (int i = 0; i < cap_; i++)
-        if (active_h_[i] && rhs_h_[i].need_update) flagged.push_back(i);
-      if (!flagged.empty()) reliable_update(flagged);
-      glob_h_->status = 0;
-      glob_h_->pending = 0;
-      QOP_CHECK_HIP(hipMemcpyAsync(glob_, glob_h_, sizeof(BsGlobal), hipMemcpyHostToDevice, st));
-    }
   }
   for (int i = 0; i < cap_; i++)
-    if (loaded_[i] && !active_h_[i] && loaded_[i] == 1) {
+    if (loaded_[i] == 1 && !active_h_[i]) {
       loaded_[i] = 2;  // finished, not yet released
       done.push_back(i);
     }
@@ -495,7 +424,7 @@ void BatchSolver::release(int slot) {
 void invert_dispatch(Links &L, InvArg *ia, ResArg *ra, double mass, DevVec &out, const DevVec &in,
                      double *flop, double *sec) {
   const Options &o = options();
-  if (o.solver == 0 || ia->evenodd == 2) {
+  if (o.solver == 0 || ia->evenodd == 2 || o.inner_prec == P64) {
     invert(L, ia, ra, mass, out, in, flop, sec);
     return;
   }
@@ -519,7 +448,7 @@ void invert_dispatch(Links &L, InvArg *ia, ResArg *ra, double mass, DevVec &out,
 void invert_multi_dispatch(Links &L, InvArg *ia, ResArg **ra, const double *masses, const int *nmass,
                            DevVec **out, DevVec *const *in, int nsrc, double *flop, double *sec) {
   const Options &o = options();
-  if (o.solver == 0 || ia->evenodd == 2) {
+  if (o.solver == 0 || ia->evenodd == 2 || o.inner_prec == P64) {
     invert_multi(L, ia, ra, masses, nmass, out, in, nsrc, flop, sec);
     return;
   }

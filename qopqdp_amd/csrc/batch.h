@@ -44,16 +44,22 @@ struct BsGlobal {
 // Kernels (batch.hip)
 void bs_launch_scalar1(BsRhs *rhs, const int *active, const int *status,
                        const double *partA, const double *partB, int nch, int n,
-                       hipStream_t st);
+                       int *ru_cont, hipStream_t st);
 void bs_launch_update(BsRhs *rhs, const int *active, const int *status, int prec,
                       int Vh, VecView rs, VecView Aps, VecView ps, VecView xs,
                       long stride, double *partC, int n, hipStream_t st);
 void bs_launch_scalar2(BsRhs *rhs, const int *active, BsGlobal *glob,
-                       const double *partC, int nch, int n, hipStream_t st);
+                       const double *partC, int nch, int n, int *ru, hipStream_t st);
+// Device reliable update, stage 0: x += xs; 1: residual, |r|^2 and <rs,ps>;
+// 2: per-slot decisions; 3: search-direction fix.
+void bs_launch_ru(BsRhs *rhs, int *active, int *ru, int *ru_cont, double *lcoef, int prec,
+                  int Vh, VecView x64, VecView xs, VecView b64, VecView ax64, VecView rs,
+                  VecView ps, long stride, double *pr2, double *pre, double *pim, int nch,
+                  int n, int stage, double *ppB, hipStream_t st);
 void bs_launch_commit(BsGlobal *glob, hipStream_t st);
 void bs_launch_pupdate(const BsRhs *rhs, const int *active, const int *status,
                        int prec, int Vh, VecView rs, VecView ps, long stride, int n,
-                       int skip_flagged, hipStream_t st);
+                       int skip_flagged, double *partB, hipStream_t st);
 void bs_launch_sum(const double *part, int nch, int n, const int *sel,
                    double *out, hipStream_t st);
 
@@ -87,10 +93,9 @@ class BatchSolver {
   double mass(int slot) const { return mass_[slot]; }
 
  private:
-  void reliable_update(const std::vector<int> &slots);
   void enqueue_iteration();
   void build_graph();
-  void apply_A64(const std::vector<int> &slots);  // ax64 = A x64 on selected
+  void apply_A64(const int *mask);  // ax64 = A x64 on the slots set in mask
 
   Links &L_;
   Lattice *lat_;
@@ -100,10 +105,11 @@ class BatchSolver {
   // FP64 per slot
   DevBatch x64_, r64_, b64_, ax64_, t64_;
   // sloppy
-  DevBatch rs_, ps_, aps_, ts_, xs_;
+  DevBatch rs_, ps_, ts_, xs_;
   BsRhs *rhs_ = nullptr, *rhs_h_ = nullptr;
   int *active_ = nullptr, *active_h_ = nullptr;
   int *sel_ = nullptr, *sel_h_ = nullptr;
+  int *ru_ = nullptr, *ru_cont_ = nullptr;  // device reliable-update masks
   BsGlobal *glob_ = nullptr, *glob_h_ = nullptr;
   double *partA_ = nullptr, *partB_ = nullptr, *partC_ = nullptr;
   double *coef_ = nullptr;      // [cap][2] hop coefficients (-1, m2)
@@ -113,12 +119,12 @@ class BatchSolver {
   std::vector<double> mass_;
   hipGraphExec_t graph_ = nullptr;
   int graph_chunk_ = 0;
-  int nb_ = 4, block_ = 256, remap_ = 1;  // tuned hop launch shape
+  int nb_ = 4, block_ = 256, remap_ = 1, nt_ = 0, inner_ = 0;  // tuned hop launch shape
 };
 
 // Tuned hop launch shape for (lattice, precision, nl, nrhs); see tune.cpp.
 struct HopShape {
-  int nb = 1, block = 256, remap = 1;
+  int nb = 1, block = 256, remap = 1, nt = 0, rhs_inner = 0;
 };
 HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &dst,
                   int eo_out);

@@ -5,7 +5,9 @@
 namespace qop {
 
 // Hop reductions per RHS (FP64): RED_OUT2 = |out|^2, RED_X2 = |x|^2.
-enum { RED_NONE = 0, RED_OUT2 = 1, RED_X2 = 2 };
+// RED_CG: fused CG update in the epilogue (dst = r is updated in place with
+// r -= alpha (beta x + alpha_hop hop), xs += alpha x, reduction |r|^2).
+enum { RED_NONE = 0, RED_OUT2 = 1, RED_X2 = 2, RED_CG = 3 };
 int hop_partial_blocks(const Lattice *lat);
 
 // Hop: dst[b] = alpha_b * sum_k W_k src[b](x+d_k) + beta_b * x[b], for the
@@ -25,6 +27,12 @@ struct HopArgs {
   const int *active = nullptr;   // device [n] or null
   const int *status = nullptr;   // device flag: kernel is a no-op when nonzero
   int remap = 1;
+  int nt = 0;                    // non-temporal link loads
+  int rhs_inner = 0;             // RHS groups innermost in the block order
+  VecView xs{};                  // RED_CG: FP32 solution increments (stride dstride)
+  const double *cg_alpha = nullptr;  // RED_CG: alpha of slot b at cg_alpha[b*cg_stride]
+  int cg_stride = 0;
+  int ngroups = 1;               // set by launch_hop_args
 };
 void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
                      bool hasx, hipStream_t st);
@@ -59,5 +67,8 @@ void launch_dot(const DotArgs &a, int pu, int pv, int mode, hipStream_t st);
 
 void launch_finalize(const double *partial, int m, int n, double *result,
                      hipStream_t st, const int *status);
+
+// streaming copy of `bytes` (multiple of 16)
+void launch_copy(const void *in, void *out, size_t bytes, hipStream_t st);
 
 }  // namespace qop

@@ -27,17 +27,27 @@ template <> struct PT<P64> { using R = double; using S = double2; };
 template <> struct PT<P32> { using R = float;  using S = float2; };
 template <> struct PT<P16> { using R = float;  using S = __half2; };
 
-template <int P>
+// Link element load.  NT: non-temporal (streaming) load, so the links, read
+// once per hop, do not evict the gathered source vectors from L2 / MALL.
+template <int P, bool NT>
 __device__ __forceinline__ void ld_c(const void *base, long i, float scl,
                                      typename PT<P>::R &re, typename PT<P>::R &im) {
   if constexpr (P == P64) {
-    double2 v = ((const double2 *)base)[i];
-    re = v.x; im = v.y;
+    const double *p = (const double *)base + 2 * i;
+    if constexpr (NT) { re = __builtin_nontemporal_load(p); im = __builtin_nontemporal_load(p + 1); }
+    else { double2 v = ((const double2 *)base)[i]; re = v.x; im = v.y; }
   } else if constexpr (P == P32) {
-    float2 v = ((const float2 *)base)[i];
-    re = v.x; im = v.y;
+    if constexpr (NT) {
+      const double *p = (const double *)base + i;  // one 8-byte load of the float2
+      double d = __builtin_nontemporal_load(p);
+      float2 v = __builtin_bit_cast(float2, d);
+      re = v.x; im = v.y;
+    } else { float2 v = ((const float2 *)base)[i]; re = v.x; im = v.y; }
   } else {
-    __half2 v = ((const __half2 *)base)[i];
+    unsigned u;
+    if constexpr (NT) u = __builtin_nontemporal_load((const unsigned *)base + i);
+    else u = ((const unsigned *)base)[i];
+    __half2 v = __builtin_bit_cast(__half2, u);
     float2 f = __half22float2(v);
     re = f.x * scl; im = f.y * scl;
   }
@@ -128,25 +138,33 @@ __device__ __forceinline__ int logical_block(int remap) {
 
 // ---------------------------------------------------------------------------
 // hop kernel
-template <int P, int NL, int NB, int RED, bool HASX>
+template <int P, int NL, int NB, int RED, bool HASX, bool NT>
 __global__ void __launch_bounds__(256)
 hop_kernel(const HopArgs a) {
   using R = typename PT<P>::R;
   const Geom g = a.g;
   const int Vh = g.Vh;
   const int lb = logical_block(a.remap);
-  const int h = lb * blockDim.x + threadIdx.x;
-  const int b0 = blockIdx.y * NB;
+  // rhs_inner: the RHS groups of one site block are consecutive logical
+  // blocks, so they run together on one XCD and share its L2 (links and the
+  // neighbour window); otherwise the groups are the grid's y dimension.
+  const int h = (a.rhs_inner ? lb / a.ngroups : lb) * blockDim.x + threadIdx.x;
+  const int b0 = (a.rhs_inner ? lb % a.ngroups : (int)blockIdx.y) * NB;
   if (a.status && *a.status != 0) return;
 
   double red[NB];
 #pragma unroll
   for (int b = 0; b < NB; b++) red[b] = 0.0;
 
-  if (h < Vh) {
-    bool act[NB];
+  bool act[NB];
+  bool any = false;
 #pragma unroll
-    for (int b = 0; b < NB; b++) act[b] = (b0 + b < a.n) && (!a.active || a.active[b0 + b]);
+  for (int b = 0; b < NB; b++) {
+    act[b] = (b0 + b < a.n) && (!a.active || a.active[b0 + b]);
+    any |= act[b];
+  }
+  // a group with no active slot reads nothing (masked launches are cheap)
+  if (h < Vh && any) {
     int x[4], lex;
     site_coords(g, h, a.parity, x, lex);
     R acc[NB][6];
@@ -166,10 +184,13 @@ hop_kernel(const HopArgs a) {
         const int nb = nbr(g, x, lex, mu, d);
         const long lbase = (long)k * 9 * Vh + h;
         float scl = 1.0f;
-        if constexpr (P == P16) scl = a.lscale[(long)k * Vh + h];
+        if constexpr (P == P16) {
+          if constexpr (NT) scl = __builtin_nontemporal_load(a.lscale + (long)k * Vh + h);
+          else scl = a.lscale[(long)k * Vh + h];
+        }
         R w[18];
 #pragma unroll
-        for (int e = 0; e < 9; e++) ld_c<P>(a.links, lbase + (long)e * Vh, scl, w[2 * e], w[2 * e + 1]);
+        for (int e = 0; e < 9; e++) ld_c<P, NT>(a.links, lbase + (long)e * Vh, scl, w[2 * e], w[2 * e + 1]);
 #pragma unroll
         for (int b = 0; b < NB; b++) {
           if (!act[b]) continue;
@@ -214,6 +235,31 @@ hop_kernel(const HopArgs a) {
       }
       char *dd = (char *)a.dst.data + (long)rb * a.dstride * sizeof(typename PT<P>::S);
       float *dn = a.dst.norm ? a.dst.norm + (long)rb * a.nstride : nullptr;
+      if constexpr (RED == RED_CG) {
+        // fused CG update: o = A p (not stored); r -= alpha o ; x += alpha p
+        // (x is FP32); reduction |r|^2 of the stored r
+        const R alr = (R)a.cg_alpha[(long)rb * a.cg_stride];
+        R rv[6];
+        vload<P>(dd, dn, h, Vh, rv);
+#pragma unroll
+        for (int i = 0; i < 6; i++) rv[i] = fma(-alr, o[i], rv[i]);
+        vstore<P>(dd, dn, h, Vh, rv);
+        if constexpr (P == P16) vload<P>(dd, dn, h, Vh, rv);
+        double s = 0;
+#pragma unroll
+        for (int i = 0; i < 6; i++) s += (double)rv[i] * (double)rv[i];
+        red[b] = s;
+        float2 *xs = (float2 *)a.xs.data + (long)rb * a.dstride;
+        const float alf = (float)a.cg_alpha[(long)rb * a.cg_stride];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          float2 t = xs[c * Vh + h];
+          t.x = fmaf(alf, (float)xv[2 * c], t.x);
+          t.y = fmaf(alf, (float)xv[2 * c + 1], t.y);
+          xs[c * Vh + h] = t;
+        }
+        continue;
+      }
       vstore<P>(dd, dn, h, Vh, o);
       if constexpr (RED == RED_OUT2) {
         double s = 0;
@@ -249,12 +295,13 @@ hop_kernel(const HopArgs a) {
 
 int hop_partial_blocks(const Lattice *lat) { return (int)((lat->Vh + 63) / 64); }
 
-template <int P, int NL, int NB>
+template <int P, int NL, int NB, bool NT>
 static void hop_dispatch_red(const HopArgs &a, dim3 grid, dim3 block, int red,
                              bool hasx, hipStream_t st) {
-#define QOP_LAUNCH(RD, HX) hipLaunchKernelGGL((hop_kernel<P, NL, NB, RD, HX>), grid, block, 0, st, a)
+#define QOP_LAUNCH(RD, HX) hipLaunchKernelGGL((hop_kernel<P, NL, NB, RD, HX, NT>), grid, block, 0, st, a)
   if (red == RED_NONE) { if (hasx) QOP_LAUNCH(RED_NONE, true); else QOP_LAUNCH(RED_NONE, false); }
   else if (red == RED_OUT2) { if (hasx) QOP_LAUNCH(RED_OUT2, true); else QOP_LAUNCH(RED_OUT2, false); }
+  else if (red == RED_CG) { QOP_LAUNCH(RED_CG, true); }
   else { QOP_LAUNCH(RED_X2, true); }
 #undef QOP_LAUNCH
 }
@@ -262,23 +309,49 @@ static void hop_dispatch_red(const HopArgs &a, dim3 grid, dim3 block, int red,
 template <int P, int NL>
 static void hop_dispatch_nb(const HopArgs &a, int nb, dim3 grid, dim3 block,
                             int red, bool hasx, hipStream_t st) {
-  switch (nb) {
-    case 1: hop_dispatch_red<P, NL, 1>(a, grid, block, red, hasx, st); break;
-    case 2: hop_dispatch_red<P, NL, 2>(a, grid, block, red, hasx, st); break;
-    case 4: hop_dispatch_red<P, NL, 4>(a, grid, block, red, hasx, st); break;
-    default: hop_dispatch_red<P, NL, 8>(a, grid, block, red, hasx, st); break;
+  if (a.nt) {
+    switch (nb) {
+      case 1: hop_dispatch_red<P, NL, 1, true>(a, grid, block, red, hasx, st); break;
+      case 2: hop_dispatch_red<P, NL, 2, true>(a, grid, block, red, hasx, st); break;
+      case 4: hop_dispatch_red<P, NL, 4, true>(a, grid, block, red, hasx, st); break;
+      case 8: hop_dispatch_red<P, NL, 8, true>(a, grid, block, red, hasx, st); break;
+      default:
+        if constexpr (P != P64 && NL == 16) hop_dispatch_red<P, NL, 16, true>(a, grid, block, red, hasx, st);
+        else fatal(__FILE__, __LINE__, "hop: 16 RHS per thread needs FP32/FP16 and Naik links");
+        break;
+    }
+  } else {
+    switch (nb) {
+      case 1: hop_dispatch_red<P, NL, 1, false>(a, grid, block, red, hasx, st); break;
+      case 2: hop_dispatch_red<P, NL, 2, false>(a, grid, block, red, hasx, st); break;
+      case 4: hop_dispatch_red<P, NL, 4, false>(a, grid, block, red, hasx, st); break;
+      case 8: hop_dispatch_red<P, NL, 8, false>(a, grid, block, red, hasx, st); break;
+      default:
+        if constexpr (P != P64 && NL == 16) hop_dispatch_red<P, NL, 16, false>(a, grid, block, red, hasx, st);
+        else fatal(__FILE__, __LINE__, "hop: 16 RHS per thread needs FP32/FP16 and Naik links");
+        break;
+    }
   }
 }
 
 void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
                      bool hasx, hipStream_t st) {
-  QOP_REQUIRE(nb == 1 || nb == 2 || nb == 4 || nb == 8, "hop: rhs per thread must be 1,2,4,8");
+  QOP_REQUIRE(nb == 1 || nb == 2 || nb == 4 || nb == 8 || nb == 16, "hop: rhs per thread must be 1,2,4,8,16");
   QOP_REQUIRE(block % 64 == 0 && block >= 64 && block <= 256, "hop: block must be 64..256, multiple of 64");
-  QOP_REQUIRE(red == RED_NONE || hasx || red == RED_OUT2, "hop: RED_X2 needs x");
+  QOP_REQUIRE(red == RED_NONE || hasx || red == RED_OUT2, "hop: RED_X2 / RED_CG need x");
+  QOP_REQUIRE(red != RED_CG || (a.cg_alpha && a.xs.data && prec != P64), "hop: RED_CG needs alpha and xs");
   QOP_REQUIRE(a.n >= 1, "hop: n >= 1");
   int nblk = (a.g.Vh + block - 1) / block;
-  if (a.remap) nblk = (nblk + 7) / 8 * 8;
-  dim3 grid(nblk, (a.n + nb - 1) / nb), blk(block);
+  a.ngroups = (a.n + nb - 1) / nb;
+  dim3 grid, blk(block);
+  if (a.rhs_inner) {
+    int tot = nblk * a.ngroups;
+    if (a.remap) tot = (tot + 7) / 8 * 8;
+    grid = dim3(tot, 1);
+  } else {
+    if (a.remap) nblk = (nblk + 7) / 8 * 8;
+    grid = dim3(nblk, a.ngroups);
+  }
   if (prec == P64) {
     if (nl == 16) hop_dispatch_nb<P64, 16>(a, nb, grid, blk, red, hasx, st);
     else hop_dispatch_nb<P64, 8>(a, nb, grid, blk, red, hasx, st);
@@ -330,9 +403,9 @@ lincomb_kernel(const LincombArgs a) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   if (a.status && *a.status != 0) return;
-  const bool act = !a.active || a.active[b];
+  if (a.active && !a.active[b]) return;  // inactive slot: no output, no partial
   double s = 0;
-  if (h < Vh && act) {
+  if (h < Vh) {
     double cr[3], ci[3];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -427,8 +500,9 @@ dot_kernel(const DotArgs a) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   if (a.status && *a.status != 0) return;
+  if (a.active && !a.active[b]) return;  // inactive slot: partial rows untouched
   double sr = 0, si = 0;
-  if (h < Vh && (!a.active || a.active[b])) {
+  if (h < Vh) {
     typename PT<PU>::R u[6];
     vload<PU>((const char *)a.u.data + (long)b * a.ustride * sizeof(typename PT<PU>::S),
               a.u.norm ? a.u.norm + (long)b * a.nstride : nullptr, h, Vh, u);
@@ -568,5 +642,21 @@ void launch_raw_to_vec(const Lattice *lat, const double *raw, DevVec &v, int eo,
 void launch_vec_to_raw(const Lattice *lat, const DevVec &v, double *raw, int eo, hipStream_t st) { vec_to_raw_t(lat, v, raw, eo, st); }
 void launch_rawf_to_vec(const Lattice *lat, const float *raw, DevVec &v, int eo, hipStream_t st) { raw_to_vec_t(lat, raw, v, eo, st); }
 void launch_vec_to_rawf(const Lattice *lat, const DevVec &v, float *raw, int eo, hipStream_t st) { vec_to_raw_t(lat, v, raw, eo, st); }
+
+// ---------------------------------------------------------------------------
+// plain streaming copy (16 B per lane): the calibration kernel for HBM
+// traffic counters and the measured copy bandwidth of the device
+__global__ void __launch_bounds__(256) copy_kernel(const float4 *__restrict__ in,
+                                                   float4 *__restrict__ out, long n) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) out[i] = in[i];
+}
+
+void launch_copy(const void *in, void *out, size_t bytes, hipStream_t st) {
+  long n = (long)(bytes / 16);
+  hipLaunchKernelGGL(copy_kernel, dim3(8192), dim3(256), 0, st, (const float4 *)in, (float4 *)out, n);
+  QOP_CHECK_HIP(hipGetLastError());
+}
 
 }  // namespace qop

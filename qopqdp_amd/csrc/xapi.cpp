@@ -16,7 +16,8 @@ extern "C" {
 // (64/32/16) on nrhs right-hand sides with the autotuned launch shape.
 // red != 0 selects the instance with the fused |out|^2 reduction (the first
 // hop of a CG iteration).  Returns the mean ms per launch from HIP events on
-// the library stream; shape[3] receives (rhs per thread, block, XCD remap).
+// the library stream; shape[5] receives (rhs per thread, block, XCD remap, non-temporal links,
+// RHS-innermost block order).
 double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, int nrep,
                                int red, int *shape) {
   Links &L = fla->L;
@@ -61,6 +62,8 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
   a.nstride = lat->Vh;
   a.n = nrhs;
   a.remap = hs.remap;
+  a.nt = hs.nt;
+  a.rhs_inner = hs.rhs_inner;
   double *part = nullptr;
   if (red) {
     QOP_CHECK_HIP(hipMalloc(&part, (size_t)nrhs * ((lat->Vh + 63) / 64) * sizeof(double)));
@@ -82,7 +85,9 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
   if (part) QOP_CHECK_HIP(hipFree(part));
   devbatch_free(src);
   devbatch_free(dst);
-  if (shape) { shape[0] = hs.nb; shape[1] = hs.block; shape[2] = hs.remap; }
+  if (shape) {
+    shape[0] = hs.nb; shape[1] = hs.block; shape[2] = hs.remap; shape[3] = hs.nt; shape[4] = hs.rhs_inner;
+  }
   return ms / nrep;
 }
 
@@ -102,7 +107,8 @@ int QOPX_D3_asqtad_invert_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla
               "invert_batch: subset must be EVEN or ODD");
   QOP_REQUIRE(n >= 0 && capacity >= 1, "invert_batch: bad sizes");
   if (n == 0) return 0;
-  const int sp = prec == 16 ? P16 : prec == 64 ? P64 : P32;
+  QOP_REQUIRE(prec == 16 || prec == 32, "invert_batch: sloppy precision must be 16 or 32");
+  const int sp = prec == 16 ? P16 : P32;
   double t0 = wall_time();
   const int cap = std::min(capacity, n);
   BatchSolver bs(L, inv_arg->evenodd, cap, sp, options().delta);
@@ -138,6 +144,33 @@ int QOPX_D3_asqtad_invert_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla
     info->status = QOP_SUCCESS;
   }
   return ok;
+}
+
+// Mean ms per launch of a plain device-to-device copy of `bytes` (read +
+// write), used to calibrate HBM traffic counters and report the measured
+// copy bandwidth next to the nominal peak.
+double QOPX_copy_time(unsigned long long bytes, int nrep) {
+  QOP_REQUIRE(bytes >= 16 && bytes % 16 == 0 && nrep >= 1, "copy_time: bytes must be a multiple of 16");
+  void *a, *b;
+  QOP_CHECK_HIP(hipMalloc(&a, bytes));
+  QOP_CHECK_HIP(hipMalloc(&b, bytes));
+  QOP_CHECK_HIP(hipMemsetAsync(a, 0, bytes, stream()));
+  hipStream_t st = stream();
+  for (int w = 0; w < 2; w++) launch_copy(a, b, bytes, st);
+  hipEvent_t e0, e1;
+  QOP_CHECK_HIP(hipEventCreate(&e0));
+  QOP_CHECK_HIP(hipEventCreate(&e1));
+  QOP_CHECK_HIP(hipEventRecord(e0, st));
+  for (int r = 0; r < nrep; r++) launch_copy(a, b, bytes, st);
+  QOP_CHECK_HIP(hipEventRecord(e1, st));
+  QOP_CHECK_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  QOP_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+  QOP_CHECK_HIP(hipEventDestroy(e0));
+  QOP_CHECK_HIP(hipEventDestroy(e1));
+  QOP_CHECK_HIP(hipFree(a));
+  QOP_CHECK_HIP(hipFree(b));
+  return ms / nrep;
 }
 
 }  // extern "C"
