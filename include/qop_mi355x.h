@@ -131,58 +131,58 @@ typedef struct QOP_D3_FermionLinksAsqtad_struct QOP_D3_FermionLinksAsqtad;
 typedef struct QOP_F3_FermionLinksAsqtad_struct QOP_F3_FermionLinksAsqtad;
 
 /* ---- general routines (include/qop_int.h:230-251; lib/common.c) ------ */
-QOP_status_t QOP_init(QOP_layout_t *layout);         /* lib/common.c:39-83 */
-QOP_status_t QOP_finalize(void);                     /* lib/common.c:85-94 */
-int QOP_is_initialized(void);                        /* lib/common.c:96-100 */
-const char *QOP_version_str(void);                   /* lib/common.c:104-108 */
-int QOP_version_int(void);                           /* lib/common.c:110-116 */
-int QOP_verbose(int level);                          /* lib/common.c:118-124 */
-int QOP_profcontrol(int level);                      /* lib/common.c:126-132 */
-int QOP_node_number_raw(int coords[]);               /* lib/common.c:134-139 */
-int QOP_node_index_raw_V(int coords[], QOP_evenodd_t evenodd); /* lib/common.c:141-146 */
-int QOP_node_index_raw_G(int coords[], QOP_evenodd_t evenodd); /* lib/common.c:155-160 */
-int QOP_sites_on_node_raw_V(QOP_evenodd_t evenodd);  /* lib/common.c:169-173 */
-int QOP_sites_on_node_raw_G(QOP_evenodd_t evenodd);  /* lib/common.c:181-185 */
+QOP_status_t QOP_init(QOP_layout_t *layout);         /* lib/common.c:40-83 */
+QOP_status_t QOP_finalize(void);                     /* lib/common.c:86-94 */
+int QOP_is_initialized(void);                        /* lib/common.c:97-100 */
+const char *QOP_version_str(void);                   /* lib/common.c:105-108 */
+int QOP_version_int(void);                           /* lib/common.c:111-116 */
+int QOP_verbose(int level);                          /* lib/common.c:119-124 */
+int QOP_profcontrol(int level);                      /* lib/common.c:127-132 */
+int QOP_node_number_raw(int coords[]);               /* lib/common.c:135-139 */
+int QOP_node_index_raw_V(int coords[], QOP_evenodd_t evenodd); /* lib/common.c:142-146 */
+int QOP_node_index_raw_G(int coords[], QOP_evenodd_t evenodd); /* lib/common.c:156-160 */
+int QOP_sites_on_node_raw_V(QOP_evenodd_t evenodd);  /* lib/common.c:170-173 */
+int QOP_sites_on_node_raw_G(QOP_evenodd_t evenodd);  /* lib/common.c:182-185 */
 /* option tags st, ns, nm, cg, eigcg_* as lib/asqtad_invert.c:68-112; tags
  * starting "qmx_" select this library's solvers (see QOPX options). */
 QOP_status_t QOP_asqtad_invert_set_opts(QOP_opt_t opts[], int nopts);
 
 /* ---- fields (include/qop_pc.h:49-121; lib/common_p.c) ---------------- */
-QOP_D3_ColorVector *QOP_D3_create_V_from_raw(QOP_D_Real *src, QOP_evenodd_t evenodd); /* common_p.c:35-48 */
+QOP_D3_ColorVector *QOP_D3_create_V_from_raw(QOP_D_Real *src, QOP_evenodd_t evenodd); /* common_p.c:37-48 */
 QOP_F3_ColorVector *QOP_F3_create_V_from_raw(QOP_F_Real *src, QOP_evenodd_t evenodd);
-void QOP_D3_extract_V_to_raw(QOP_D_Real *dest, QOP_D3_ColorVector *src, QOP_evenodd_t evenodd); /* common_p.c:99-104 */
+void QOP_D3_extract_V_to_raw(QOP_D_Real *dest, QOP_D3_ColorVector *src, QOP_evenodd_t evenodd); /* common_p.c:100-104 */
 void QOP_F3_extract_V_to_raw(QOP_F_Real *dest, QOP_F3_ColorVector *src, QOP_evenodd_t evenodd);
-void QOP_D3_destroy_V(QOP_D3_ColorVector *field); /* qop_pc.h:64 */
+void QOP_D3_destroy_V(QOP_D3_ColorVector *field); /* common_p.c:133-145 */
 void QOP_F3_destroy_V(QOP_F3_ColorVector *field);
-QOP_D3_ColorVector *QOP_D3_convert_V_from_raw(QOP_D_Real *src, QOP_evenodd_t evenodd); /* qop_pc.h:72 */
+QOP_D3_ColorVector *QOP_D3_convert_V_from_raw(QOP_D_Real *src, QOP_evenodd_t evenodd); /* common_p.c:173-183 */
 QOP_F3_ColorVector *QOP_F3_convert_V_from_raw(QOP_F_Real *src, QOP_evenodd_t evenodd);
-QOP_D_Real *QOP_D3_convert_V_to_raw(QOP_D3_ColorVector *src, QOP_evenodd_t evenodd); /* qop_pc.h:80 */
+QOP_D_Real *QOP_D3_convert_V_to_raw(QOP_D3_ColorVector *src, QOP_evenodd_t evenodd); /* common_p.c:224-240 */
 QOP_F_Real *QOP_F3_convert_V_to_raw(QOP_F3_ColorVector *src, QOP_evenodd_t evenodd);
 
-QOP_D3_GaugeField *QOP_D3_create_G_from_raw(QOP_D_Real *links[], QOP_evenodd_t evenodd); /* common_p.c:65-78 */
+QOP_D3_GaugeField *QOP_D3_create_G_from_raw(QOP_D_Real *links[], QOP_evenodd_t evenodd); /* common_p.c:67-78 */
 QOP_F3_GaugeField *QOP_F3_create_G_from_raw(QOP_F_Real *links[], QOP_evenodd_t evenodd);
-void QOP_D3_extract_G_to_raw(QOP_D_Real *dest[], QOP_D3_GaugeField *src, QOP_evenodd_t evenodd); /* qop_pc.h:58 */
+void QOP_D3_extract_G_to_raw(QOP_D_Real *dest[], QOP_D3_GaugeField *src, QOP_evenodd_t evenodd); /* common_p.c:114-121 */
 void QOP_F3_extract_G_to_raw(QOP_F_Real *dest[], QOP_F3_GaugeField *src, QOP_evenodd_t evenodd);
-void QOP_D3_destroy_G(QOP_D3_GaugeField *field); /* qop_pc.h:66 */
+void QOP_D3_destroy_G(QOP_D3_GaugeField *field); /* common_p.c:147-160 */
 void QOP_F3_destroy_G(QOP_F3_GaugeField *field);
-/* staggered signs and boundary phases in place (common_p.c:455-526) */
+/* staggered signs and boundary phases in place (common_p.c:464-526) */
 void QOP_D3_rephase_G(QOP_D3_GaugeField *links, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign);
 void QOP_F3_rephase_G(QOP_F3_GaugeField *links, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign);
 
 /* ---- asqtad links (include/qop_pc.h:217-271; lib/asqtad_dslash_p.c) -- */
-QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_raw(QOP_D_Real *fatlinks[], QOP_D_Real *longlinks[], QOP_evenodd_t evenodd); /* :305-329 */
+QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_raw(QOP_D_Real *fatlinks[], QOP_D_Real *longlinks[], QOP_evenodd_t evenodd); /* :307-329 */
 QOP_F3_FermionLinksAsqtad *QOP_F3_asqtad_create_L_from_raw(QOP_F_Real *fatlinks[], QOP_F_Real *longlinks[], QOP_evenodd_t evenodd);
 QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_G(QOP_info_t *info, QOP_asqtad_coeffs_t *coeffs, QOP_D3_GaugeField *gauge); /* :642-664 */
 QOP_F3_FermionLinksAsqtad *QOP_F3_asqtad_create_L_from_G(QOP_info_t *info, QOP_asqtad_coeffs_t *coeffs, QOP_F3_GaugeField *gauge);
-QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_G2(QOP_info_t *info, QOP_asqtad_coeffs_t *coeffs, QOP_D3_GaugeField *gFat, QOP_D3_GaugeField *gLong); /* :666-689 */
-void QOP_D3_asqtad_extract_L_to_raw(QOP_D_Real *fatlinks[], QOP_D_Real *longlinks[], QOP_D3_FermionLinksAsqtad *src, QOP_evenodd_t evenodd); /* :915-938 */
+QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_G2(QOP_info_t *info, QOP_asqtad_coeffs_t *coeffs, QOP_D3_GaugeField *gFat, QOP_D3_GaugeField *gLong); /* :668-689 */
+void QOP_D3_asqtad_extract_L_to_raw(QOP_D_Real *fatlinks[], QOP_D_Real *longlinks[], QOP_D3_FermionLinksAsqtad *src, QOP_evenodd_t evenodd); /* :916-938 */
 void QOP_F3_asqtad_extract_L_to_raw(QOP_F_Real *fatlinks[], QOP_F_Real *longlinks[], QOP_F3_FermionLinksAsqtad *src, QOP_evenodd_t evenodd);
-void QOP_D3_asqtad_load_L_from_raw(QOP_D3_FermionLinksAsqtad *asqtad, QOP_D_Real *fatlinks[], QOP_D_Real *longlinks[], QOP_evenodd_t evenodd); /* :331-360 */
-void QOP_D3_asqtad_load_L_from_G(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *asqtad, QOP_asqtad_coeffs_t *coeffs, QOP_D3_GaugeField *gauge); /* :860-877 */
-void QOP_D3_asqtad_destroy_L(QOP_D3_FermionLinksAsqtad *field); /* :940-968 */
+void QOP_D3_asqtad_load_L_from_raw(QOP_D3_FermionLinksAsqtad *asqtad, QOP_D_Real *fatlinks[], QOP_D_Real *longlinks[], QOP_evenodd_t evenodd); /* :332-360 */
+void QOP_D3_asqtad_load_L_from_G(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *asqtad, QOP_asqtad_coeffs_t *coeffs, QOP_D3_GaugeField *gauge); /* :857-875 */
+void QOP_D3_asqtad_destroy_L(QOP_D3_FermionLinksAsqtad *field); /* :941-968 */
 void QOP_F3_asqtad_destroy_L(QOP_F3_FermionLinksAsqtad *field);
-void QOP_D3_asqtad_rephase_L(QOP_D3_FermionLinksAsqtad *fla, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign); /* :1447-1482 */
-/* precision copies (include/qop_poc.h:19-20; asqtad_dslash_p.c:742-838) */
+void QOP_D3_asqtad_rephase_L(QOP_D3_FermionLinksAsqtad *fla, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign); /* :1448-1482 */
+/* precision copies (asqtad_dslash_p.c:784-790, 832-838) */
 QOP_F3_FermionLinksAsqtad *QOP_FD3_asqtad_create_L_from_L(QOP_D3_FermionLinksAsqtad *fla_src);
 QOP_D3_FermionLinksAsqtad *QOP_DF3_asqtad_create_L_from_L(QOP_F3_FermionLinksAsqtad *fla_src);
 
@@ -236,6 +236,20 @@ int QOPX_D3_asqtad_invert_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla
                                 QOP_D3_ColorVector *out[], QOP_D3_ColorVector *in[], int n,
                                 int capacity, int prec, double *latency);
 
+/* Multi-mass solves of nsrc sources over one mass list (nmass <= 16):
+ * out[i*nmass+j] = m_j A_j^-1 in[i] on the EVEN or ODD subset, each to FP64
+ * true relative residual sqrt(res[i*nmass+j]->rsqmin).  A sloppy multi-shift
+ * CG batched over the sources reaches relative residual ms_tol (0 selects
+ * 1e-5 for FP32, 2e-3 for FP16), then every (source, mass) is refined with
+ * FP64 reliable updates, `capacity` systems at a time.  ms_iters[i]
+ * (optional): multi-shift iterations of source i.  Returns the number of
+ * systems that converged. */
+int QOPX_D3_asqtad_invert_multi_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla,
+                                      QOP_invert_arg_t *inv_arg, QOP_resid_arg_t *res[],
+                                      double masses[], int nmass, QOP_D3_ColorVector *out[],
+                                      QOP_D3_ColorVector *in[], int nsrc, int capacity, int prec,
+                                      double ms_tol, int *ms_iters);
+
 /* Mean ms per launch of the even->odd hop kernel (precision 64/32/16, nrhs
  * right-hand sides, autotuned shape (rhs per thread, block, XCD remap,
  * non-temporal links, RHS-innermost order) returned in shape[5]; red != 0 times the
@@ -251,6 +265,77 @@ double QOPX_copy_time(unsigned long long bytes, int nrep);
 int QOPX_device_ok(void);
 /* Last error message (empty when none). */
 const char *QOPX_last_error(void);
+
+/* ==== precision-generic names ==========================================
+ * As in the reference (include/qop.h:8-48 selects the precision, default
+ * single; include/qop_d3_generic.h / qop_f3_generic.h map the names), a
+ * program built with QOP_Precision 'D' or 'F' (or QOP_PrecisionInt 2 / 1)
+ * may call the generic names below.  Only Nc = 3 exists here. */
+#ifndef QOP_MI355X_NO_GENERIC
+#ifndef QOP_Precision
+#  ifndef QOP_PrecisionInt
+#    define QOP_PrecisionInt 1
+#  endif
+#  if QOP_PrecisionInt == 1
+#    define QOP_Precision 'F'
+#  elif QOP_PrecisionInt == 2
+#    define QOP_Precision 'D'
+#  else
+#    error "bad QOP_PrecisionInt"
+#  endif
+#endif
+#if QOP_Precision == 'D'
+#  define QOP_Real QOP_D_Real
+#  define QOP_ColorVector QOP_D3_ColorVector
+#  define QOP_GaugeField QOP_D3_GaugeField
+#  define QOP_FermionLinksAsqtad QOP_D3_FermionLinksAsqtad
+#  define QOP_create_V_from_raw QOP_D3_create_V_from_raw
+#  define QOP_extract_V_to_raw QOP_D3_extract_V_to_raw
+#  define QOP_destroy_V QOP_D3_destroy_V
+#  define QOP_convert_V_from_raw QOP_D3_convert_V_from_raw
+#  define QOP_convert_V_to_raw QOP_D3_convert_V_to_raw
+#  define QOP_create_G_from_raw QOP_D3_create_G_from_raw
+#  define QOP_extract_G_to_raw QOP_D3_extract_G_to_raw
+#  define QOP_destroy_G QOP_D3_destroy_G
+#  define QOP_rephase_G QOP_D3_rephase_G
+#  define QOP_asqtad_create_L_from_raw QOP_D3_asqtad_create_L_from_raw
+#  define QOP_asqtad_create_L_from_G QOP_D3_asqtad_create_L_from_G
+#  define QOP_asqtad_create_L_from_G2 QOP_D3_asqtad_create_L_from_G2
+#  define QOP_asqtad_extract_L_to_raw QOP_D3_asqtad_extract_L_to_raw
+#  define QOP_asqtad_load_L_from_raw QOP_D3_asqtad_load_L_from_raw
+#  define QOP_asqtad_load_L_from_G QOP_D3_asqtad_load_L_from_G
+#  define QOP_asqtad_destroy_L QOP_D3_asqtad_destroy_L
+#  define QOP_asqtad_rephase_L QOP_D3_asqtad_rephase_L
+#  define QOP_asqtad_dslash QOP_D3_asqtad_dslash
+#  define QOP_asqtad_diaginv QOP_D3_asqtad_diaginv
+#  define QOP_asqtad_invert QOP_D3_asqtad_invert
+#  define QOP_asqtad_invert_multi QOP_D3_asqtad_invert_multi
+#elif QOP_Precision == 'F'
+#  define QOP_Real QOP_F_Real
+#  define QOP_ColorVector QOP_F3_ColorVector
+#  define QOP_GaugeField QOP_F3_GaugeField
+#  define QOP_FermionLinksAsqtad QOP_F3_FermionLinksAsqtad
+#  define QOP_create_V_from_raw QOP_F3_create_V_from_raw
+#  define QOP_extract_V_to_raw QOP_F3_extract_V_to_raw
+#  define QOP_destroy_V QOP_F3_destroy_V
+#  define QOP_convert_V_from_raw QOP_F3_convert_V_from_raw
+#  define QOP_convert_V_to_raw QOP_F3_convert_V_to_raw
+#  define QOP_create_G_from_raw QOP_F3_create_G_from_raw
+#  define QOP_extract_G_to_raw QOP_F3_extract_G_to_raw
+#  define QOP_destroy_G QOP_F3_destroy_G
+#  define QOP_rephase_G QOP_F3_rephase_G
+#  define QOP_asqtad_create_L_from_raw QOP_F3_asqtad_create_L_from_raw
+#  define QOP_asqtad_create_L_from_G QOP_F3_asqtad_create_L_from_G
+#  define QOP_asqtad_extract_L_to_raw QOP_F3_asqtad_extract_L_to_raw
+#  define QOP_asqtad_destroy_L QOP_F3_asqtad_destroy_L
+#  define QOP_asqtad_dslash QOP_F3_asqtad_dslash
+#  define QOP_asqtad_diaginv QOP_F3_asqtad_diaginv
+#  define QOP_asqtad_invert QOP_F3_asqtad_invert
+#  define QOP_asqtad_invert_multi QOP_F3_asqtad_invert_multi
+#else
+#  error "bad QOP_Precision"
+#endif
+#endif /* QOP_MI355X_NO_GENERIC */
 
 #ifdef __cplusplus
 }

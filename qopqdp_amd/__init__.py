@@ -149,6 +149,9 @@ SIGNATURES = [
     ("QOPX_D3_asqtad_invert_batch", C.c_int, [C.POINTER(QOPInfo), _VP, C.POINTER(QOPInvertArg), _PP,
                                               C.POINTER(C.c_double), _PP, _PP, C.c_int, C.c_int, C.c_int,
                                               C.POINTER(C.c_double)]),
+    ("QOPX_D3_asqtad_invert_multi_batch", C.c_int, [C.POINTER(QOPInfo), _VP, C.POINTER(QOPInvertArg), _PP,
+                                                    C.POINTER(C.c_double), C.c_int, _PP, _PP, C.c_int,
+                                                    C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int)]),
     ("QOPX_D3_asqtad_hop_time", C.c_double, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     ("QOPX_copy_time", C.c_double, [C.c_ulonglong, C.c_int]),
     ("QOPX_device_ok", C.c_int, []),

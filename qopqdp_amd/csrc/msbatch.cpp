@@ -1,0 +1,301 @@
+// Batched multi-shift CG driver (msbatch.h) and the multi-mass entry point
+// that refines every (source, mass) to its FP64 true residual.
+#include "msbatch.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+
+#include "handles.h"
+
+namespace qop {
+
+MultiShiftBatch::MultiShiftBatch(Links &L, int eo, int capacity, int sloppy_prec,
+                                 const double *masses, int ns)
+    : L_(L), lat_(L.lat), eo_(eo), cap_(capacity), sp_(sloppy_prec), ns_(ns) {
+  QOP_REQUIRE(eo == 0 || eo == 1, "multi-shift: subset must be even or odd");
+  QOP_REQUIRE(ns >= 1 && ns <= MS_MAXSHIFT, "multi-shift: 1..16 masses");
+  QOP_REQUIRE(capacity >= 1 && capacity <= 64, "multi-shift: capacity 1..64");
+  QOP_REQUIRE(sp_ == P32 || sp_ == P16, "multi-shift: sloppy precision must be FP32 or FP16");
+  masses_.assign(masses, masses + ns);
+  imin_ = 0;
+  for (int i = 1; i < ns; i++)
+    if (masses[i] * masses[i] < masses[imin_] * masses[imin_]) imin_ = i;
+  nch_ = (int)((lat_->Vh + 63) / 64);
+  devbatch_alloc(b64_, lat_, P64, cap_);
+  devbatch_alloc(rs_, lat_, sp_, cap_);
+  devbatch_alloc(ps_, lat_, sp_, cap_);
+  devbatch_alloc(ts_, lat_, sp_, cap_);
+  devbatch_alloc(pm_, lat_, sp_, cap_ * ns_);
+  devbatch_alloc(xm_, lat_, P32, cap_ * ns_);
+  QOP_CHECK_HIP(hipMalloc(&slot_, cap_ * sizeof(MsSlot)));
+  QOP_CHECK_HIP(hipHostMalloc(&slot_h_, cap_ * sizeof(MsSlot), hipHostMallocDefault));
+  QOP_CHECK_HIP(hipMalloc(&active_, 2 * cap_ * sizeof(int)));
+  sel_ = active_ + cap_;
+  QOP_CHECK_HIP(hipHostMalloc(&active_h_, cap_ * sizeof(int), hipHostMallocDefault));
+  QOP_CHECK_HIP(hipMalloc(&partA_, 3 * (size_t)cap_ * nch_ * sizeof(double)));
+  partB_ = partA_ + (size_t)cap_ * nch_;
+  partC_ = partB_ + (size_t)cap_ * nch_;
+  QOP_CHECK_HIP(hipMalloc(&coef_, 2 * (size_t)cap_ * sizeof(double)));
+  QOP_CHECK_HIP(hipMalloc(&sums_, 2 * (size_t)cap_ * sizeof(double)));
+  QOP_CHECK_HIP(hipMalloc(&zero_status_, sizeof(int)));
+  QOP_CHECK_HIP(hipMemset(zero_status_, 0, sizeof(int)));
+  memset(slot_h_, 0, cap_ * sizeof(MsSlot));
+  memset(active_h_, 0, cap_ * sizeof(int));
+  QOP_CHECK_HIP(hipMemset(active_, 0, 2 * cap_ * sizeof(int)));
+  const double m0 = masses[imin_];
+  std::vector<double> hc(2 * cap_);
+  for (int b = 0; b < cap_; b++) { hc[2 * b] = -1.0; hc[2 * b + 1] = m0 * m0; }
+  QOP_CHECK_HIP(hipMemcpy(coef_, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice));
+  shape_ = tune_hop(L_, sp_, cap_, ps_, ts_, 1 - eo_);
+  sync();
+}
+
+MultiShiftBatch::~MultiShiftBatch() {
+  sync();
+  if (graph_) (void)hipGraphExecDestroy(graph_);
+  for (DevBatch *b : {&b64_, &rs_, &ps_, &ts_, &pm_, &xm_}) devbatch_free(*b);
+  (void)hipFree(slot_);
+  (void)hipHostFree(slot_h_);
+  (void)hipFree(active_);
+  (void)hipHostFree(active_h_);
+  (void)hipFree(partA_);
+  (void)hipFree(coef_);
+  (void)hipFree(sums_);
+  (void)hipFree(zero_status_);
+}
+
+void MultiShiftBatch::load(int slot, const DevVec &src, double ms_tol, int max_iter) {
+  QOP_REQUIRE(slot >= 0 && slot < cap_, "multi-shift: bad slot");
+  hipStream_t st = stream();
+  const long Vh = lat_->Vh;
+  auto copy_view = [&](VecView dst, int pd, VecView s, int ps) {
+    LincombArgs a;
+    a.Vh = (int)Vh;
+    a.out = dst;
+    a.v[0] = s;
+    a.cr[0] = 1.0;
+    int pin[1] = {ps};
+    launch_lincomb(a, pd, pin, 1, false, st);
+  };
+  copy_view(b64_.slot(slot), P64, src.half(eo_), P64);
+  copy_view(rs_.slot(slot), sp_, src.half(eo_), P64);   // r = b (x = 0)
+  copy_view(ps_.slot(slot), sp_, rs_.slot(slot), sp_);  // p = r
+  for (int i = 0; i < ns_; i++) {
+    const int k = i * cap_ + slot;
+    QOP_CHECK_HIP(hipMemsetAsync(xm_.slot(k).data, 0, 3 * Vh * sizeof(float2), st));
+    if (i != imin_) copy_view(pm_.slot(k), sp_, rs_.slot(slot), sp_);
+  }
+  // |b|^2 (FP64), |r|^2 and |p|^2 of the stored sloppy vectors
+  std::vector<int> sel(cap_, 0);
+  sel[slot] = 1;
+  QOP_CHECK_HIP(hipMemcpyAsync(sel_, sel.data(), cap_ * sizeof(int), hipMemcpyHostToDevice, st));
+  DotArgs d;
+  d.Vh = (int)Vh;
+  d.n = cap_;
+  d.ustride = 3 * Vh;
+  d.nstride = Vh;
+  d.active = sel_;
+  d.u = b64_.slot(0);
+  d.partial = partA_;
+  launch_dot(d, P64, P64, 0, st);
+  bs_launch_sum(partA_, nch_, cap_, sel_, sums_, st);
+  d.u = ps_.slot(0);
+  d.partial = partB_;  // |p|^2 rows for the first iteration
+  launch_dot(d, sp_, sp_, 0, st);
+  bs_launch_sum(partB_, nch_, cap_, sel_, sums_ + cap_, st);
+  std::vector<double> h(2 * cap_);
+  QOP_CHECK_HIP(hipMemcpyAsync(h.data(), sums_, 2 * cap_ * sizeof(double), hipMemcpyDeviceToHost, st));
+  sync();
+  MsSlot &s = slot_h_[slot];
+  memset(&s, 0, sizeof s);
+  s.ns = ns_;
+  s.imin = imin_;
+  s.max_iter = max_iter;
+  s.b2 = h[slot];
+  s.r2 = h[cap_ + slot];  // |r|^2 = |p|^2 of the stored sloppy r
+  const double m02 = masses_[imin_] * masses_[imin_];
+  s.m2 = m02;
+  for (int i = 0; i < ns_; i++) {
+    s.shift[i] = masses_[i] * masses_[i] - m02;
+    s.stop2[i] = ms_tol * ms_tol * s.b2;
+    s.z[i] = s.zo[i] = s.zn[i] = 1;
+    s.bo[i] = -1;
+    s.a[i] = 0;
+    s.b[i] = 0;
+    s.live_next[i] = 1;
+  }
+  const bool zero = s.b2 == 0.0;
+  if (zero) s.done = 1;
+  active_h_[slot] = zero ? 0 : 1;
+  QOP_CHECK_HIP(hipMemcpyAsync(slot_ + slot, &s, sizeof s, hipMemcpyHostToDevice, st));
+  QOP_CHECK_HIP(hipMemcpyAsync(active_ + slot, active_h_ + slot, sizeof(int), hipMemcpyHostToDevice, st));
+  sync();
+}
+
+void MultiShiftBatch::enqueue_iteration() {
+  hipStream_t st = stream();
+  LinkSet &ls = L_.get(sp_);
+  const int opp = 1 - eo_;
+  const long Vh = lat_->Vh;
+  HopArgs a;
+  a.g = lat_->g;
+  a.parity = opp;
+  a.links = ls.parity_data(lat_, opp);
+  a.lscale = ls.parity_scale(lat_, opp);
+  a.src = ps_.slot(0);
+  a.dst = ts_.slot(0);
+  a.sstride = a.dstride = a.xstride = 3 * Vh;
+  a.nstride = Vh;
+  a.n = cap_;
+  a.active = active_;
+  a.partial = partA_;
+  a.remap = shape_.remap;
+  a.nt = shape_.nt;
+  a.rhs_inner = shape_.rhs_inner;
+  launch_hop_args(a, sp_, ls.nl, shape_.nb, shape_.block, RED_OUT2, false, st);
+  ms_launch_scalarA(slot_, active_, partA_, partB_, nch_, cap_, st);
+  // r -= beta_0 (m_0^2 p - D t), x_base += beta_0 p, |r|^2
+  HopArgs b = a;
+  b.parity = eo_;
+  b.links = ls.parity_data(lat_, eo_);
+  b.lscale = ls.parity_scale(lat_, eo_);
+  b.src = ts_.slot(0);
+  b.dst = rs_.slot(0);
+  b.x = ps_.slot(0);
+  b.xs = xm_.slot(imin_ * cap_);
+  b.coef = coef_;
+  b.cg_alpha = (const double *)((const char *)slot_ + offsetof(MsSlot, beta0));
+  b.cg_stride = (int)(sizeof(MsSlot) / sizeof(double));
+  b.partial = partC_;
+  launch_hop_args(b, sp_, ls.nl, shape_.nb, shape_.block, RED_CG, true, st);
+  ms_launch_scalarB(slot_, active_, partC_, nch_, cap_, st);
+  ms_launch_update(slot_, active_, sp_, (int)Vh, rs_.slot(0), ps_.slot(0), pm_.slot(0), xm_.slot(0),
+                   3 * Vh, 3 * Vh * cap_, cap_, ns_, partB_, st);
+}
+
+void MultiShiftBatch::run() {
+  hipStream_t st = stream();
+  const int chunk = options().chunk > 0 ? options().chunk : 16;
+  if (!graph_) {
+    hipGraph_t g;
+    QOP_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int c = 0; c < chunk; c++) enqueue_iteration();
+    QOP_CHECK_HIP(hipStreamEndCapture(st, &g));
+    QOP_CHECK_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
+    QOP_CHECK_HIP(hipGraphDestroy(g));
+  }
+  while (true) {
+    bool any = false;
+    for (int i = 0; i < cap_; i++) any |= active_h_[i] != 0;
+    if (!any) break;
+    QOP_CHECK_HIP(hipGraphLaunch(graph_, st));
+    QOP_CHECK_HIP(hipMemcpyAsync(active_h_, active_, cap_ * sizeof(int), hipMemcpyDeviceToHost, st));
+    sync();
+  }
+  QOP_CHECK_HIP(hipMemcpy(slot_h_, slot_, cap_ * sizeof(MsSlot), hipMemcpyDeviceToHost));
+}
+
+void MultiShiftBatch::solution(int slot, int k, DevVec &out) {
+  LincombArgs a;
+  a.Vh = (int)lat_->Vh;
+  a.out = out.half(eo_);
+  a.v[0] = xm_.slot(k * cap_ + slot);
+  a.cr[0] = 1.0;
+  int pin[1] = {P32};
+  launch_lincomb(a, out.prec, pin, 1, false, stream());
+}
+
+int MultiShiftBatch::iterations(int slot) { return slot_h_[slot].iter; }
+
+}  // namespace qop
+
+using namespace qop;
+
+extern "C" {
+
+// Multi-mass solves of nsrc sources over one shared mass list:
+// out[i*nmass + j] = m_j A_j^-1 in[i] on the EVEN or ODD subset, each to FP64
+// true relative residual sqrt(res[i*nmass+j]->rsqmin).  Sloppy multi-shift CG
+// batched over the sources to relative residual ms_tol (0: 1e-5 in FP32,
+// 2e-3 in FP16), then FP64 reliable-update refinement of every (source, mass)
+// from its multi-shift solution, `capacity` systems at a time.
+int QOPX_D3_asqtad_invert_multi_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla,
+                                      QOP_invert_arg_t *inv_arg, QOP_resid_arg_t *res[],
+                                      double masses[], int nmass, QOP_D3_ColorVector *out[],
+                                      QOP_D3_ColorVector *in[], int nsrc, int capacity, int prec,
+                                      double ms_tol, int *ms_iters) {
+  Links &L = fla->L;
+  const int eo = inv_arg->evenodd;
+  QOP_REQUIRE(eo == QOP_EVEN || eo == QOP_ODD, "invert_multi_batch: subset must be EVEN or ODD");
+  QOP_REQUIRE(prec == 16 || prec == 32, "invert_multi_batch: sloppy precision must be 16 or 32");
+  QOP_REQUIRE(nmass >= 1 && nmass <= MS_MAXSHIFT && nsrc >= 0 && capacity >= 1, "invert_multi_batch: sizes");
+  if (nsrc == 0) return 0;
+  const int sp = prec == 16 ? P16 : P32;
+  if (ms_tol <= 0) ms_tol = sp == P16 ? 2e-3 : 1e-5;
+  double t0 = wall_time();
+  Lattice *lat = L.lat;
+  const int n = nsrc * nmass;
+  // stage 1: multi-shift, sources in groups of up to `capacity`
+  std::vector<DevVec> guess(n);
+  long ms_total = 0;
+  {
+    const int cap = std::min(capacity, nsrc);
+    MultiShiftBatch ms(L, eo, cap, sp, masses, nmass);
+    for (int s0 = 0; s0 < nsrc; s0 += cap) {
+      const int cnt = std::min(cap, nsrc - s0);
+      for (int b = 0; b < cnt; b++) ms.load(b, in[s0 + b]->v, ms_tol, inv_arg->max_iter);
+      ms.run();
+      for (int b = 0; b < cnt; b++) {
+        if (ms_iters) ms_iters[s0 + b] = ms.iterations(b);
+        ms_total += ms.iterations(b);
+        for (int j = 0; j < nmass; j++) {
+          DevVec &g = guess[(s0 + b) * nmass + j];
+          devvec_alloc(g, lat, P64);
+          ms.solution(b, j, g);
+        }
+      }
+      sync();
+    }
+  }
+  // stage 2: FP64 reliable-update refinement of every (source, mass)
+  const int cap = std::min(capacity, n);
+  BatchSolver bs(L, eo, cap, sp, options().delta);
+  std::vector<int> slot_of(cap, -1);
+  int next = 0, finished = 0, ok = 0;
+  long iters = 0;
+  while (finished < n) {
+    int s;
+    while (next < n && (s = bs.free_slot()) >= 0) {
+      const int src = next / nmass, j = next % nmass;
+      bs.load(s, in[src]->v, &guess[next], masses[j], std::sqrt(res[next]->rsqmin), inv_arg->max_iter);
+      slot_of[s] = next++;
+    }
+    std::vector<int> done;
+    bs.step(done);
+    for (int d : done) {
+      const int k = slot_of[d];
+      SlotResult r;
+      bs.result(d, out[k]->v, true, &r);
+      res[k]->final_iter = r.iter;
+      res[k]->final_rsq = r.true_rsq;
+      res[k]->final_rel = 0;
+      res[k]->final_restart = r.nupdate;
+      ok += r.reason == 1;
+      iters += r.iter;
+      bs.release(d);
+      devvec_free(guess[k]);
+      finished++;
+    }
+  }
+  if (info) {
+    info->final_sec = wall_time() - t0;
+    const double nflop = 0.5 * (20 * 3 + 2 * 8 * 9 * L.nl), nflopm = 0.5 * 10 * 3;
+    info->final_flop = ((nflop + nflopm * (nmass - 1)) * ms_total + nflop * iters) * (double)lat->V;
+    info->status = QOP_SUCCESS;
+  }
+  return ok;
+}
+
+}  // extern "C"

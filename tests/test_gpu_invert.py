@@ -202,3 +202,75 @@ def test_mixed_batch_is_batch_invariant(setup, oracle):
                 assert oracle.true_resid_even(h, m, alone, srcs[i]) <= 1e-10
     finally:
         s.set_opts(qmx_solver=0)
+
+
+def batch_api(s, fla, srcs, masses, tol, prec, multi=False, capacity=16):
+    """QOPX_D3_asqtad_invert_batch (one mass per source) or
+    QOPX_D3_asqtad_invert_multi_batch (every source x every mass)."""
+    P = C.c_void_p
+    vins = [s.vec(x) for x in srcs]
+    n = len(srcs) * len(masses) if multi else len(srcs)
+    vouts = [s.zeros() for _ in range(n)]
+    ras = [q.resid_arg_default() for _ in range(n)]
+    for r in ras:
+        r.rsqmin = tol * tol
+    ia = q.invert_arg_default()
+    ia.evenodd = EVEN
+    ia.max_iter = 20000
+    info = q.QOPInfo()
+    rp = (P * n)(*[C.cast(C.pointer(r), P) for r in ras])
+    if multi:
+        ms_it = (C.c_int * len(srcs))()
+        ok = s.lib.QOPX_D3_asqtad_invert_multi_batch(
+            C.byref(info), fla, C.byref(ia), rp, (C.c_double * len(masses))(*masses), len(masses),
+            (P * n)(*vouts), (P * len(srcs))(*vins), len(srcs), capacity, prec, 0.0, ms_it)
+    else:
+        ok = s.lib.QOPX_D3_asqtad_invert_batch(
+            C.byref(info), fla, C.byref(ia), rp, (C.c_double * n)(*masses), (P * n)(*vouts),
+            (P * n)(*vins), n, capacity, prec, None)
+    outs = [s.get(v) for v in vouts]
+    for v in vins + vouts:
+        s.free(v)
+    return ok, outs, ras
+
+
+@pytest.mark.parametrize("prec", [32, 16])
+def test_multi_mass_batch_every_shift_converges(setup, oracle, prec):
+    """Sources x masses (config-3 shape at 8^4): batched sloppy multi-shift CG
+    then FP64 reliable-update refinement; every (source, mass) must reach the
+    FP64 true residual and agree with the oracle's single-mass solve."""
+    s, L, h = setup["s"], setup["L"], setup["h"]
+    srcs = [q.gen_source(L, 2060 + i, EVEN) for i in range(3)]
+    masses = [0.005, 0.0242978, 0.0697106, 0.2]
+    tol = 1e-10
+    ok, outs, ras = batch_api(s, setup["fla"], srcs, masses, tol, prec, multi=True, capacity=5)
+    assert ok == len(outs)
+    V = s.V
+    for i, b in enumerate(srcs):
+        for j, m in enumerate(masses):
+            out = outs[i * len(masses) + j]
+            tr = oracle.true_resid_even(h, m, out, b)
+            assert tr <= tol, (i, m, tr)
+            assert ras[i * len(masses) + j].final_rsq == pytest.approx(tr * tr, rel=1e-2)
+            if i == 0:
+                ref, _, _ = oracle.invert(h, m, b, EVEN, 1e-24)
+                d = np.linalg.norm(out[: V // 2] - ref[: V // 2]) / np.linalg.norm(ref[: V // 2])
+                assert d < 1e-8, (m, d)
+
+
+def test_batch_api_matches_qop_invert(setup, oracle):
+    """QOPX batched solves equal the QOP entry point with qmx_solver=1 bit for
+    bit (same per-slot arithmetic), for a batch of mixed masses."""
+    s, L, h = setup["s"], setup["L"], setup["h"]
+    srcs = [q.gen_source(L, 2070 + i, EVEN) for i in range(4)]
+    masses = [0.01, 0.05, 0.01, 0.3]
+    ok, outs, ras = batch_api(s, setup["fla"], srcs, masses, 1e-10, 32, capacity=3)
+    assert ok == 4
+    assert s.set_opts(qmx_solver=1, qmx_prec=32) == 0
+    try:
+        for b, m, o, r in zip(srcs, masses, outs, ras):
+            alone, ra, _ = solve(s, setup["fla"], m, b, EVEN, 1e-20)
+            assert np.array_equal(alone, o)
+            assert ra.final_iter == r.final_iter
+    finally:
+        s.set_opts(qmx_solver=0)
