@@ -56,11 +56,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only time the dominant hop kernel (for the rocprofv3 cross-check)")
+    ap.add_argument("--serve", action="store_true",
+                    help="serving mix instead: Poisson arrivals over several lattices")
+    ap.add_argument("--serve-lattices", default="16,16,16,16;24,24,24,48;32,32,32,64")
+    ap.add_argument("--serve-requests", type=int, default=48, help="requests per rate")
+    ap.add_argument("--serve-rates", default="0.5,1,2,4",
+                    help="arrival rates as multiples of the FP64 one-at-a-time capacity")
+    ap.add_argument("--serve-capacity", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the build has no "
               "multi-GPU path", file=sys.stderr)
         sys.exit(2)
+    if args.serve:
+        serve_bench(args)
+        return
 
     import qopqdp_amd as q
 
@@ -234,6 +244,110 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def serve_bench(args):
+    """Serving mix: requests over lattices x 2 resident configurations each,
+    masses log-uniform on [0.005, 0.1], tolerances in {1e-8, 1e-10}, seeded
+    Gaussian even-site sources, Poisson arrivals (seed 31337) at rates swept
+    as multiples of the FP64 one-at-a-time capacity.  The batched server
+    (mixed precision, continuous batching) and the FP64 one-at-a-time server
+    see the same request streams; prints one JSON line."""
+    import qopqdp_amd as q
+    lattices = [[int(x) for x in t.split(",")] for t in args.serve_lattices.split(";")]
+    rng = np.random.default_rng(31337)
+    t0 = time.time()
+    gauges = []
+    for li, L in enumerate(lattices):
+        for k in range(2):
+            gauges.append((L, 1000 + 10 * li + k))
+
+    def make(prec, cap):
+        srv = q.Server(capacity=cap, prec=prec)
+        for L, seed in gauges:
+            U = q.gen_gauge(L, seed, 0.3)
+            srv.add_gauge(L, U)
+            del U
+        return srv
+
+    n_max = args.serve_requests
+    pool = []  # request parameters, drawn once per rate from the same generator
+    src_cache = {}
+
+    def source(cfg, seed):
+        key = (cfg, seed)
+        if key not in src_cache:
+            src_cache[key] = q.gen_source(gauges[cfg][0], seed, q.QOP_EVEN)
+        return src_cache[key]
+
+    def draw(n):
+        reqs = []
+        for i in range(n):
+            cfg = int(rng.integers(len(gauges)))
+            m = float(np.exp(rng.uniform(np.log(0.005), np.log(0.1))))
+            tol = [1e-8, 1e-10][int(rng.integers(2))]
+            reqs.append((cfg, m, tol, 5000 + int(rng.integers(8))))
+        return reqs
+
+    def run(srv, reqs, rate):
+        gaps = rng.exponential(1.0 / rate, len(reqs)) if rate else np.zeros(len(reqs))
+        ids, t_sub = [], []
+        start = time.perf_counter()
+        t_next = start
+        for (cfg, m, tol, sd), gap in zip(reqs, gaps):
+            t_next += gap
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            t_sub.append(time.perf_counter())
+            ids.append(srv.submit(cfg, source(cfg, sd), m, tol, q.QOP_EVEN, max_iter=100000))
+        lat, ok = [], 0
+        for rid in ids:
+            out, st, res, l = srv.wait(rid)
+            ok += st == 0
+            lat.append(l)
+        done = max(ts + l for ts, l in zip(t_sub, lat))
+        return dict(solves_per_s=round(len(reqs) / (done - start), 3),
+                    p50_ms=round(float(np.percentile(lat, 50)) * 1e3, 1),
+                    p99_ms=round(float(np.percentile(lat, 99)) * 1e3, 1), converged=ok)
+
+    # FP64 one-at-a-time capacity from back-to-back requests
+    srv64 = make(64, 1)
+    log(f"serve: setup {time.time() - t0:.1f}s")
+    warm = draw(len(gauges))
+    run(srv64, [(c, 0.1, 1e-8, 5000) for c in range(len(gauges))], 0)
+    cap_reqs = draw(max(8, n_max // 4))
+    cap = run(srv64, cap_reqs, 0)
+    c64 = cap["solves_per_s"]
+    log(f"serve: FP64 one-at-a-time capacity {c64} solves/s")
+    rates = [float(x) for x in args.serve_rates.split(",")]
+    fp64_half = run(srv64, draw(max(8, n_max // 4)), 0.5 * c64)
+    log(f"serve: FP64 loop at 0.5x capacity: {fp64_half}")
+    srv64.close()
+    srv = make(args.prec, args.serve_capacity)
+    run(srv, warm, 0)  # builds and tunes every batch
+    sweep = []
+    for f in rates:
+        r = run(srv, draw(n_max), f * c64)
+        r["rate_x_fp64"] = f
+        r["arrival_per_s"] = round(f * c64, 3)
+        sweep.append(r)
+        log(f"serve: rate {f}x -> {r}")
+    st = srv.stats()
+    srv.close()
+    best = max(sweep, key=lambda r: r["solves_per_s"])
+    print(json.dumps({
+        "metric": "serving solves/s + p50/p99 latency, continuous batching vs FP64 one-at-a-time",
+        "value": best["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+        "higher_is_better": True, "data": "synthetic",
+        "dtype": f"fp{args.prec} storage, fp32 arithmetic, fp64 reductions/true residual",
+        "config": {"workload": "asqtad even-subset solve requests, Poisson arrivals",
+                   "lattices": lattices, "configs_per_lattice": 2,
+                   "mass": "log-uniform [0.005, 0.1]", "tol": [1e-8, 1e-10],
+                   "requests_per_rate": n_max, "capacity": args.serve_capacity},
+        "fp64_one_at_a_time": {"capacity_solves_per_s": c64, "at_0.5x": fp64_half},
+        "sweep": sweep, "batches": st["batches"], "chunks": st["chunks"],
+    }), flush=True)
 
 
 def cpu_baseline(s, fla, L, src, args, gpu_iters):

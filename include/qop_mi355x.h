@@ -261,6 +261,55 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
  * the measured copy bandwidth and the calibration kernel for HBM counters. */
 double QOPX_copy_time(unsigned long long bytes, int nrep);
 
+/* ---- request server: continuous batching over resident configurations ----
+ * A server keeps fat/long links of any number of gauge configurations, of
+ * any lattice sizes, resident on the device (QOP_init is not needed and its
+ * lattice is not used) and solves EVEN/ODD requests out = m A^-1 src to FP64
+ * true relative residual `tol`.  A host worker thread admits requests as
+ * slots and HBM free up into per-(configuration, parity) batches that are
+ * already iterating, so requests that arrive while others run join them at
+ * the next iteration chunk; batches of different configurations run
+ * concurrently on separate streams.  A request's result does not depend on
+ * what it was batched with.
+ *   capacity  slots per batch (1..256)
+ *   prec      16 or 32: sloppy precision of the reliable-update CG;
+ *             64: the FP64 reference algorithm, one request at a time
+ *   max_bytes HBM budget for links and batches (0: the device's free memory
+ *             less a reserve); a batch is created only when it fits, idle
+ *             batches of other configurations are evicted to make room.
+ * Returns null when no device is usable or an argument is invalid. */
+typedef struct QOPX_Server_struct QOPX_Server;
+QOPX_Server *QOPX_serve_create(int capacity, int prec, unsigned long long max_bytes);
+/* Add a configuration from a raw FP64 gauge field (host): rephased with r0,
+ * bc, sign (as QOP_D3_rephase_G; null pointers skip) and smeared with coeffs
+ * on the device.  Returns the configuration index, or -1 if over budget. */
+int QOPX_serve_add_gauge(QOPX_Server *s, const int L[4], QOP_D_Real *U[4],
+                         QOP_asqtad_coeffs_t *coeffs, int *r0, QOP_bc_t *bc,
+                         QOP_staggered_sign_t *sign);
+/* Add a configuration from raw FP64 fat and long links (host, as
+ * QOP_D3_asqtad_create_L_from_raw; lng may be null). */
+int QOPX_serve_add_links(QOPX_Server *s, const int L[4], QOP_D_Real *fat[4], QOP_D_Real *lng[4]);
+/* Queue a solve on subset evenodd (QOP_EVEN or QOP_ODD) of configuration
+ * `config`.  src and out are raw FP64 host vectors [V][3][2] that must stay
+ * valid until the request is waited for; only the sites of evenodd of out are
+ * written.  max_iter <= 0 selects 20000.  Returns a request id, or -1. */
+long QOPX_serve_submit(QOPX_Server *s, int config, QOP_evenodd_t evenodd, double mass, double tol,
+                       int max_iter, const QOP_D_Real *src, QOP_D_Real *out);
+/* Block until the request is done; fill res (final_rsq = FP64 true
+ * |r|^2/|src|^2, final_iter, final_restart = reliable updates) and latency
+ * (seconds from submit to completion).  Returns 0 if it converged, 1 if it
+ * stopped at max_iter, -1 for an unknown or failed request.  A request is
+ * waited for once. */
+int QOPX_serve_wait(QOPX_Server *s, long request, QOP_resid_arg_t *res, double *latency);
+/* 1 if the request is done, 0 if not, -1 if unknown. */
+int QOPX_serve_test(QOPX_Server *s, long request);
+/* completed, queued, running, batches, bytes used, byte budget, chunks
+ * launched, failed. */
+void QOPX_serve_stats(QOPX_Server *s, double stats[8]);
+/* Finish every admitted request, then free the server.  Requests that can
+ * never be admitted fail. */
+void QOPX_serve_destroy(QOPX_Server *s);
+
 /* Library status: 1 when a HIP device is usable. */
 int QOPX_device_ok(void);
 /* Last error message (empty when none). */

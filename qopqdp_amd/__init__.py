@@ -154,6 +154,15 @@ SIGNATURES = [
                                                     C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int)]),
     ("QOPX_D3_asqtad_hop_time", C.c_double, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     ("QOPX_copy_time", C.c_double, [C.c_ulonglong, C.c_int]),
+    ("QOPX_serve_create", _VP, [C.c_int, C.c_int, C.c_ulonglong]),
+    ("QOPX_serve_add_gauge", C.c_int, [_VP, C.POINTER(C.c_int), _PP, C.POINTER(QOPAsqtadCoeffs),
+                                       C.POINTER(C.c_int), C.POINTER(QOPBc), C.POINTER(QOPSign)]),
+    ("QOPX_serve_add_links", C.c_int, [_VP, C.POINTER(C.c_int), _PP, _PP]),
+    ("QOPX_serve_submit", C.c_long, [_VP, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _VP, _VP]),
+    ("QOPX_serve_wait", C.c_int, [_VP, C.c_long, C.POINTER(QOPResidArg), C.POINTER(C.c_double)]),
+    ("QOPX_serve_test", C.c_int, [_VP, C.c_long]),
+    ("QOPX_serve_stats", None, [_VP, C.POINTER(C.c_double)]),
+    ("QOPX_serve_destroy", None, [_VP]),
     ("QOPX_device_ok", C.c_int, []),
     ("QOPX_last_error", C.c_char_p, []),
 ]
@@ -315,3 +324,76 @@ def milc_rephase_args(L, antiperiodic_t=True):
     sm = ivec([0, 1, 3, 7])
     sign = QOPSign(sm)
     return r0, bc, sign, (ph, sm)
+
+
+class Server:
+    """QOPX_serve_* request server: resident links of several lattices and
+    configurations, continuous batching of EVEN/ODD solves.  Source and
+    output arrays are kept alive here until their request is waited for."""
+
+    def __init__(self, capacity=16, prec=16, max_bytes=0):
+        self.lib = lib()
+        self.h = self.lib.QOPX_serve_create(capacity, prec, max_bytes)
+        if not self.h:
+            raise RuntimeError("QOPX_serve_create failed (no HIP device or bad arguments)")
+        self.L = {}
+        self._keep = {}
+
+    def add_gauge(self, L, U, coeffs=None, milc_phases=True):
+        U = [np.ascontiguousarray(u, dtype=np.float64) for u in U]
+        c = QOPAsqtadCoeffs(**(coeffs or MILC_ASQTAD))
+        if milc_phases:
+            r0, bc, sign, keep = milc_rephase_args(L)
+            cfg = self.lib.QOPX_serve_add_gauge(self.h, ivec(L), ptr_array(U), C.byref(c), r0,
+                                                C.byref(bc), C.byref(sign))
+        else:
+            cfg = self.lib.QOPX_serve_add_gauge(self.h, ivec(L), ptr_array(U), C.byref(c), None,
+                                                None, None)
+        if cfg < 0:
+            raise RuntimeError("configuration does not fit the HBM budget")
+        self.L[cfg] = list(L)
+        return cfg
+
+    def add_links(self, L, fat, lng):
+        fat = [np.ascontiguousarray(f, dtype=np.float64) for f in fat]
+        lng = [np.ascontiguousarray(x, dtype=np.float64) for x in lng] if lng is not None else None
+        cfg = self.lib.QOPX_serve_add_links(self.h, ivec(L), ptr_array(fat),
+                                            ptr_array(lng) if lng is not None else None)
+        if cfg < 0:
+            raise RuntimeError("configuration does not fit the HBM budget")
+        self.L[cfg] = list(L)
+        return cfg
+
+    def submit(self, cfg, src, mass, tol, eo=QOP_EVEN, max_iter=0, out=None):
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        if out is None:
+            out = np.zeros_like(src)
+        rid = self.lib.QOPX_serve_submit(self.h, cfg, eo, mass, tol, max_iter,
+                                         src.ctypes.data, out.ctypes.data)
+        if rid < 0:
+            raise RuntimeError("QOPX_serve_submit rejected the request")
+        self._keep[rid] = (src, out)
+        return rid
+
+    def wait(self, rid):
+        """-> (out, status, QOPResidArg, latency seconds)"""
+        res = QOPResidArg()
+        lat = C.c_double(0)
+        st = self.lib.QOPX_serve_wait(self.h, rid, C.byref(res), C.byref(lat))
+        src, out = self._keep.pop(rid)
+        return out, st, res, lat.value
+
+    def test(self, rid):
+        return self.lib.QOPX_serve_test(self.h, rid)
+
+    def stats(self):
+        s = (C.c_double * 8)()
+        self.lib.QOPX_serve_stats(self.h, s)
+        keys = ("completed", "queued", "running", "batches", "bytes_used", "byte_budget",
+                "chunks", "failed")
+        return dict(zip(keys, list(s)))
+
+    def close(self):
+        if self.h:
+            self.lib.QOPX_serve_destroy(self.h)
+            self.h = None

@@ -157,6 +157,25 @@ void rephase_dev(Lattice *lat, double *const U[4], int *r0, QOP_bc_t *bc, QOP_st
 
 }  // namespace
 
+namespace qop {
+
+void links_from_raw64(Links &L, double *fat[4], double *lng[4]) {
+  build_links_from_raw(L, fat, lng, QOP_EVENODD);
+  sync();
+}
+
+void links_from_gauge64(Links &L, double *const U_host[4], const QOP_asqtad_coeffs_t *c, int *r0,
+                        QOP_bc_t *bc, QOP_staggered_sign_t *sign) {
+  double *U[4];
+  for (int mu = 0; mu < 4; mu++) U[mu] = upload_mat(L.lat, U_host[mu], QOP_EVENODD);
+  rephase_dev(L.lat, U, r0, bc, sign, 1, 1, nullptr, 0);
+  fatten(L, L.lat, U, U, c, nullptr);
+  sync();
+  for (int mu = 0; mu < 4; mu++) QOP_CHECK_HIP(hipFree(U[mu]));
+}
+
+}  // namespace qop
+
 extern "C" {
 
 // ---- general ------------------------------------------------------------

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 
 namespace qop {
@@ -14,6 +15,8 @@ namespace qop {
 // autotuner: time the multi-RHS hop for each launch shape and keep the best.
 HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &dst, int eo_out) {
   static std::map<std::tuple<const Lattice *, int, int, int>, HopShape> cache;
+  static std::mutex mu;  // solvers may be built from several host threads
+  std::lock_guard<std::mutex> lk(mu);
   auto key = std::make_tuple((const Lattice *)L.lat, prec, L.nl, nrhs);
   auto it = cache.find(key);
   if (it != cache.end()) return it->second;
@@ -69,8 +72,10 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
 }
 
 // ---------------------------------------------------------------------------
-BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double delta)
-    : L_(L), lat_(L.lat), eo_(eo), cap_(capacity), sp_(sloppy_prec), delta_(delta) {
+BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double delta,
+                         hipStream_t st)
+    : L_(L), lat_(L.lat), eo_(eo), cap_(capacity), sp_(sloppy_prec), delta_(delta),
+      st_(st ? st : stream()) {
   QOP_REQUIRE(eo == 0 || eo == 1, "batch solver: subset must be even or odd");
   QOP_REQUIRE(capacity >= 1 && capacity <= 256, "batch solver: capacity 1..256");
   QOP_REQUIRE(sp_ == P32 || sp_ == P16, "batch solver: sloppy precision must be FP32 or FP16");
@@ -92,7 +97,7 @@ BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double
   sel_h_ = active_h_ + cap_;
   QOP_CHECK_HIP(hipMalloc(&ru_, 2 * cap_ * sizeof(int)));
   ru_cont_ = ru_ + cap_;
-  QOP_CHECK_HIP(hipMemsetAsync(ru_, 0, 2 * cap_ * sizeof(int), stream()));
+  QOP_CHECK_HIP(hipMemsetAsync(ru_, 0, 2 * cap_ * sizeof(int), st_));
   QOP_CHECK_HIP(hipMalloc(&glob_, sizeof(BsGlobal)));
   QOP_CHECK_HIP(hipHostMalloc(&glob_h_, sizeof(BsGlobal), hipHostMallocDefault));
   QOP_CHECK_HIP(hipMalloc(&partA_, 3 * (size_t)cap_ * nch_ * sizeof(double)));
@@ -106,10 +111,9 @@ BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double
   memset(active_h_, 0, 2 * cap_ * sizeof(int));
   glob_h_->status = 0;
   glob_h_->pending = 0;  // the status word stays 0: slots never pause the batch
-  hipStream_t st = stream();
-  QOP_CHECK_HIP(hipMemcpyAsync(rhs_, rhs_h_, cap_ * sizeof(BsRhs), hipMemcpyHostToDevice, st));
-  QOP_CHECK_HIP(hipMemcpyAsync(active_, active_h_, 2 * cap_ * sizeof(int), hipMemcpyHostToDevice, st));
-  QOP_CHECK_HIP(hipMemcpyAsync(glob_, glob_h_, sizeof(BsGlobal), hipMemcpyHostToDevice, st));
+  QOP_CHECK_HIP(hipMemcpyAsync(rhs_, rhs_h_, cap_ * sizeof(BsRhs), hipMemcpyHostToDevice, st_));
+  QOP_CHECK_HIP(hipMemcpyAsync(active_, active_h_, 2 * cap_ * sizeof(int), hipMemcpyHostToDevice, st_));
+  QOP_CHECK_HIP(hipMemcpyAsync(glob_, glob_h_, sizeof(BsGlobal), hipMemcpyHostToDevice, st_));
   loaded_.assign(cap_, 0);
   mass_.assign(cap_, 0.0);
   HopShape hs = tune_hop(L_, sp_, cap_, ps_, ts_, 1 - eo_);
@@ -118,11 +122,15 @@ BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double
   remap_ = hs.remap;
   nt_ = hs.nt;
   inner_ = hs.rhs_inner;
+  L_.get(P64);
+  // allocations are cleared and links converted on the library stream
   sync();
+  QOP_CHECK_HIP(hipStreamSynchronize(st_));
 }
 
 BatchSolver::~BatchSolver() {
-  sync();
+  QOP_CHECK_HIP(hipStreamSynchronize(st_));
+  if (ev_) (void)hipEventDestroy(ev_);
   if (graph_) (void)hipGraphExecDestroy(graph_);
   for (DevBatch *b : {&x64_, &r64_, &b64_, &ax64_, &t64_, &rs_, &ps_, &ts_, &xs_}) devbatch_free(*b);
   (void)hipFree(rhs_);
@@ -174,7 +182,7 @@ static void batch_lincomb(Lattice *lat, int cap, const DevBatch &out, int nin, c
 }
 
 void BatchSolver::apply_A64(const int *mask) {
-  hipStream_t st = stream();
+  hipStream_t st = st_;
   LinkSet &ls = L_.get(P64);
   const int opp = 1 - eo_;
   HopArgs a;
@@ -207,7 +215,7 @@ void BatchSolver::load(int slot, const DevVec &src, const DevVec *x0, double mas
                        int max_iter) {
   QOP_REQUIRE(slot >= 0 && slot < cap_ && !loaded_[slot], "batch solver: slot not free");
   QOP_REQUIRE(src.lat == lat_, "batch solver: lattice mismatch");
-  hipStream_t st = stream();
+  hipStream_t st = st_;
   const long Vh = lat_->Vh;
   auto copy_half = [&](VecView dst, int pdst, VecView s, int ps_) {
     LincombArgs a;
@@ -272,7 +280,7 @@ void BatchSolver::load(int slot, const DevVec &src, const DevVec *x0, double mas
     QOP_CHECK_HIP(hipMemsetAsync(xs_.slot(slot).data, 0, 3 * Vh * sizeof(float2), st));
   }
   QOP_CHECK_HIP(hipMemcpyAsync(sums_h_, sums_, 2 * cap_ * sizeof(double), hipMemcpyDeviceToHost, st));
-  sync();
+  QOP_CHECK_HIP(hipStreamSynchronize(st_));
   BsRhs &R = rhs_h_[slot];
   memset(&R, 0, sizeof R);
   R.m2 = mass * mass;
@@ -289,11 +297,11 @@ void BatchSolver::load(int slot, const DevVec &src, const DevVec *x0, double mas
   active_h_[slot] = done ? 0 : 1;
   QOP_CHECK_HIP(hipMemcpyAsync(rhs_ + slot, &R, sizeof R, hipMemcpyHostToDevice, st));
   QOP_CHECK_HIP(hipMemcpyAsync(active_ + slot, active_h_ + slot, sizeof(int), hipMemcpyHostToDevice, st));
-  sync();
+  QOP_CHECK_HIP(hipStreamSynchronize(st_));
 }
 
 void BatchSolver::enqueue_iteration() {
-  hipStream_t st = stream();
+  hipStream_t st = st_;
   LinkSet &ls = L_.get(sp_);
   const int opp = 1 - eo_;
   const long Vh = lat_->Vh;
@@ -349,7 +357,7 @@ void BatchSolver::build_graph() {
   const int chunk = options().chunk > 0 ? options().chunk : 16;
   if (graph_ && graph_chunk_ == chunk) return;
   if (graph_) QOP_CHECK_HIP(hipGraphExecDestroy(graph_));
-  hipStream_t st = stream();
+  hipStream_t st = st_;
   hipGraph_t g;
   QOP_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   for (int c = 0; c < chunk; c++) enqueue_iteration();
@@ -359,27 +367,48 @@ void BatchSolver::build_graph() {
   graph_chunk_ = chunk;
 }
 
-void BatchSolver::step(std::vector<int> &done) {
-  hipStream_t st = stream();
+void BatchSolver::launch_chunk() {
+  QOP_REQUIRE(!inflight_, "batch solver: a chunk is already in flight");
   bool any = false;
   for (int i = 0; i < cap_; i++) any |= active_h_[i] != 0;
-  if (any) {
-    if (options().use_graph) {
-      build_graph();
-      QOP_CHECK_HIP(hipGraphLaunch(graph_, st));
-    } else {
-      const int chunk = options().chunk > 0 ? options().chunk : 16;
-      for (int c = 0; c < chunk; c++) enqueue_iteration();
-    }
-    // slots finish on the device (bs_ru_scalar clears active[b])
-    QOP_CHECK_HIP(hipMemcpyAsync(active_h_, active_, cap_ * sizeof(int), hipMemcpyDeviceToHost, st));
-    sync();
+  if (!any) return;
+  if (options().use_graph) {
+    build_graph();
+    QOP_CHECK_HIP(hipGraphLaunch(graph_, st_));
+  } else {
+    const int chunk = options().chunk > 0 ? options().chunk : 16;
+    for (int c = 0; c < chunk; c++) enqueue_iteration();
+  }
+  // slots finish on the device (bs_ru_scalar clears active[b])
+  QOP_CHECK_HIP(hipMemcpyAsync(active_h_, active_, cap_ * sizeof(int), hipMemcpyDeviceToHost, st_));
+  if (!ev_) QOP_CHECK_HIP(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
+  QOP_CHECK_HIP(hipEventRecord(ev_, st_));
+  inflight_ = true;
+}
+
+bool BatchSolver::chunk_done() {
+  if (!inflight_) return true;
+  hipError_t e = hipEventQuery(ev_);
+  if (e == hipErrorNotReady) return false;
+  QOP_CHECK_HIP(e);
+  return true;
+}
+
+void BatchSolver::collect(std::vector<int> &done) {
+  if (inflight_) {
+    QOP_CHECK_HIP(hipEventSynchronize(ev_));
+    inflight_ = false;
   }
   for (int i = 0; i < cap_; i++)
     if (loaded_[i] == 1 && !active_h_[i]) {
       loaded_[i] = 2;  // finished, not yet released
       done.push_back(i);
     }
+}
+
+void BatchSolver::step(std::vector<int> &done) {
+  launch_chunk();
+  collect(done);
 }
 
 void BatchSolver::run_all(std::vector<int> &done) {
@@ -389,7 +418,7 @@ void BatchSolver::run_all(std::vector<int> &done) {
 
 void BatchSolver::result(int slot, DevVec &out, bool times_mass, SlotResult *res) {
   QOP_REQUIRE(slot >= 0 && slot < cap_ && loaded_[slot], "batch solver: slot not loaded");
-  hipStream_t st = stream();
+  hipStream_t st = st_;
   LincombArgs a;
   a.Vh = (int)lat_->Vh;
   a.n = 1;
@@ -398,7 +427,7 @@ void BatchSolver::result(int slot, DevVec &out, bool times_mass, SlotResult *res
   a.cr[0] = times_mass ? mass_[slot] : 1.0;
   int pin[1] = {P64};
   launch_lincomb(a, out.prec, pin, 1, false, st);
-  sync();
+  QOP_CHECK_HIP(hipStreamSynchronize(st_));
   if (res) {
     QOP_CHECK_HIP(hipMemcpy(rhs_h_ + slot, rhs_ + slot, sizeof(BsRhs), hipMemcpyDeviceToHost));
     const BsRhs &R = rhs_h_[slot];
@@ -413,7 +442,7 @@ void BatchSolver::release(int slot) {
   QOP_REQUIRE(slot >= 0 && slot < cap_, "batch solver: bad slot");
   loaded_[slot] = 0;
   active_h_[slot] = 0;
-  QOP_CHECK_HIP(hipMemcpyAsync(active_ + slot, active_h_ + slot, sizeof(int), hipMemcpyHostToDevice, stream()));
+  QOP_CHECK_HIP(hipMemcpyAsync(active_ + slot, active_h_ + slot, sizeof(int), hipMemcpyHostToDevice, st_));
 }
 
 // ---------------------------------------------------------------------------

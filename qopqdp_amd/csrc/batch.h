@@ -70,7 +70,9 @@ struct SlotResult {
 
 class BatchSolver {
  public:
-  BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double delta);
+  // st: stream the solver's work runs on (nullptr: the library stream)
+  BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double delta,
+              hipStream_t st = nullptr);
   ~BatchSolver();
   BatchSolver(const BatchSolver &) = delete;
   BatchSolver &operator=(const BatchSolver &) = delete;
@@ -85,6 +87,16 @@ class BatchSolver {
   // Run one chunk of iterations and service reliable updates.  Slots that
   // finished are appended to `done`.
   void step(std::vector<int> &done);
+  // step() in two halves: launch_chunk() enqueues one chunk and returns at
+  // once; after chunk_done() reports true, collect() lists finished slots.
+  // Nothing else may be called on the solver in between.
+  void launch_chunk();
+  bool chunk_done();
+  void collect(std::vector<int> &done);
+  bool in_flight() const { return inflight_; }
+  hipStream_t stream_of() const { return st_; }
+  int eo() const { return eo_; }
+  Lattice *lattice() const { return lat_; }
   // Write mass * x (the m/A normalisation) or x into out's eo half.
   void result(int slot, DevVec &out, bool times_mass, SlotResult *res);
   void release(int slot);
@@ -101,6 +113,9 @@ class BatchSolver {
   Lattice *lat_;
   int eo_, cap_, sp_;
   double delta_;
+  hipStream_t st_;
+  hipEvent_t ev_ = nullptr;
+  bool inflight_ = false;
   int nch_;
   // FP64 per slot
   DevBatch x64_, r64_, b64_, ax64_, t64_;
@@ -122,7 +137,8 @@ class BatchSolver {
   int nb_ = 4, block_ = 256, remap_ = 1, nt_ = 0, inner_ = 0;  // tuned hop launch shape
 };
 
-// Tuned hop launch shape for (lattice, precision, nl, nrhs); see tune.cpp.
+// Tuned hop launch shape for (lattice, precision, nl, nrhs), timed on first use
+// and cached (batch.cpp).
 struct HopShape {
   int nb = 1, block = 256, remap = 1, nt = 0, rhs_inner = 0;
 };

@@ -27,3 +27,14 @@ struct QOP_D3_FermionLinksAsqtad_struct {
 struct QOP_F3_FermionLinksAsqtad_struct {
   qop::Links L;
 };
+
+namespace qop {
+// Link preparation shared by the QOP entry points and the request server
+// (api.cpp).  links_from_raw64: FP64 raw fat/long links (host, already
+// rephased and smeared, lng may be null).  links_from_gauge64: raw FP64 gauge
+// field (host) -> device, rephase (r0/bc/sign as QOP_D3_rephase_G; null
+// skips), asqtad smearing with coefficients c.
+void links_from_raw64(Links &L, double *fat[4], double *lng[4]);
+void links_from_gauge64(Links &L, double *const U_host[4], const QOP_asqtad_coeffs_t *c, int *r0,
+                        QOP_bc_t *bc, QOP_staggered_sign_t *sign);
+}  // namespace qop

@@ -92,14 +92,20 @@ void linkset_free(LinkSet &l) {
 LinkSet &Links::get(int prec) {
   if (prec == P64) return l64;
   LinkSet &d = prec == P32 ? l32 : l16;
-  if (!d.data) convert_links(lat, l64, d, prec, stream());
+  if (!d.data) {
+    convert_links(lat, l64, d, prec, stream());
+    sync();  // the converted set may be read from any stream
+  }
   return d;
 }
 
-static hipStream_t g_stream = nullptr;
 hipStream_t stream() {
-  if (!g_stream) QOP_CHECK_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
-  return g_stream;
+  static hipStream_t s = [] {
+    hipStream_t t;
+    QOP_CHECK_HIP(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+    return t;
+  }();
+  return s;
 }
 void sync() { QOP_CHECK_HIP(hipStreamSynchronize(stream())); }
 

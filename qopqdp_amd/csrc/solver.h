@@ -45,7 +45,7 @@ void cgms_launch_relnorm(CgmsDev d, int prec, int Vh, VecView r, VecView x,
 struct Options {
   int style = 1, nsvec = 8, nvec = 8, cgtype = 0;
   int eigcg_nev = 10, eigcg_m = 100, eigcg_numax = 200;
-  // library extensions (tags "qmp_solver", "qmp_prec", ...)
+  // library extensions (tags "qmx_solver", "qmx_prec", ...)
   int solver = 0;        // 0: reference algorithm; 1: reliable-update mixed precision
   int inner_prec = P32;  // inner precision of the mixed solver
   double delta = 0.1;    // reliable-update residual drop
