@@ -14,9 +14,11 @@ latencies (step start to that solve's completion).  Beside it the plain FP64
 GPU CG with the reference algorithm and stopping rule is timed on the same
 lattice and sources, and the FP64 true residual of every solve is checked.
 
-roofline: the dominant kernel of the step, the multi-RHS FP16 hop of the CG
-iteration (first hop, fused |D p|^2), timed with HIP events on the library
-stream; algorithmic bytes per launch = (16*40 + 2*B*16) * Vh.
+roofline: the dominant kernel of the step, the second multi-RHS FP16 hop of
+the CG iteration with the CG update fused into it, timed with HIP events on
+the library stream; algorithmic bytes per launch = (16*40 + B*112) * Vh
+(hop_bytes_per_site).  traffic: HBM bytes per launch measured with rocprofv3
+FETCH_SIZE / WRITE_SIZE (profiles/traffic.json).
 cpu_baseline: the C oracle's FP64 CG (the reference algorithm) on the host
 cores for a bounded number of iterations, extrapolated to the GPU FP64
 solve's iteration count.
@@ -90,16 +92,18 @@ def main():
     log(f"setup {time.time() - t0:.1f}s (fattening {finfo.final_sec * 1e3:.1f} ms on GPU)")
 
     if args.roofline_only:
-        shape = (C.c_int * 5)()
-        ms = s.lib.QOPX_D3_asqtad_hop_time(fla, args.prec, B, 50, 1, shape)
-        s_link, s_vec = (40, 16) if args.prec == 16 else (72, 24)
-        byts = (16 * s_link + 2 * B * s_vec) * Vh
+        res = {}
+        for red in (1, 2):
+            shape = (C.c_int * 5)()
+            ms = s.lib.QOPX_D3_asqtad_hop_time(fla, args.prec, B, 50, red, shape)
+            byts = hop_bytes_per_site(args.prec, B, red) * Vh
+            res[f"red{red}"] = {"kernel_us": round(ms * 1e3, 1), "GBs": round(byts / ms / 1e6, 1),
+                                "shape": list(shape), "algorithmic_bytes": byts}
         cbytes = 1 << 30
         cms = s.lib.QOPX_copy_time(cbytes, 20)
-        print(json.dumps({"kernel_us": round(ms * 1e3, 1), "GBs": round(byts / ms / 1e6, 1),
-                          "shape": list(shape), "algorithmic_bytes": byts,
-                          "copy_bytes_read": cbytes, "copy_us": round(cms * 1e3, 1),
-                          "copy_GBs_rw": round(2 * cbytes / cms / 1e6, 1)}))
+        res["copy"] = {"bytes_read": cbytes, "us": round(cms * 1e3, 1),
+                       "GBs_rw": round(2 * cbytes / cms / 1e6, 1)}
+        print(json.dumps(res))
         s.lib.QOP_D3_asqtad_destroy_L(fla)
         s.close()
         return
@@ -176,13 +180,17 @@ def main():
         log(f"FP64 CG solve {j}: {fp64_t[-1]:.3f}s, {ra.final_iter} iterations, rsq {ra.final_rsq:.2e}")
     fp64_rate = len(fp64_t) / sum(fp64_t)
 
-    # dominant kernel roofline: multi-RHS hop (first hop of the iteration)
-    shape = (C.c_int * 5)()
+    # dominant kernel roofline: the second hop of the batched CG iteration
+    # (fused CG update); the first hop beside it
+    shape, shape1 = (C.c_int * 5)(), (C.c_int * 5)()
     nrep = 50
-    hop_ms = s.lib.QOPX_D3_asqtad_hop_time(fla, args.prec, B, nrep, 1, shape)
-    s_link, s_vec = (40, 16) if args.prec == 16 else (72, 24)
-    hop_bytes = (16 * s_link + 2 * B * s_vec) * Vh
+    hop_ms = s.lib.QOPX_D3_asqtad_hop_time(fla, args.prec, B, nrep, 2, shape)
+    hop_bytes = hop_bytes_per_site(args.prec, B, 2) * Vh
     achieved = hop_bytes / (hop_ms * 1e-3) / 1e9
+    hop1_ms = s.lib.QOPX_D3_asqtad_hop_time(fla, args.prec, B, nrep, 1, shape1)
+    hop1_bytes = hop_bytes_per_site(args.prec, B, 1) * Vh
+    kname = f"hop_kernel fp{args.prec} x{B} RHS, fused CG update"
+    traffic = measured_traffic(kname, L)
     # single-RHS bandwidth baseline (FP64 / FP32 / FP16 even->odd Dslash)
     single = {}
     for prec, sl, sv in ((64, 144, 48), (32, 72, 24), (16, 40, 16)):
@@ -231,19 +239,52 @@ def main():
         "single_rhs_dslash": single,
         "roofline": {
             "bound": "hbm",
-            "kernel": f"hop_kernel fp{args.prec} x{B} RHS (fused |Dp|^2)",
+            "kernel": kname,
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "launch_us": round(hop_ms * 1e3, 1),
             "algorithmic_bytes": hop_bytes,
             "shape": list(shape),
         },
+        "hop1": {
+            "kernel": f"hop_kernel fp{args.prec} x{B} RHS, fused |Dp|^2",
+            "launch_us": round(hop1_ms * 1e3, 1), "algorithmic_bytes": hop1_bytes,
+            "GBs": round(hop1_bytes / (hop1_ms * 1e-3) / 1e9, 1),
+            "frac": round(hop1_bytes / (hop1_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "shape": list(shape1),
+        },
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def hop_bytes_per_site(prec, nrhs, red):
+    """Algorithmic bytes per output site of one multi-RHS hop launch: 16
+    links (FP16: 9 half2 + a float scale = 40 B; FP32 72 B) read once, and per
+    right-hand side the gathered source vector (FP16 12 B + 4 B norm; FP32
+    24 B) and the written output.  red 2 (fused CG update) reads p, r and the
+    FP32 solution x and writes r and x instead of writing the hop output."""
+    s_link, s_vec = (40, 16) if prec == 16 else (72, 24)
+    if red == 2:
+        return 16 * s_link + nrhs * (s_vec + 3 * s_vec + 2 * 24)
+    return 16 * s_link + 2 * nrhs * s_vec
+
+
+def measured_traffic(kernel, L):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs summarised in profiles/traffic.json (null if absent)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f)
+        e = t[kernel]
+        if list(e["lattice"]) != list(L):
+            return None
+        return e["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def serve_bench(args):

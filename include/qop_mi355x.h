@@ -252,8 +252,10 @@ int QOPX_D3_asqtad_invert_multi_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqta
 
 /* Mean ms per launch of the even->odd hop kernel (precision 64/32/16, nrhs
  * right-hand sides, autotuned shape (rhs per thread, block, XCD remap,
- * non-temporal links, RHS-innermost order) returned in shape[5]; red != 0 times the
- * instance with the fused |out|^2 reduction), from HIP events. */
+ * non-temporal links, RHS-innermost order) returned in shape[5]), from HIP
+ * events.  red = 0: plain hop; 1: with the fused |out|^2 reduction (first hop
+ * of a batched CG iteration); 2: with the fused CG update
+ * r -= a (m^2 p - D t), x += a p, |r|^2 (second hop; precision 32/16). */
 double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, int nrep,
                                int red, int *shape);
 

@@ -14,10 +14,12 @@ extern "C" {
 
 // Time `nrep` launches of the even->odd hop kernel in precision prec
 // (64/32/16) on nrhs right-hand sides with the autotuned launch shape.
-// red != 0 selects the instance with the fused |out|^2 reduction (the first
-// hop of a CG iteration).  Returns the mean ms per launch from HIP events on
-// the library stream; shape[5] receives (rhs per thread, block, XCD remap, non-temporal links,
-// RHS-innermost block order).
+// red = 1 selects the instance with the fused |out|^2 reduction (the first
+// hop of a batched CG iteration), red = 2 the instance with the fused CG
+// update r -= a (m^2 p - D t), x_FP32 += a p, |r|^2 (the second hop; sloppy
+// precisions only).  Returns the mean ms per launch from
+// HIP events on the library stream; shape[5] receives (rhs per thread, block,
+// XCD remap, non-temporal links, RHS-innermost block order).
 double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, int nrep,
                                int red, int *shape) {
   Links &L = fla->L;
@@ -64,18 +66,40 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
   a.remap = hs.remap;
   a.nt = hs.nt;
   a.rhs_inner = hs.rhs_inner;
-  double *part = nullptr;
+  QOP_REQUIRE(red >= 0 && red <= 2 && (red != 2 || p != P64), "hop_time: bad reduction");
+  double *part = nullptr, *alpha = nullptr;
+  DevBatch xb, xs;
   if (red) {
     QOP_CHECK_HIP(hipMalloc(&part, (size_t)nrhs * ((lat->Vh + 63) / 64) * sizeof(double)));
     a.partial = part;
   }
+  if (red == 2) {  // operands of the fused CG update: p (x), r (dst), x_FP32
+    devbatch_alloc(xb, lat, p, nrhs);
+    devbatch_alloc(xs, lat, P32, nrhs);
+    QOP_CHECK_HIP(hipMalloc(&alpha, nrhs * sizeof(double)));
+    QOP_CHECK_HIP(hipMemsetAsync(alpha, 0, nrhs * sizeof(double), stream()));
+    QOP_CHECK_HIP(hipMemcpyAsync(xb.data, src.data, (size_t)nrhs * 3 * lat->Vh * prec_cbytes(p),
+                                 hipMemcpyDeviceToDevice, stream()));
+    if (p == P16)
+      QOP_CHECK_HIP(hipMemcpyAsync(xb.norm, src.norm, (size_t)nrhs * lat->Vh * sizeof(float),
+                                   hipMemcpyDeviceToDevice, stream()));
+    a.x = xb.slot(0);
+    a.xstride = 3 * lat->Vh;
+    a.xs = xs.slot(0);
+    a.cg_alpha = alpha;
+    a.cg_stride = 1;
+    a.alpha = -1.0;
+    a.beta = 1e-4;
+    sync();
+  }
+  const int rk = red == 2 ? RED_CG : red == 1 ? RED_OUT2 : RED_NONE;
   hipStream_t st = stream();
-  for (int w = 0; w < 3; w++) launch_hop_args(a, p, ls.nl, hs.nb, hs.block, red ? RED_OUT2 : RED_NONE, false, st);
+  for (int w = 0; w < 3; w++) launch_hop_args(a, p, ls.nl, hs.nb, hs.block, rk, red == 2, st);
   hipEvent_t e0, e1;
   QOP_CHECK_HIP(hipEventCreate(&e0));
   QOP_CHECK_HIP(hipEventCreate(&e1));
   QOP_CHECK_HIP(hipEventRecord(e0, st));
-  for (int r = 0; r < nrep; r++) launch_hop_args(a, p, ls.nl, hs.nb, hs.block, red ? RED_OUT2 : RED_NONE, false, st);
+  for (int r = 0; r < nrep; r++) launch_hop_args(a, p, ls.nl, hs.nb, hs.block, rk, red == 2, st);
   QOP_CHECK_HIP(hipEventRecord(e1, st));
   QOP_CHECK_HIP(hipEventSynchronize(e1));
   float ms = 0;
@@ -83,6 +107,11 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
   QOP_CHECK_HIP(hipEventDestroy(e0));
   QOP_CHECK_HIP(hipEventDestroy(e1));
   if (part) QOP_CHECK_HIP(hipFree(part));
+  if (alpha) QOP_CHECK_HIP(hipFree(alpha));
+  if (red == 2) {
+    devbatch_free(xb);
+    devbatch_free(xs);
+  }
   devbatch_free(src);
   devbatch_free(dst);
   if (shape) {
