@@ -122,6 +122,39 @@ typedef struct {
 } QOP_asqtad_coeffs_t; /* :150-157 */
 #define QOP_ASQTAD_COEFFS_ZERO ((QOP_asqtad_coeffs_t){0, 0, 0, 0, 0, 0})
 
+typedef enum { QOP_UNITARIZE_U3 = 0, QOP_UNITARIZE_SU3 = 1 } QOP_hisq_unitarize_group_t; /* :29-32 */
+typedef enum {
+  QOP_UNITARIZE_NONE = 0,
+  QOP_UNITARIZE_RATIONAL = 3,
+  QOP_UNITARIZE_ANALYTIC = 5
+} QOP_hisq_unitarize_method_t; /* :35-43 */
+#define QOP_MAX_NAIK 20
+#define QOP_EPS_NAIK_ZERO {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}
+typedef struct {
+  int n_naiks;
+  double eps_naik[QOP_MAX_NAIK];
+  QOP_hisq_unitarize_group_t ugroup;
+  QOP_hisq_unitarize_method_t umethod;
+  double fat7_one_link;
+  double fat7_three_staple;
+  double fat7_five_staple;
+  double fat7_seven_staple;
+  double fat7_lepage;
+  double asqtad_one_link;
+  double asqtad_three_staple;
+  double asqtad_five_staple;
+  double asqtad_seven_staple;
+  double asqtad_lepage;
+  double asqtad_naik;
+  double difference_one_link;
+  double difference_naik;
+} QOP_hisq_coeffs_t; /* :166-184 */
+#define QOP_HISQ_COEFFS_ZERO                                                   \
+  ((QOP_hisq_coeffs_t){1, QOP_EPS_NAIK_ZERO, QOP_UNITARIZE_U3, QOP_UNITARIZE_RATIONAL, \
+                       0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0})
+/* U(3) projections that fell back from the analytic formula (:115) */
+#define QOP_info_hisq_svd_counter(info) ((info)->count1)
+
 /* opaque handles (include/qop_pc.h:10-18) */
 typedef struct QOP_D3_ColorVector_struct QOP_D3_ColorVector;
 typedef struct QOP_F3_ColorVector_struct QOP_F3_ColorVector;
@@ -129,6 +162,8 @@ typedef struct QOP_D3_GaugeField_struct QOP_D3_GaugeField;
 typedef struct QOP_F3_GaugeField_struct QOP_F3_GaugeField;
 typedef struct QOP_D3_FermionLinksAsqtad_struct QOP_D3_FermionLinksAsqtad;
 typedef struct QOP_F3_FermionLinksAsqtad_struct QOP_F3_FermionLinksAsqtad;
+typedef struct QOP_D3_FermionLinksHisq_struct QOP_D3_FermionLinksHisq;
+typedef struct QOP_F3_FermionLinksHisq_struct QOP_F3_FermionLinksHisq;
 
 /* ---- general routines (include/qop_int.h:230-251; lib/common.c) ------ */
 QOP_status_t QOP_init(QOP_layout_t *layout);         /* lib/common.c:40-83 */
@@ -182,6 +217,22 @@ void QOP_D3_asqtad_load_L_from_G(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *as
 void QOP_D3_asqtad_destroy_L(QOP_D3_FermionLinksAsqtad *field); /* :941-968 */
 void QOP_F3_asqtad_destroy_L(QOP_F3_FermionLinksAsqtad *field);
 void QOP_D3_asqtad_rephase_L(QOP_D3_FermionLinksAsqtad *fla, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign); /* :1448-1482 */
+/* ---- HISQ links (include/qop_d3.h:474-485; lib/hisq_links_p.c:62-301) --
+ * fat7 smearing, U(3) projection, asqtad smearing of the projected links:
+ * get_asqtad_links_from_hisq returns n_naiks asqtad link objects (owned by
+ * the HISQ object) usable with every QOP_*_asqtad_* function. */
+QOP_status_t QOP_hisq_links_set_opts(QOP_opt_t opts[], int nopts); /* lib/hisq_links.c:31-77 */
+QOP_D3_FermionLinksHisq *QOP_D3_hisq_create_L_from_G(QOP_info_t *info, QOP_hisq_coeffs_t *coeffs,
+                                                     QOP_D3_GaugeField *gauge); /* :62-256 */
+QOP_F3_FermionLinksHisq *QOP_F3_hisq_create_L_from_G(QOP_info_t *info, QOP_hisq_coeffs_t *coeffs,
+                                                     QOP_F3_GaugeField *gauge);
+void QOP_D3_hisq_destroy_L(QOP_D3_FermionLinksHisq *field); /* :258-289 */
+void QOP_F3_hisq_destroy_L(QOP_F3_FermionLinksHisq *field);
+QOP_D3_FermionLinksAsqtad **QOP_D3_get_asqtad_links_from_hisq(QOP_D3_FermionLinksHisq *hl); /* :291-295 */
+QOP_F3_FermionLinksAsqtad **QOP_F3_get_asqtad_links_from_hisq(QOP_F3_FermionLinksHisq *hl);
+QOP_D3_FermionLinksAsqtad *QOP_D3_get_asqtad_deps_links_from_hisq(QOP_D3_FermionLinksHisq *hl); /* :297-301 */
+QOP_F3_FermionLinksAsqtad *QOP_F3_get_asqtad_deps_links_from_hisq(QOP_F3_FermionLinksHisq *hl);
+
 /* precision copies (asqtad_dslash_p.c:784-790, 832-838) */
 QOP_F3_FermionLinksAsqtad *QOP_FD3_asqtad_create_L_from_L(QOP_D3_FermionLinksAsqtad *fla_src);
 QOP_D3_FermionLinksAsqtad *QOP_DF3_asqtad_create_L_from_L(QOP_F3_FermionLinksAsqtad *fla_src);
@@ -361,6 +412,11 @@ const char *QOPX_last_error(void);
 #  define QOP_asqtad_diaginv QOP_D3_asqtad_diaginv
 #  define QOP_asqtad_invert QOP_D3_asqtad_invert
 #  define QOP_asqtad_invert_multi QOP_D3_asqtad_invert_multi
+#  define QOP_FermionLinksHisq QOP_D3_FermionLinksHisq
+#  define QOP_hisq_create_L_from_G QOP_D3_hisq_create_L_from_G
+#  define QOP_hisq_destroy_L QOP_D3_hisq_destroy_L
+#  define QOP_get_asqtad_links_from_hisq QOP_D3_get_asqtad_links_from_hisq
+#  define QOP_get_asqtad_deps_links_from_hisq QOP_D3_get_asqtad_deps_links_from_hisq
 #elif QOP_Precision == 'F'
 #  define QOP_Real QOP_F_Real
 #  define QOP_ColorVector QOP_F3_ColorVector
@@ -383,6 +439,11 @@ const char *QOPX_last_error(void);
 #  define QOP_asqtad_diaginv QOP_F3_asqtad_diaginv
 #  define QOP_asqtad_invert QOP_F3_asqtad_invert
 #  define QOP_asqtad_invert_multi QOP_F3_asqtad_invert_multi
+#  define QOP_FermionLinksHisq QOP_F3_FermionLinksHisq
+#  define QOP_hisq_create_L_from_G QOP_F3_hisq_create_L_from_G
+#  define QOP_hisq_destroy_L QOP_F3_hisq_destroy_L
+#  define QOP_get_asqtad_links_from_hisq QOP_F3_get_asqtad_links_from_hisq
+#  define QOP_get_asqtad_deps_links_from_hisq QOP_F3_get_asqtad_deps_links_from_hisq
 #else
 #  error "bad QOP_Precision"
 #endif

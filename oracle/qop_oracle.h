@@ -86,6 +86,21 @@ void or_invert_multi(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t *ra,
 /* Fermilab relative residual (invert_utilities_p.c:7-40). */
 double or_relnorm2(or_links_t *l, const double *r, const double *x, int eo);
 
+/* HISQ link preparation (hisq_oracle.c). */
+typedef struct {
+  int allow_svd, svd_only;
+  double svd_rel_error, svd_abs_error;
+} or_reunit_opts_t;
+/* W = V (V^dagger V)^-1/2 by the analytic cubic
+ * (hisq_reunit_utilities_p.c:97-297); returns 1 when the fallback (unitary
+ * polar factor from a Jacobi SVD) was taken; flops accumulated. */
+int or_u3_un_analytic(const double *V, double *W, const or_reunit_opts_t *o, double *flops);
+void or_u3_polar_jacobi(const double *V, double *W);
+/* fat7 smearing of U then U(3) projection of every link into W
+ * (hisq_links_p.c:62-152); returns the number of fallback projections. */
+int or_hisq_W(const int L[4], const or_asqtad_coeffs_t *fat7, double *const U[4], double *W[4],
+              const or_reunit_opts_t *o, double *flops);
+
 /* Number of iterations done by the last inner solver calls (diagnostics). */
 int or_set_verbose(int v);
 int or_num_threads(void);

@@ -59,6 +59,35 @@ class QOPOpt(C.Structure):
     _fields_ = [("tag", C.c_char_p), ("value", C.c_double), ("extra", C.c_void_p)]
 
 
+class QOPHisqCoeffs(C.Structure):
+    _fields_ = [("n_naiks", C.c_int), ("eps_naik", C.c_double * 20), ("ugroup", C.c_int),
+                ("umethod", C.c_int)] + [(n, C.c_double) for n in (
+                    "fat7_one_link", "fat7_three_staple", "fat7_five_staple", "fat7_seven_staple",
+                    "fat7_lepage", "asqtad_one_link", "asqtad_three_staple", "asqtad_five_staple",
+                    "asqtad_seven_staple", "asqtad_lepage", "asqtad_naik", "difference_one_link",
+                    "difference_naik")]
+
+
+# MILC HISQ coefficients (fat7 level, then the asqtad level 2 with
+# epsilon_Naik = 0); free-field fat weights 1 and 9/8
+MILC_HISQ = dict(fat7_one_link=1 / 8, fat7_three_staple=-1 / 16, fat7_five_staple=1 / 64,
+                 fat7_seven_staple=-1 / 384, fat7_lepage=0.0, asqtad_one_link=1.0,
+                 asqtad_three_staple=-1 / 16, asqtad_five_staple=1 / 64,
+                 asqtad_seven_staple=-1 / 384, asqtad_lepage=-1 / 8, asqtad_naik=-1 / 24,
+                 difference_one_link=0.0, difference_naik=-1 / 24)
+
+
+def hisq_coeffs(eps_naik=(0.0,), **kw):
+    c = QOPHisqCoeffs()
+    c.n_naiks = len(eps_naik)
+    for i, e in enumerate(eps_naik):
+        c.eps_naik[i] = e
+    c.ugroup, c.umethod = 0, 3
+    for k, v in {**MILC_HISQ, **kw}.items():
+        setattr(c, k, v)
+    return c
+
+
 class QOPAsqtadCoeffs(C.Structure):
     _fields_ = [("one_link", C.c_double), ("three_staple", C.c_double),
                 ("five_staple", C.c_double), ("seven_staple", C.c_double),
@@ -130,6 +159,15 @@ SIGNATURES = [
     ("QOP_D3_asqtad_destroy_L", None, [_VP]),
     ("QOP_F3_asqtad_destroy_L", None, [_VP]),
     ("QOP_D3_asqtad_rephase_L", None, [_VP, C.POINTER(C.c_int), C.POINTER(QOPBc), C.POINTER(QOPSign)]),
+    ("QOP_hisq_links_set_opts", C.c_int, [C.POINTER(QOPOpt), C.c_int]),
+    ("QOP_D3_hisq_create_L_from_G", _VP, [C.POINTER(QOPInfo), C.POINTER(QOPHisqCoeffs), _VP]),
+    ("QOP_F3_hisq_create_L_from_G", _VP, [C.POINTER(QOPInfo), C.POINTER(QOPHisqCoeffs), _VP]),
+    ("QOP_D3_hisq_destroy_L", None, [_VP]),
+    ("QOP_F3_hisq_destroy_L", None, [_VP]),
+    ("QOP_D3_get_asqtad_links_from_hisq", C.POINTER(_VP), [_VP]),
+    ("QOP_F3_get_asqtad_links_from_hisq", C.POINTER(_VP), [_VP]),
+    ("QOP_D3_get_asqtad_deps_links_from_hisq", _VP, [_VP]),
+    ("QOP_F3_get_asqtad_deps_links_from_hisq", _VP, [_VP]),
     ("QOP_FD3_asqtad_create_L_from_L", _VP, [_VP]),
     ("QOP_DF3_asqtad_create_L_from_L", _VP, [_VP]),
     ("QOP_D3_asqtad_dslash", None, [C.POINTER(QOPInfo), _VP, C.c_double, _VP, _VP, C.c_int, C.c_int]),

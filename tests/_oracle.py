@@ -28,6 +28,11 @@ class ResArg(C.Structure):
                 ("final_rel", C.c_double), ("final_iter", C.c_int), ("final_restart", C.c_int)]
 
 
+class ReunitOpts(C.Structure):
+    _fields_ = [("allow_svd", C.c_int), ("svd_only", C.c_int), ("svd_rel_error", C.c_double),
+                ("svd_abs_error", C.c_double)]
+
+
 _D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _PP = C.POINTER(C.c_void_p)
 _IV = C.POINTER(C.c_int)
@@ -74,6 +79,12 @@ class Oracle:
                                           _D, _IV, _PP, _PP, C.c_int, C.POINTER(C.c_double)]
             L.or_relnorm2.restype = C.c_double
             L.or_relnorm2.argtypes = [C.c_void_p, _D, _D, C.c_int]
+            L.or_u3_un_analytic.restype = C.c_int
+            L.or_u3_un_analytic.argtypes = [_D, _D, C.POINTER(ReunitOpts), C.POINTER(C.c_double)]
+            L.or_u3_polar_jacobi.argtypes = [_D, _D]
+            L.or_hisq_W.restype = C.c_int
+            L.or_hisq_W.argtypes = [_IV, C.POINTER(Coeffs), _PP, _PP, C.POINTER(ReunitOpts),
+                                    C.POINTER(C.c_double)]
             L.or_set_verbose.restype = C.c_int
             L.or_num_threads.restype = C.c_int
             Oracle._lib = L
@@ -180,6 +191,39 @@ class Oracle:
         """|src - D out| / |src| over all sites."""
         Dx = self.dslash(h, mass, out, EVENODD, EVENODD)
         return float(np.linalg.norm(src - Dx) / np.linalg.norm(src))
+
+    # HISQ preparation
+    @staticmethod
+    def reunit_opts(allow_svd=1, svd_only=0, rel=1e-8, abs_=1e-8):
+        return ReunitOpts(allow_svd, svd_only, rel, abs_)
+
+    def u3_un_analytic(self, V, opts=None):
+        """V: (3,3) complex -> (W, used_fallback, flops)"""
+        o = opts or self.reunit_opts()
+        v = np.ascontiguousarray(np.stack([V.real, V.imag], -1), dtype=np.float64).reshape(-1)
+        w = np.zeros(18)
+        fl = C.c_double(0)
+        svd = self.lib.or_u3_un_analytic(v, w, C.byref(o), C.byref(fl))
+        w = w.reshape(3, 3, 2)
+        return w[..., 0] + 1j * w[..., 1], svd, fl.value
+
+    def u3_polar_jacobi(self, V):
+        v = np.ascontiguousarray(np.stack([V.real, V.imag], -1), dtype=np.float64).reshape(-1)
+        w = np.zeros(18)
+        self.lib.or_u3_polar_jacobi(v, w)
+        w = w.reshape(3, 3, 2)
+        return w[..., 0] + 1j * w[..., 1]
+
+    def hisq_W(self, L, U, fat7, opts=None):
+        """fat7 smearing + U(3) projection: list of 4 (V,3,3,2) arrays, #fallbacks, flops"""
+        V = int(np.prod(L))
+        W = [np.zeros((V, 3, 3, 2)) for _ in range(4)]
+        o = opts or self.reunit_opts()
+        c = Coeffs(**fat7)
+        fl = C.c_double(0)
+        n = self.lib.or_hisq_W(_iv(L), C.byref(c), _pa([np.ascontiguousarray(u) for u in U]), _pa(W),
+                               C.byref(o), C.byref(fl))
+        return W, n, fl.value
 
     def threads(self):
         return self.lib.or_num_threads()

@@ -17,6 +17,7 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
 def declared_functions():
     src = open(HEADER).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    src = re.sub(r"^#.*?(?<!\\)$", "", src, flags=re.M | re.S)  # preprocessor lines (macros)
     names = set(re.findall(r"\b(QOPX?_\w+)\s*\(", src))
     return {n for n in names if not n.endswith("_DEFAULT") and not n.endswith("_ZERO")}
 
