@@ -65,6 +65,12 @@ def main():
     ap.add_argument("--serve-rates", default="0.5,1,2,4",
                     help="arrival rates as multiples of the FP64 one-at-a-time capacity")
     ap.add_argument("--serve-capacity", type=int, default=16)
+    ap.add_argument("--multi", action="store_true",
+                    help="batched multi-mass solves (16 sources x 8 masses, 24^3x48) instead")
+    ap.add_argument("--hisq", action="store_true",
+                    help="HISQ links, one source x 12 masses at 48^3x96 instead")
+    ap.add_argument("--fp64-sources", type=int, default=16,
+                    help="--multi: sequential FP64 multi-mass solves timed for the baseline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the build has no "
@@ -72,6 +78,9 @@ def main():
         sys.exit(2)
     if args.serve:
         serve_bench(args)
+        return
+    if args.multi or args.hisq:
+        multi_bench(args)
         return
 
     import qopqdp_amd as q
@@ -94,7 +103,7 @@ def main():
     if args.roofline_only:
         res = {}
         for red in (1, 2):
-            shape = (C.c_int * 5)()
+            shape = (C.c_int * 6)()
             ms = s.lib.QOPX_D3_asqtad_hop_time(fla, args.prec, B, 50, red, shape)
             byts = hop_bytes_per_site(args.prec, B, red) * Vh
             res[f"red{red}"] = {"kernel_us": round(ms * 1e3, 1), "GBs": round(byts / ms / 1e6, 1),
@@ -182,7 +191,7 @@ def main():
 
     # dominant kernel roofline: the second hop of the batched CG iteration
     # (fused CG update); the first hop beside it
-    shape, shape1 = (C.c_int * 5)(), (C.c_int * 5)()
+    shape, shape1 = (C.c_int * 6)(), (C.c_int * 6)()
     nrep = 50
     hop_ms = s.lib.QOPX_D3_asqtad_hop_time(fla, args.prec, B, nrep, 2, shape)
     hop_bytes = hop_bytes_per_site(args.prec, B, 2) * Vh
@@ -285,6 +294,135 @@ def measured_traffic(kernel, L):
         return e["bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
+
+
+def multi_bench(args):
+    """Multi-mass solves, every (source, mass) to FP64 true residual 1e-10.
+    --multi: 24^3x48 asqtad links, 16 Gaussian even-site sources, 8 masses
+    log-spaced 0.005..0.2; the batched multi-shift + reliable-update solver
+    against sequential FP64 multi-mass solves (reference algorithm) of the
+    same sources.  --hisq: 48^3x96 HISQ links (fat7, U(3) projection, level 2,
+    epsilon_Naik = 0), FP16 links with per-link scales, one source, 12 masses
+    log-spaced 0.002..0.5, the FP64 true residual of every shift reported."""
+    import qopqdp_amd as q
+    if args.hisq:
+        L, nsrc, masses = [48, 48, 48, 96], 1, list(np.geomspace(0.002, 0.5, 12))
+    else:
+        L, nsrc, masses = [24, 24, 24, 48], 16, list(np.geomspace(0.005, 0.2, 8))
+    V = int(np.prod(L))
+    Vh = V // 2
+    t0 = time.time()
+    U = q.gen_gauge(L, 1000 + 10 * 4 + 0, 0.3)
+    s = q.Session(L)
+    g = s.gauge(U)
+    del U
+    r0, bc, sign, keep = q.milc_rephase_args(L)
+    s.lib.QOP_D3_rephase_G(g, r0, C.byref(bc), C.byref(sign))
+    hisq = None
+    if args.hisq:
+        info = q.QOPInfo()
+        hisq = s.lib.QOP_D3_hisq_create_L_from_G(C.byref(info), C.byref(q.hisq_coeffs()), g)
+        fla = s.lib.QOP_D3_get_asqtad_links_from_hisq(hisq)[0]
+        prep = {"sec": round(info.final_sec, 3), "svd_fallbacks": info.count1,
+                "gflops": round(info.final_flop / info.final_sec / 1e9, 1)}
+    else:
+        fla, finfo = s.links_from_gauge(g)
+        prep = {"sec": round(finfo.final_sec, 3)}
+    s.lib.QOP_D3_destroy_G(g)
+    log(f"setup {time.time() - t0:.1f}s, links {prep}")
+    srcs = [q.gen_source(L, 2000 + 100 * 4 + j, q.QOP_EVEN) for j in range(nsrc)]
+    vins = [s.vec(x) for x in srcs]
+    nm = len(masses)
+    n = nsrc * nm
+    P = C.c_void_p
+
+    def batched():
+        vouts = [s.zeros() for _ in range(n)]
+        ras = [q.resid_arg_default() for _ in range(n)]
+        for r in ras:
+            r.rsqmin = args.tol * args.tol
+        ia = q.invert_arg_default()
+        ia.evenodd = q.QOP_EVEN
+        ia.max_iter = 100000
+        ms_it = (C.c_int * nsrc)()
+        info = q.QOPInfo()
+        ts = time.perf_counter()
+        ok = s.lib.QOPX_D3_asqtad_invert_multi_batch(
+            C.byref(info), fla, C.byref(ia), (P * n)(*[C.cast(C.pointer(r), P) for r in ras]),
+            (C.c_double * nm)(*masses), nm, (P * n)(*vouts), (P * nsrc)(*vins), nsrc,
+            min(n, 16), args.prec, 0.0, ms_it)
+        dt = time.perf_counter() - ts
+        return ok, dt, vouts, ras, list(ms_it)
+
+    batched()  # warmup: autotuning and graph capture
+    ok, dt, vouts, ras, ms_it = batched()
+    log(f"batched: {ok}/{n} converged in {dt:.2f}s, multi-shift iterations {ms_it[:4]}")
+    # FP64 true residual of every (source, mass) with the FP64 GPU operator
+    tr = []
+    tmp, Ax = s.zeros(), s.zeros()
+    info = q.QOPInfo()
+    for i in range(nsrc):
+        for j, m in enumerate(masses):
+            x = s.get(vouts[i * nm + j]) / m
+            vx = s.vec(x)
+            s.lib.QOP_D3_asqtad_dslash(info, fla, 0.0, tmp, vx, q.QOP_ODD, q.QOP_EVEN)
+            s.lib.QOP_D3_asqtad_dslash(info, fla, 0.0, Ax, tmp, q.QOP_EVEN, q.QOP_ODD)
+            r = srcs[i][:Vh] - (m * m * x - s.get(Ax))[:Vh]
+            tr.append(float(np.linalg.norm(r) / np.linalg.norm(srcs[i][:Vh])))
+            s.free(vx)
+    for v in vouts + [tmp, Ax]:
+        s.free(v)
+    per_shift = [max(tr[i * nm + j] for i in range(nsrc)) for j in range(nm)]
+    # FP64 reference-algorithm multi-mass solves, one source at a time
+    nfp = min(args.fp64_sources, nsrc)
+    fp_t, fp_ok = [], 0
+    for i in range(nfp):
+        outs = [s.zeros() for _ in range(nm)]
+        rl = [q.resid_arg_default() for _ in range(nm)]
+        for r in rl:
+            r.rsqmin = args.tol * args.tol
+        ia = q.invert_arg_default()
+        ia.evenodd = q.QOP_EVEN
+        ia.max_iter = 100000
+        ia.max_restarts = 50
+        ra_p = (C.POINTER(q.QOPResidArg) * nm)(*[C.pointer(r) for r in rl])
+        out_p = (P * nm)(*outs)
+        ts = time.perf_counter()
+        s.lib.QOP_D3_asqtad_invert_multi(
+            C.byref(info), fla, C.byref(ia), (P * 1)(C.cast(ra_p, P)),
+            (P * 1)(C.cast((C.c_double * nm)(*masses), P)), (C.c_int * 1)(nm),
+            (P * 1)(C.cast(out_p, P)), (P * 1)(vins[i]), 1)
+        fp_t.append(time.perf_counter() - ts)
+        fp_ok += sum(r.final_rsq <= args.tol * args.tol * 1.0001 for r in rl)
+        log(f"FP64 multi-mass source {i}: {fp_t[-1]:.2f}s, iters {[r.final_iter for r in rl][:3]}, "
+            f"max rsq {max(r.final_rsq for r in rl):.2e}")
+        for o in outs:
+            s.free(o)
+    for v in vins:
+        s.free(v)
+    if hisq:
+        s.lib.QOP_D3_hisq_destroy_L(hisq)
+    else:
+        s.lib.QOP_D3_asqtad_destroy_L(fla)
+    s.close()
+    fp_rate = nm * len(fp_t) / sum(fp_t)
+    rate = n / dt
+    print(json.dumps({
+        "metric": "multi-mass solves/s at 1e-10 FP64 true residual per (source, mass), "
+                  "batched mixed precision vs sequential FP64 multi-shift CG",
+        "value": round(rate, 3), "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+        "data": "synthetic",
+        "dtype": f"fp{args.prec} storage, fp32 arithmetic, fp64 reductions/recurrences/true residual",
+        "config": {"workload": ("HISQ" if args.hisq else "asqtad") + f" {'x'.join(map(str, L))}, "
+                   f"{nsrc} sources x {nm} masses", "lattice": L, "sources": nsrc,
+                   "masses": [round(m, 8) for m in masses], "tol": args.tol},
+        "links": prep, "seconds": round(dt, 3), "converged": ok, "solves": n,
+        "multi_shift_iters": ms_it, "max_fp64_true_resid_per_mass": per_shift,
+        "fp64_sequential": {"solves_per_s": round(fp_rate, 3), "sources_timed": len(fp_t),
+                            "seconds_per_source": round(float(np.mean(fp_t)), 3),
+                            "converged": int(fp_ok)},
+        "speedup_vs_fp64": round(rate / fp_rate, 2),
+    }), flush=True)
 
 
 def serve_bench(args):

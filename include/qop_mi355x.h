@@ -303,12 +303,18 @@ int QOPX_D3_asqtad_invert_multi_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqta
 
 /* Mean ms per launch of the even->odd hop kernel (precision 64/32/16, nrhs
  * right-hand sides, autotuned shape (rhs per thread, block, XCD remap,
- * non-temporal links, RHS-innermost order) returned in shape[5]), from HIP
+ * non-temporal links, RHS-innermost order, x0 tile) returned in shape[6]), from HIP
  * events.  red = 0: plain hop; 1: with the fused |out|^2 reduction (first hop
  * of a batched CG iteration); 2: with the fused CG update
  * r -= a (m^2 p - D t), x += a p, |r|^2 (second hop; precision 32/16). */
 double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, int nrep,
                                int red, int *shape);
+
+/* Force the multi-RHS hop launch shape {rhs per thread, block, XCD remap,
+ * non-temporal links, RHS-innermost order, x0 tile} of solvers built from
+ * now on, where it is valid for their lattice and precision (tests of
+ * launch-shape independence); null restores autotuning. */
+void QOPX_hop_shape_override(const int *shape);
 
 /* Mean ms per launch of a plain device copy of `bytes` (multiple of 16):
  * the measured copy bandwidth and the calibration kernel for HBM counters. */

@@ -192,6 +192,7 @@ SIGNATURES = [
                                                     C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int)]),
     ("QOPX_D3_asqtad_hop_time", C.c_double, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     ("QOPX_copy_time", C.c_double, [C.c_ulonglong, C.c_int]),
+    ("QOPX_hop_shape_override", None, [C.POINTER(C.c_int)]),
     ("QOPX_serve_create", _VP, [C.c_int, C.c_int, C.c_ulonglong]),
     ("QOPX_serve_add_gauge", C.c_int, [_VP, C.POINTER(C.c_int), _PP, C.POINTER(QOPAsqtadCoeffs),
                                        C.POINTER(C.c_int), C.POINTER(QOPBc), C.POINTER(QOPSign)]),

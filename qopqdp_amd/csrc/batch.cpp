@@ -13,10 +13,32 @@ namespace qop {
 
 // ---------------------------------------------------------------------------
 // autotuner: time the multi-RHS hop for each launch shape and keep the best.
+static std::mutex g_tune_mu;  // solvers may be built from several host threads
+static bool g_override = false;
+static HopShape g_override_shape;
+
+// A forced launch shape (tests of shape independence); invalid for a given
+// lattice or precision it is ignored there.
+void set_hop_shape_override(const HopShape *s) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_override = s != nullptr;
+  if (s) g_override_shape = *s;
+}
+
+static bool shape_valid(const Lattice *lat, int prec, int nl, int nrhs, const HopShape &s) {
+  const int slab = (int)(lat->Vh / lat->L[0]);
+  if (s.nb != 1 && s.nb != 2 && s.nb != 4 && s.nb != 8 && s.nb != 16) return false;
+  if (s.nb == 16 && (prec == P64 || nl != 16)) return false;
+  if (s.nb > 1 && s.nb > nrhs) return false;
+  if (s.block != 64 && s.block != 128 && s.block != 256) return false;
+  if (s.tile && (s.tile % s.block != 0 || slab % s.tile != 0 || s.tile >= slab)) return false;
+  return true;
+}
+
 HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &dst, int eo_out) {
   static std::map<std::tuple<const Lattice *, int, int, int>, HopShape> cache;
-  static std::mutex mu;  // solvers may be built from several host threads
-  std::lock_guard<std::mutex> lk(mu);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (g_override && shape_valid(L.lat, prec, L.nl, nrhs, g_override_shape)) return g_override_shape;
   auto key = std::make_tuple((const Lattice *)L.lat, prec, L.nl, nrhs);
   auto it = cache.find(key);
   if (it != cache.end()) return it->second;
@@ -40,6 +62,8 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
   float best_ms = 1e30f;
   const int nbs[] = {1, 2, 4, 8, 16};
   const int blocks[] = {64, 128, 256};
+  const int tiles[] = {0, 512, 1024, 2048, 4096};
+  const int slab = (int)(lat->Vh / lat->L[0]);
   const int reps = lat->Vh > 200000 ? 4 : 10;
   for (int nb : nbs) {
     if (nb > 1 && nb > nrhs) continue;
@@ -47,11 +71,12 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
     for (int blk : blocks)
       for (int rm = 0; rm < 2; rm++)
       for (int nt = 0; nt < 2; nt++)
-      for (int ri = 0; ri < 2; ri++) {
+      for (int ri = 0; ri < 2; ri++)
+      for (int tl : tiles) {
         if (ri && nrhs <= nb) continue;
-        a.remap = rm;
-        a.nt = nt;
-        a.rhs_inner = ri;
+        if (tl && (tl % blk != 0 || slab % tl != 0 || tl >= slab)) continue;
+        HopShape cand{nb, blk, rm, nt, ri, tl};
+        cand.apply(a);
         launch_hop_args(a, prec, ls.nl, nb, blk, RED_NONE, false, st);
         QOP_CHECK_HIP(hipEventRecord(e0, st));
         for (int r = 0; r < reps; r++) launch_hop_args(a, prec, ls.nl, nb, blk, RED_NONE, false, st);
@@ -59,14 +84,15 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
         QOP_CHECK_HIP(hipEventSynchronize(e1));
         float ms = 0;
         QOP_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best_ms) { best_ms = ms; best = HopShape{nb, blk, rm, nt, ri}; }
+        if (ms < best_ms) { best_ms = ms; best = cand; }
       }
   }
   QOP_CHECK_HIP(hipEventDestroy(e0));
   QOP_CHECK_HIP(hipEventDestroy(e1));
   if (options().verbose)
-    printf("TUNE: prec %d nrhs %d -> nb %d block %d remap %d nt %d inner %d (%.3f ms/hop)\n", prec,
-           nrhs, best.nb, best.block, best.remap, best.nt, best.rhs_inner, best_ms / reps);
+    printf("TUNE: prec %d nrhs %d -> nb %d block %d remap %d nt %d inner %d tile %d (%.3f ms/hop)\n",
+           prec, nrhs, best.nb, best.block, best.remap, best.nt, best.rhs_inner, best.tile,
+           best_ms / reps);
   cache[key] = best;
   return best;
 }
@@ -117,11 +143,7 @@ BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double
   loaded_.assign(cap_, 0);
   mass_.assign(cap_, 0.0);
   HopShape hs = tune_hop(L_, sp_, cap_, ps_, ts_, 1 - eo_);
-  nb_ = hs.nb;
-  block_ = hs.block;
-  remap_ = hs.remap;
-  nt_ = hs.nt;
-  inner_ = hs.rhs_inner;
+  shape_ = hs;
   L_.get(P64);
   // allocations are cleared and links converted on the library stream
   sync();
@@ -318,10 +340,8 @@ void BatchSolver::enqueue_iteration() {
   a.active = active_;
   a.status = &glob_->status;
   a.partial = partA_;
-  a.remap = remap_;
-  a.nt = nt_;
-  a.rhs_inner = inner_;
-  launch_hop_args(a, sp_, ls.nl, nb_, block_, RED_OUT2, false, st);
+  shape_.apply(a);
+  launch_hop_args(a, sp_, ls.nl, shape_.nb, shape_.block, RED_OUT2, false, st);
   // second hop with the CG update fused: r -= alpha (m^2 p - D t), x += alpha p
   HopArgs b = a;
   b.parity = eo_;
@@ -336,7 +356,7 @@ void BatchSolver::enqueue_iteration() {
   b.cg_stride = (int)(sizeof(BsRhs) / sizeof(double));
   b.partial = partC_;
   bs_launch_scalar1(rhs_, active_, &glob_->status, partA_, partB_, nch_, cap_, ru_cont_, st);
-  launch_hop_args(b, sp_, ls.nl, nb_, block_, RED_CG, true, st);
+  launch_hop_args(b, sp_, ls.nl, shape_.nb, shape_.block, RED_CG, true, st);
   bs_launch_scalar2(rhs_, active_, glob_, partC_, nch_, cap_, ru_, st);
   bs_launch_pupdate(rhs_, active_, &glob_->status, sp_, (int)Vh, rs_.slot(0), ps_.slot(0), 3 * Vh, cap_, 1,
                     partB_, st);

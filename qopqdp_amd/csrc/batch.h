@@ -63,6 +63,17 @@ void bs_launch_pupdate(const BsRhs *rhs, const int *active, const int *status,
 void bs_launch_sum(const double *part, int nch, int n, const int *sel,
                    double *out, hipStream_t st);
 
+// Tuned hop launch shape for (lattice, precision, nl, nrhs), timed on first use
+// and cached (batch.cpp).
+struct HopShape {
+  int nb = 1, block = 256, remap = 1, nt = 0, rhs_inner = 0, tile = 0;
+  void apply(HopArgs &a) const {
+    a.remap = remap;
+    a.nt = nt;
+    a.rhs_inner = rhs_inner;
+    a.tile = tile;
+  }
+};
 struct SlotResult {
   int iter = 0, nupdate = 0, reason = 0;
   double true_rsq = 0;  // |src - A x|^2 / |src|^2 in FP64
@@ -134,15 +145,11 @@ class BatchSolver {
   std::vector<double> mass_;
   hipGraphExec_t graph_ = nullptr;
   int graph_chunk_ = 0;
-  int nb_ = 4, block_ = 256, remap_ = 1, nt_ = 0, inner_ = 0;  // tuned hop launch shape
+  HopShape shape_;  // tuned hop launch shape
 };
 
-// Tuned hop launch shape for (lattice, precision, nl, nrhs), timed on first use
-// and cached (batch.cpp).
-struct HopShape {
-  int nb = 1, block = 256, remap = 1, nt = 0, rhs_inner = 0;
-};
 HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &dst,
                   int eo_out);
+void set_hop_shape_override(const HopShape *s);  // null: tune again
 
 }  // namespace qop

@@ -136,6 +136,24 @@ __device__ __forceinline__ int logical_block(int remap) {
   return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
 
+// Output site of this thread for site block sb.  Plain order: blocks cover
+// consecutive sites.  x0-tiled order (tile > 0 sites, dividing the Vh/L0
+// sites of an x0 slab): consecutive blocks sweep x0 for one tile of the slab
+// before moving to the next tile, so the gathered x0 +-1, +-3 neighbours are
+// reused from L2 a few blocks later instead of one slab later.  Blocks past
+// the last site return Vh (no site).
+__device__ __forceinline__ int site_of_block(const Geom &g, int sb, int tile) {
+  if (!tile) return sb * blockDim.x + threadIdx.x;
+  const int nblk = g.Vh / blockDim.x;
+  if (sb >= nblk) return g.Vh;
+  const int tb = tile / blockDim.x;
+  const int col = g.L[0] * tb;
+  const int j = sb / col;
+  const int rem = sb - j * col;
+  const int x0 = rem / tb;
+  return x0 * (g.Vh / g.L[0]) + j * tile + (rem - x0 * tb) * blockDim.x + threadIdx.x;
+}
+
 // ---------------------------------------------------------------------------
 // hop kernel
 template <int P, int NL, int NB, int RED, bool HASX, bool NT>
@@ -148,7 +166,8 @@ hop_kernel(const HopArgs a) {
   // rhs_inner: the RHS groups of one site block are consecutive logical
   // blocks, so they run together on one XCD and share its L2 (links and the
   // neighbour window); otherwise the groups are the grid's y dimension.
-  const int h = (a.rhs_inner ? lb / a.ngroups : lb) * blockDim.x + threadIdx.x;
+  const int sb = a.rhs_inner ? lb / a.ngroups : lb;
+  const int h = site_of_block(g, sb, a.tile);
   const int b0 = (a.rhs_inner ? lb % a.ngroups : (int)blockIdx.y) * NB;
   if (a.status && *a.status != 0) return;
 
@@ -341,6 +360,10 @@ void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
   QOP_REQUIRE(red == RED_NONE || hasx || red == RED_OUT2, "hop: RED_X2 / RED_CG need x");
   QOP_REQUIRE(red != RED_CG || (a.cg_alpha && a.xs.data && prec != P64), "hop: RED_CG needs alpha and xs");
   QOP_REQUIRE(a.n >= 1, "hop: n >= 1");
+  if (a.tile) {
+    const int slab = a.g.Vh / a.g.L[0];
+    QOP_REQUIRE(a.tile % block == 0 && slab % a.tile == 0, "hop: tile must divide the x0 slab");
+  }
   int nblk = (a.g.Vh + block - 1) / block;
   a.ngroups = (a.n + nb - 1) / nb;
   dim3 grid, blk(block);

@@ -151,9 +151,7 @@ void MultiShiftBatch::enqueue_iteration() {
   a.n = cap_;
   a.active = active_;
   a.partial = partA_;
-  a.remap = shape_.remap;
-  a.nt = shape_.nt;
-  a.rhs_inner = shape_.rhs_inner;
+  shape_.apply(a);
   launch_hop_args(a, sp_, ls.nl, shape_.nb, shape_.block, RED_OUT2, false, st);
   ms_launch_scalarA(slot_, active_, partA_, partB_, nch_, cap_, st);
   // r -= beta_0 (m_0^2 p - D t), x_base += beta_0 p, |r|^2

@@ -18,8 +18,8 @@ extern "C" {
 // hop of a batched CG iteration), red = 2 the instance with the fused CG
 // update r -= a (m^2 p - D t), x_FP32 += a p, |r|^2 (the second hop; sloppy
 // precisions only).  Returns the mean ms per launch from
-// HIP events on the library stream; shape[5] receives (rhs per thread, block,
-// XCD remap, non-temporal links, RHS-innermost block order).
+// HIP events on the library stream; shape[6] receives (rhs per thread, block,
+// XCD remap, non-temporal links, RHS-innermost block order, x0 tile).
 double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, int nrep,
                                int red, int *shape) {
   Links &L = fla->L;
@@ -63,9 +63,7 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
   a.sstride = a.dstride = 3 * lat->Vh;
   a.nstride = lat->Vh;
   a.n = nrhs;
-  a.remap = hs.remap;
-  a.nt = hs.nt;
-  a.rhs_inner = hs.rhs_inner;
+  hs.apply(a);
   QOP_REQUIRE(red >= 0 && red <= 2 && (red != 2 || p != P64), "hop_time: bad reduction");
   double *part = nullptr, *alpha = nullptr;
   DevBatch xb, xs;
@@ -116,8 +114,21 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
   devbatch_free(dst);
   if (shape) {
     shape[0] = hs.nb; shape[1] = hs.block; shape[2] = hs.remap; shape[3] = hs.nt; shape[4] = hs.rhs_inner;
+    shape[5] = hs.tile;
   }
   return ms / nrep;
+}
+
+// Force the multi-RHS hop launch shape (rhs per thread, block, XCD remap,
+// non-temporal links, RHS-innermost order, x0 tile) for solvers built from
+// now on, wherever it is valid; null restores autotuning.
+void QOPX_hop_shape_override(const int *shape) {
+  if (!shape) {
+    set_hop_shape_override(nullptr);
+    return;
+  }
+  HopShape s{shape[0], shape[1], shape[2], shape[3], shape[4], shape[5]};
+  set_hop_shape_override(&s);
 }
 
 // Batched mixed-precision solves of n EVEN (or ODD) systems
