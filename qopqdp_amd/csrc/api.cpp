@@ -454,9 +454,7 @@ void QOP_D3_asqtad_load_L_from_G(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *f,
 }
 static void destroy_links(Links &L) {
   sync();
-  linkset_free(L.l64);
-  linkset_free(L.l32);
-  linkset_free(L.l16);
+  L.free_all();
 }
 void QOP_D3_asqtad_destroy_L(QOP_D3_FermionLinksAsqtad *f) {
   if (!f) return;

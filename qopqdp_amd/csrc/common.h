@@ -114,7 +114,13 @@ struct Links {
   Lattice *lat = nullptr;
   int nl = 16;
   LinkSet l64, l32, l16;
-  LinkSet &get(int prec);  // builds lower precisions from l64 on demand
+  // single-stored forward links (styles 0 and 2: backward links are formed
+  // in the hop as -U(x-d)^dagger): [p][nl/2][9][Vh], FP64 / FP32
+  LinkSet s64, s32;
+  LinkSet &get(int prec);         // builds lower precisions from l64 on demand
+  LinkSet &get_single(int prec);  // builds the single-stored set on demand
+  void drop_derived();            // frees every set derived from l64
+  void free_all();
 };
 
 // ---------------------------------------------------------------------

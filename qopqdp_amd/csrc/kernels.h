@@ -30,6 +30,10 @@ struct HopArgs {
   int nt = 0;                    // non-temporal link loads
   int rhs_inner = 0;             // RHS groups innermost in the block order
   int tile = 0;                  // x0-tiled site order: sites per tile of an x0 slab (0: plain)
+  // single-stored links: `links` holds the forward links of the output
+  // parity, links_bck those of the source parity ([nl/2][9][Vh] each)
+  int single = 0;
+  const void *links_bck = nullptr;
   VecView xs{};                  // RED_CG: FP32 solution increments (stride dstride)
   const double *cg_alpha = nullptr;  // RED_CG: alpha of slot b at cg_alpha[b*cg_stride]
   int cg_stride = 0;

@@ -156,7 +156,7 @@ __device__ __forceinline__ int site_of_block(const Geom &g, int sb, int tile) {
 
 // ---------------------------------------------------------------------------
 // hop kernel
-template <int P, int NL, int NB, int RED, bool HASX, bool NT>
+template <int P, int NL, int NB, int RED, bool HASX, bool NT, bool SGL>
 __global__ void __launch_bounds__(256)
 hop_kernel(const HopArgs a) {
   using R = typename PT<P>::R;
@@ -201,15 +201,40 @@ hop_kernel(const HopArgs a) {
                                : (j == 0 ? 1 : -1);
         const int k = PER * mu + j;
         const int nb = nbr(g, x, lex, mu, d);
-        const long lbase = (long)k * 9 * Vh + h;
-        float scl = 1.0f;
-        if constexpr (P == P16) {
-          if constexpr (NT) scl = __builtin_nontemporal_load(a.lscale + (long)k * Vh + h);
-          else scl = a.lscale[(long)k * Vh + h];
-        }
         R w[18];
+        if constexpr (SGL) {
+          // single-stored: forward link of this site, or -U(x-d)^dagger from
+          // the forward link of the neighbour (source parity)
+          constexpr int HP = PER / 2;
+          const bool fwd = j < HP;  // resolved at compile time once j is unrolled
+          const int ks = HP * mu + (fwd ? j : j - HP);
+          const long lb2 = (long)ks * 9 * Vh + (fwd ? h : nb);
+          const void *base = fwd ? a.links : a.links_bck;
+          R u[18];
 #pragma unroll
-        for (int e = 0; e < 9; e++) ld_c<P, NT>(a.links, lbase + (long)e * Vh, scl, w[2 * e], w[2 * e + 1]);
+          for (int e = 0; e < 9; e++) ld_c<P, false>(base, lb2 + (long)e * Vh, 1.0f, u[2 * e], u[2 * e + 1]);
+#pragma unroll
+          for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+              if (fwd) {
+                w[6 * r + 2 * c] = u[6 * r + 2 * c];
+                w[6 * r + 2 * c + 1] = u[6 * r + 2 * c + 1];
+              } else {
+                w[6 * r + 2 * c] = -u[6 * c + 2 * r];
+                w[6 * r + 2 * c + 1] = u[6 * c + 2 * r + 1];
+              }
+            }
+        } else {
+          const long lbase = (long)k * 9 * Vh + h;
+          float scl = 1.0f;
+          if constexpr (P == P16) {
+            if constexpr (NT) scl = __builtin_nontemporal_load(a.lscale + (long)k * Vh + h);
+            else scl = a.lscale[(long)k * Vh + h];
+          }
+#pragma unroll
+          for (int e = 0; e < 9; e++) ld_c<P, NT>(a.links, lbase + (long)e * Vh, scl, w[2 * e], w[2 * e + 1]);
+        }
 #pragma unroll
         for (int b = 0; b < NB; b++) {
           if (!act[b]) continue;
@@ -314,10 +339,10 @@ hop_kernel(const HopArgs a) {
 
 int hop_partial_blocks(const Lattice *lat) { return (int)((lat->Vh + 63) / 64); }
 
-template <int P, int NL, int NB, bool NT>
+template <int P, int NL, int NB, bool NT, bool SGL = false>
 static void hop_dispatch_red(const HopArgs &a, dim3 grid, dim3 block, int red,
                              bool hasx, hipStream_t st) {
-#define QOP_LAUNCH(RD, HX) hipLaunchKernelGGL((hop_kernel<P, NL, NB, RD, HX, NT>), grid, block, 0, st, a)
+#define QOP_LAUNCH(RD, HX) hipLaunchKernelGGL((hop_kernel<P, NL, NB, RD, HX, NT, SGL>), grid, block, 0, st, a)
   if (red == RED_NONE) { if (hasx) QOP_LAUNCH(RED_NONE, true); else QOP_LAUNCH(RED_NONE, false); }
   else if (red == RED_OUT2) { if (hasx) QOP_LAUNCH(RED_OUT2, true); else QOP_LAUNCH(RED_OUT2, false); }
   else if (red == RED_CG) { QOP_LAUNCH(RED_CG, true); }
@@ -328,6 +353,15 @@ static void hop_dispatch_red(const HopArgs &a, dim3 grid, dim3 block, int red,
 template <int P, int NL>
 static void hop_dispatch_nb(const HopArgs &a, int nb, dim3 grid, dim3 block,
                             int red, bool hasx, hipStream_t st) {
+  if (a.single) {
+    if constexpr (P != P16) {
+      QOP_REQUIRE(nb == 1 && !a.nt, "hop: single-stored links take one RHS per thread");
+      hop_dispatch_red<P, NL, 1, false, true>(a, grid, block, red, hasx, st);
+    } else {
+      fatal(__FILE__, __LINE__, "hop: single-stored links exist in FP64 and FP32 only");
+    }
+    return;
+  }
   if (a.nt) {
     switch (nb) {
       case 1: hop_dispatch_red<P, NL, 1, true>(a, grid, block, red, hasx, st); break;

@@ -90,8 +90,7 @@ void build_links64(Links &L, const double *fat_dev[4], const double *lng_dev[4],
   QOP_CHECK_HIP(hipGetLastError());
   QOP_CHECK_HIP(hipStreamSynchronize(st));
   QOP_CHECK_HIP(hipFree(ptrs));
-  linkset_free(L.l32);
-  linkset_free(L.l16);
+  L.drop_derived();
 }
 
 // ---------------------------------------------------------------------------

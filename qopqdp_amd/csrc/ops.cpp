@@ -1,5 +1,6 @@
 // Field operations on device vectors (see ops.h).
 #include "ops.h"
+#include "solver.h"
 
 namespace qop {
 
@@ -15,18 +16,32 @@ Sub subset(int eo) {
 }
 }  // namespace
 
+void set_hop_links(HopArgs &a, Links &L, int prec, int out_parity) {
+  Lattice *lat = L.lat;
+  if ((options().style & 1) == 0 && prec != P16) {
+    LinkSet &s = L.get_single(prec);
+    a.single = 1;
+    a.links = s.parity_data(lat, out_parity);
+    a.links_bck = s.parity_data(lat, 1 - out_parity);
+    a.lscale = nullptr;
+  } else {
+    LinkSet &ls = L.get(prec);
+    a.single = 0;
+    a.links = ls.parity_data(lat, out_parity);
+    a.lscale = ls.parity_scale(lat, out_parity);
+  }
+}
+
 void hop(Links &L, int prec, DevVec &dst, int out_parity, const DevVec &src,
          double alpha, double beta, const DevVec *x, int red, double *dev_result,
          hipStream_t st, const int *status) {
   Lattice *lat = L.lat;
   QOP_REQUIRE(dst.prec == prec && src.prec == prec && (!x || x->prec == prec),
               "hop: precision mismatch");
-  LinkSet &ls = L.get(prec);
   HopArgs a;
   a.g = lat->g;
   a.parity = out_parity;
-  a.links = ls.parity_data(lat, out_parity);
-  a.lscale = ls.parity_scale(lat, out_parity);
+  set_hop_links(a, L, prec, out_parity);
   a.src = src.half(1 - out_parity);
   a.dst = dst.half(out_parity);
   if (x) a.x = x->half(out_parity);
@@ -36,7 +51,7 @@ void hop(Links &L, int prec, DevVec &dst, int out_parity, const DevVec &src,
   a.status = status;
   int nch = hop_partial_blocks(lat);
   if (red != RED_NONE) a.partial = red_partial(nch);
-  launch_hop_args(a, prec, ls.nl, 1, 256, red, x != nullptr, st);
+  launch_hop_args(a, prec, L.nl, 1, 256, red, x != nullptr, st);
   if (red != RED_NONE) launch_finalize(a.partial, nch, 1, dev_result, st, status);
 }
 

@@ -15,6 +15,11 @@ inline int oppsub(int eo) { return (4 - eo) % 3; }
 
 // dst(out_parity) = alpha * hop(src) + beta * x(out_parity); optional FP64
 // reduction (RED_OUT2 / RED_X2) written to dev_result[0] (device double).
+// Link arguments of a one-RHS hop onto out_parity: single-stored links when
+// the QOP style option is even (lib/asqtad_dslash_p.c:8, dblstore_style) and
+// the precision is FP64/FP32, the double-stored set otherwise.
+void set_hop_links(HopArgs &a, Links &L, int prec, int out_parity);
+
 void hop(Links &L, int prec, DevVec &dst, int out_parity, const DevVec &src,
          double alpha, double beta, const DevVec *x, int red, double *dev_result,
          hipStream_t st, const int *status = nullptr);

@@ -99,6 +99,42 @@ LinkSet &Links::get(int prec) {
   return d;
 }
 
+LinkSet &Links::get_single(int prec) {
+  QOP_REQUIRE(prec == P64 || prec == P32, "single-stored links exist in FP64 and FP32");
+  LinkSet &d = prec == P64 ? s64 : s32;
+  if (!d.data) {
+    // copy the forward blocks (p, k = 4mu+{0,1} or 2mu) of the double-stored set
+    LinkSet &src = get(prec);
+    const int nls = nl / 2, per = nl / 4;
+    const size_t blk = (size_t)9 * lat->Vh * prec_cbytes(prec);
+    d.prec = prec;
+    d.nl = nls;
+    QOP_CHECK_HIP(hipMalloc(&d.data, (size_t)2 * nls * blk));
+    for (int p = 0; p < 2; p++)
+      for (int mu = 0; mu < 4; mu++)
+        for (int j = 0; j < per / 2; j++) {
+          const int ks = (per / 2) * mu + j, kd = per * mu + j;
+          QOP_CHECK_HIP(hipMemcpyAsync((char *)d.data + ((size_t)p * nls + ks) * blk,
+                                       (const char *)src.data + ((size_t)p * nl + kd) * blk, blk,
+                                       hipMemcpyDeviceToDevice, stream()));
+        }
+    sync();
+  }
+  return d;
+}
+
+void Links::drop_derived() {
+  linkset_free(l32);
+  linkset_free(l16);
+  linkset_free(s64);
+  linkset_free(s32);
+}
+
+void Links::free_all() {
+  linkset_free(l64);
+  drop_derived();
+}
+
 hipStream_t stream() {
   static hipStream_t s = [] {
     hipStream_t t;

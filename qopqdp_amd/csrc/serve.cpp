@@ -113,11 +113,7 @@ class Server {
     cv_work_.notify_all();
     worker_.join();
     for (auto &e : engines_) drop_engine(*e);
-    for (auto &c : cfgs_) {
-      linkset_free(c->links.l64);
-      linkset_free(c->links.l32);
-      linkset_free(c->links.l16);
-    }
+    for (auto &c : cfgs_) c->links.free_all();
   }
 
   int add_config(const int L[4], double *const U[4], double *fat[4], double *lng[4],

@@ -138,21 +138,23 @@ void invert_cgms(Links &L, int prec, double mass, int eo, InvArg *ia, ResArg **r
   B.push(st);
 
   const int Vh = (int)lat->Vh;
-  LinkSet &ls = L.get(prec);
+  const int nl = L.nl;
   const int with_rel = relstop > 0;
   auto enqueue_iteration = [&]() {
     // linop: tmp = D p (|tmp|^2), Mp = m^2 p - D tmp (|p|^2)
     HopArgs a1;
-    a1.g = lat->g; a1.parity = opp; a1.links = ls.parity_data(lat, opp);
-    a1.lscale = ls.parity_scale(lat, opp); a1.src = p.v.half(eo); a1.dst = tmp.v.half(opp);
+    a1.g = lat->g; a1.parity = opp;
+    set_hop_links(a1, L, prec, opp);
+    a1.src = p.v.half(eo); a1.dst = tmp.v.half(opp);
     a1.partial = B.d.partA; a1.status = &B.d.st->status;
-    launch_hop_args(a1, prec, ls.nl, 1, 256, RED_OUT2, false, st);
+    launch_hop_args(a1, prec, nl, 1, 256, RED_OUT2, false, st);
     HopArgs a2;
-    a2.g = lat->g; a2.parity = eo; a2.links = ls.parity_data(lat, eo);
-    a2.lscale = ls.parity_scale(lat, eo); a2.src = tmp.v.half(opp); a2.dst = Mp.v.half(eo);
+    a2.g = lat->g; a2.parity = eo;
+    set_hop_links(a2, L, prec, eo);
+    a2.src = tmp.v.half(opp); a2.dst = Mp.v.half(eo);
     a2.x = p.v.half(eo); a2.alpha = -1.0; a2.beta = mass * mass;
     a2.partial = B.d.partB; a2.status = &B.d.st->status;
-    launch_hop_args(a2, prec, ls.nl, 1, 256, RED_X2, true, st);
+    launch_hop_args(a2, prec, nl, 1, 256, RED_X2, true, st);
     cgms_launch_scalarA(B.d, B.nch, st);
     cgms_launch_update(B.d, prec, Vh, r.v.half(eo), Mp.v.half(eo), ns, st);
     if (with_rel) cgms_launch_relnorm(B.d, prec, Vh, r.v.half(eo), out[irel]->half(eo), st);

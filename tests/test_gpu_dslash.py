@@ -112,3 +112,52 @@ def test_links_roundtrip_and_diaginv(setup):
     assert np.all(got[: V // 2] == 0)
     s.free(vin)
     s.free(vout)
+
+
+@pytest.mark.parametrize("with_long", [True, False])
+def test_single_stored_style_matches_double_stored(setup, oracle, with_long):
+    """Style 0 (single-stored links, backward links formed as -U(x-d)^dagger
+    in the hop; lib/asqtad_dslash_p.c:1195-1244) gives the same operator bit
+    for bit as style 1 (double-stored), in FP64 and FP32, and the same solve."""
+    s, L = setup["s"], setup["L"]
+    V = s.V
+    lng = setup["lng"] if with_long else None
+    fla = s.links_from_raw(setup["fat"], lng)
+    h = oracle.links(L, setup["fat"], lng)
+    src = q.gen_source(L, 2006, EVENODD)
+    res = {}
+    try:
+        for st in (1, 0):
+            assert s.set_opts(st=st) == 0
+            vin, vout = s.vec(src), s.zeros()
+            s.lib.QOP_D3_asqtad_dslash(q.QOPInfo(), fla, 0.3, vout, vin, EVENODD, EVENODD)
+            res[st] = s.get(vout)
+            s.free(vin)
+            s.free(vout)
+            flf = s.lib.QOP_FD3_asqtad_create_L_from_L(fla)
+            vin, vout = s.vec(src.astype(np.float32), prec="F"), s.zeros(prec="F")
+            s.lib.QOP_F3_asqtad_dslash(q.QOPInfo(), flf, 0.0, vout, vin, ODD, EVEN)
+            res[(st, "F")] = s.get(vout, prec="F")
+            s.free(vin, prec="F")
+            s.free(vout, prec="F")
+            s.lib.QOP_F3_asqtad_destroy_L(flf)
+            b = src.copy()
+            b[V // 2:] = 0
+            ia = q.invert_arg_default()
+            ia.evenodd = EVEN
+            ra = q.resid_arg_default()
+            ra.rsqmin = 1e-20
+            vin, vout = s.vec(b), s.zeros()
+            s.lib.QOP_D3_asqtad_invert(C.byref(q.QOPInfo()), fla, C.byref(ia), C.byref(ra), 0.1, vout, vin)
+            res[(st, "x")] = (s.get(vout), ra.final_iter)
+            s.free(vin)
+            s.free(vout)
+        assert rel(res[0], oracle.dslash(h, 0.3, src, EVENODD, EVENODD)) <= 1e-13
+        assert np.array_equal(res[0], res[1])
+        assert np.array_equal(res[(0, "F")], res[(1, "F")])
+        assert np.array_equal(res[(0, "x")][0], res[(1, "x")][0])
+        assert res[(0, "x")][1] == res[(1, "x")][1]
+    finally:
+        s.set_opts(st=1)
+        s.lib.QOP_D3_asqtad_destroy_L(fla)
+        oracle.free_links(h)
