@@ -86,6 +86,30 @@ __device__ __forceinline__ void st3(void *data, float *norm, long h, long Vh,
   }
 }
 
+// Store v and replace it by the value as stored (FP16: rounded halves times
+// the site norm, bit-identical to a reload).
+template <int P>
+__device__ __forceinline__ void st3_q(void *data, float *norm, long h, long Vh,
+                                      typename BT<P>::R v[6]) {
+  if constexpr (P == P16) {
+    float m = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 6; i++) m = fmaxf(m, fabsf(v[i]));
+    const float inv = m > 0.0f ? 1.0f / m : 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const __half2 q = __floats2half2_rn(v[2 * c] * inv, v[2 * c + 1] * inv);
+      ((__half2 *)data)[c * Vh + h] = q;
+      const float2 f = __half22float2(q);
+      v[2 * c] = f.x * m;
+      v[2 * c + 1] = f.y * m;
+    }
+    norm[h] = m;
+  } else {
+    st3<P>(data, norm, h, Vh, v);
+  }
+}
+
 // pAp = |D p|^2 + m^2 |p|^2 ; alpha = r2 / pAp
 __global__ void __launch_bounds__(1024)
 bs_scalar1(BsRhs *rhs, const int *active, const int *status, const double *partA,
@@ -126,10 +150,7 @@ bs_update(const BsRhs *rhs, const int *active, const int *status, int Vh, VecVie
     ld3<P>((const char *)ps.data + off * cb, ps.norm ? ps.norm + noff : nullptr, h, Vh, p);
 #pragma unroll
     for (int i = 0; i < 6; i++) r[i] = fma(-al, a[i], r[i]);
-    st3<P>((char *)rs.data + off * cb, rs.norm ? rs.norm + noff : nullptr, h, Vh, r);
-    if constexpr (P == P16) {  // the norm uses the stored value
-      ld3<P>((const char *)rs.data + off * cb, rs.norm + noff, h, Vh, r);
-    }
+    st3_q<P>((char *)rs.data + off * cb, rs.norm ? rs.norm + noff : nullptr, h, Vh, r);
 #pragma unroll
     for (int i = 0; i < 6; i++) acc += (double)r[i] * (double)r[i];
     // xs is FP32
@@ -193,8 +214,7 @@ bs_pupdate(const BsRhs *rhs, const int *active, const int *status, int Vh, VecVi
     ld3<P>((const char *)ps.data + off * cb, pn, h, Vh, p);
 #pragma unroll
     for (int i = 0; i < 6; i++) p[i] = fma(be, p[i], r[i]);
-    st3<P>((char *)ps.data + off * cb, pn, h, Vh, p);
-    if constexpr (P == P16) ld3<P>((const char *)ps.data + off * cb, pn, h, Vh, p);
+    st3_q<P>((char *)ps.data + off * cb, pn, h, Vh, p);
 #pragma unroll
     for (int i = 0; i < 6; i++) acc += (double)p[i] * (double)p[i];
   }
@@ -257,9 +277,9 @@ bs_ru_resid(const int *ru, int Vh, VecView b64, VecView ax64, VecView rs, VecVie
       rv[2 * c + 1] = (R)xi;
     }
     float *rn = rs.norm ? rs.norm + noff : nullptr;
-    st3<P>((char *)rs.data + off * cb, rn, h, Vh, rv);
-    R q[6], p[6];
-    ld3<P>((const char *)rs.data + off * cb, rn, h, Vh, q);
+    st3_q<P>((char *)rs.data + off * cb, rn, h, Vh, rv);  // rv: the stored rs
+    R p[6];
+    const R *q = rv;
     ld3<P>((const char *)ps.data + off * cb, ps.norm ? ps.norm + noff : nullptr, h, Vh, p);
 #pragma unroll
     for (int c = 0; c < 3; c++) {  // <rs, ps> = sum conj(rs) ps
@@ -334,8 +354,7 @@ bs_ru_pfix(const int *ru_cont, const double *lcoef, int Vh, VecView rs, VecView 
     p[2 * c + 1] = (R)pi;
   }
   float *pn = ps.norm ? ps.norm + noff : nullptr;
-  st3<P>((char *)ps.data + off * cb, pn, h, Vh, p);
-  if constexpr (P == P16) ld3<P>((const char *)ps.data + off * cb, pn, h, Vh, p);
+  st3_q<P>((char *)ps.data + off * cb, pn, h, Vh, p);
 #pragma unroll
   for (int i = 0; i < 6; i++) acc += (double)p[i] * (double)p[i];
   }

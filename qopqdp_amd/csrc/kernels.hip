@@ -98,6 +98,31 @@ __device__ __forceinline__ void vstore(void *data, float *norm, long h, long Vh,
   }
 }
 
+// Store v and replace it by the value as stored (for FP16 the rounded
+// half-precision components times the site norm, bit-identical to loading it
+// back), so reductions of the stored vector need no reload.
+template <int P>
+__device__ __forceinline__ void vstore_q(void *data, float *norm, long h, long Vh,
+                                         typename PT<P>::R v[6]) {
+  if constexpr (P == P16) {
+    float m = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 6; i++) m = fmaxf(m, fabsf(v[i]));
+    const float inv = m > 0.0f ? 1.0f / m : 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const __half2 q = __floats2half2_rn(v[2 * c] * inv, v[2 * c + 1] * inv);
+      ((__half2 *)data)[c * Vh + h] = q;
+      const float2 f = __half22float2(q);
+      v[2 * c] = f.x * m;
+      v[2 * c + 1] = f.y * m;
+    }
+    norm[h] = m;
+  } else {
+    vstore<P>(data, norm, h, Vh, v);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // wavefront reduction of a double (64 lanes) via cross-lane shuffles
 __device__ __forceinline__ double wave_sum(double v) {
@@ -287,8 +312,7 @@ hop_kernel(const HopArgs a) {
         vload<P>(dd, dn, h, Vh, rv);
 #pragma unroll
         for (int i = 0; i < 6; i++) rv[i] = fma(-alr, o[i], rv[i]);
-        vstore<P>(dd, dn, h, Vh, rv);
-        if constexpr (P == P16) vload<P>(dd, dn, h, Vh, rv);
+        vstore_q<P>(dd, dn, h, Vh, rv);
         double s = 0;
 #pragma unroll
         for (int i = 0; i < 6; i++) s += (double)rv[i] * (double)rv[i];
@@ -304,18 +328,11 @@ hop_kernel(const HopArgs a) {
         }
         continue;
       }
-      vstore<P>(dd, dn, h, Vh, o);
+      vstore_q<P>(dd, dn, h, Vh, o);  // o becomes the stored value
       if constexpr (RED == RED_OUT2) {
         double s = 0;
-        if constexpr (P == P16) {  // reduce what was stored
-          R q[6];
-          vload<P>(dd, dn, h, Vh, q);
 #pragma unroll
-          for (int i = 0; i < 6; i++) s += (double)q[i] * (double)q[i];
-        } else {
-#pragma unroll
-          for (int i = 0; i < 6; i++) s += (double)o[i] * (double)o[i];
-        }
+        for (int i = 0; i < 6; i++) s += (double)o[i] * (double)o[i];
         red[b] = s;
       } else if constexpr (RED == RED_X2) {
         double s = 0;
@@ -493,12 +510,10 @@ lincomb_kernel(const LincombArgs a) {
     for (int i = 0; i < 6; i++) oo[i] = (typename PT<PO>::R)o[i];
     char *d = (char *)a.out.data + (long)b * a.ostride * sizeof(typename PT<PO>::S);
     float *nm = a.out.norm ? a.out.norm + (long)b * a.nstride : nullptr;
-    vstore<PO>(d, nm, h, Vh, oo);
+    vstore_q<PO>(d, nm, h, Vh, oo);
     if constexpr (RED) {
-      typename PT<PO>::R q[6];
-      vload<PO>(d, nm, h, Vh, q);
 #pragma unroll
-      for (int i = 0; i < 6; i++) s += (double)q[i] * (double)q[i];
+      for (int i = 0; i < 6; i++) s += (double)oo[i] * (double)oo[i];
     }
   }
   if constexpr (RED) {
