@@ -142,6 +142,10 @@ BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double
   QOP_CHECK_HIP(hipMemcpyAsync(glob_, glob_h_, sizeof(BsGlobal), hipMemcpyHostToDevice, st_));
   loaded_.assign(cap_, 0);
   mass_.assign(cap_, 0.0);
+  // Build the sloppy and FP64 link sets now: the iteration is captured into
+  // a graph later, and a conversion (allocation, host sync) cannot run then.
+  L_.get(sp_);
+  L_.get(P64);
   HopShape hs = tune_hop(L_, sp_, cap_, ps_, ts_, 1 - eo_);
   shape_ = hs;
   L_.get(P64);

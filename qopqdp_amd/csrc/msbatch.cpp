@@ -48,6 +48,7 @@ MultiShiftBatch::MultiShiftBatch(Links &L, int eo, int capacity, int sloppy_prec
   std::vector<double> hc(2 * cap_);
   for (int b = 0; b < cap_; b++) { hc[2 * b] = -1.0; hc[2 * b + 1] = m0 * m0; }
   QOP_CHECK_HIP(hipMemcpy(coef_, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice));
+  L_.get(sp_);  // convert the links before any graph capture
   shape_ = tune_hop(L_, sp_, cap_, ps_, ts_, 1 - eo_);
   sync();
 }

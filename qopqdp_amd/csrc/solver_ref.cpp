@@ -140,17 +140,20 @@ void invert_cgms(Links &L, int prec, double mass, int eo, InvArg *ia, ResArg **r
   const int Vh = (int)lat->Vh;
   const int nl = L.nl;
   const int with_rel = relstop > 0;
+  // link arguments of both hops, resolved (and any link conversion done)
+  // before the iterations are captured into a graph
+  HopArgs l_opp, l_eo;
+  set_hop_links(l_opp, L, prec, opp);
+  set_hop_links(l_eo, L, prec, eo);
   auto enqueue_iteration = [&]() {
     // linop: tmp = D p (|tmp|^2), Mp = m^2 p - D tmp (|p|^2)
-    HopArgs a1;
+    HopArgs a1 = l_opp;
     a1.g = lat->g; a1.parity = opp;
-    set_hop_links(a1, L, prec, opp);
     a1.src = p.v.half(eo); a1.dst = tmp.v.half(opp);
     a1.partial = B.d.partA; a1.status = &B.d.st->status;
     launch_hop_args(a1, prec, nl, 1, 256, RED_OUT2, false, st);
-    HopArgs a2;
+    HopArgs a2 = l_eo;
     a2.g = lat->g; a2.parity = eo;
-    set_hop_links(a2, L, prec, eo);
     a2.src = tmp.v.half(opp); a2.dst = Mp.v.half(eo);
     a2.x = p.v.half(eo); a2.alpha = -1.0; a2.beta = mass * mass;
     a2.partial = B.d.partB; a2.status = &B.d.st->status;
