@@ -12,7 +12,9 @@
 namespace qop {
 
 // ---------------------------------------------------------------------------
-// autotuner: time the multi-RHS hop for each launch shape and keep the best.
+// autotuner: time the multi-RHS hop over launch shapes and keep the best:
+// first RHS per thread x block size, then remap / link loads / block order /
+// site tiling for the best of those.
 static std::mutex g_tune_mu;  // solvers may be built from several host threads
 static bool g_override = false;
 static HopShape g_override_shape;
@@ -65,28 +67,34 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
   const int tiles[] = {0, 512, 1024, 2048, 4096};
   const int slab = (int)(lat->Vh / lat->L[0]);
   const int reps = lat->Vh > 200000 ? 4 : 10;
+  auto time_shape = [&](const HopShape &cand) {
+    cand.apply(a);
+    launch_hop_args(a, prec, ls.nl, cand.nb, cand.block, RED_NONE, false, st);
+    QOP_CHECK_HIP(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; r++) launch_hop_args(a, prec, ls.nl, cand.nb, cand.block, RED_NONE, false, st);
+    QOP_CHECK_HIP(hipEventRecord(e1, st));
+    QOP_CHECK_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    QOP_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best_ms) { best_ms = ms; best = cand; }
+  };
+  // stage 1: RHS per thread and block size, with XCD remap, plain link loads,
+  // RHS-innermost order and the plain site order
   for (int nb : nbs) {
     if (nb > 1 && nb > nrhs) continue;
     if (nb == 16 && (prec == P64 || L.nl != 16)) continue;
-    for (int blk : blocks)
-      for (int rm = 0; rm < 2; rm++)
-      for (int nt = 0; nt < 2; nt++)
-      for (int ri = 0; ri < 2; ri++)
-      for (int tl : tiles) {
-        if (ri && nrhs <= nb) continue;
-        if (tl && (tl % blk != 0 || slab % tl != 0 || tl >= slab)) continue;
-        HopShape cand{nb, blk, rm, nt, ri, tl};
-        cand.apply(a);
-        launch_hop_args(a, prec, ls.nl, nb, blk, RED_NONE, false, st);
-        QOP_CHECK_HIP(hipEventRecord(e0, st));
-        for (int r = 0; r < reps; r++) launch_hop_args(a, prec, ls.nl, nb, blk, RED_NONE, false, st);
-        QOP_CHECK_HIP(hipEventRecord(e1, st));
-        QOP_CHECK_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        QOP_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best_ms) { best_ms = ms; best = cand; }
-      }
+    for (int blk : blocks) time_shape(HopShape{nb, blk, 1, 0, nrhs > nb ? 1 : 0, 0});
   }
+  // stage 2: the remaining switches for the best (RHS per thread, block)
+  const int nb = best.nb, blk = best.block;
+  for (int rm = 0; rm < 2; rm++)
+    for (int nt = 0; nt < 2; nt++)
+      for (int ri = 0; ri < 2; ri++)
+        for (int tl : tiles) {
+          if (ri && nrhs <= nb) continue;
+          if (tl && (tl % blk != 0 || slab % tl != 0 || tl >= slab)) continue;
+          time_shape(HopShape{nb, blk, rm, nt, ri, tl});
+        }
   QOP_CHECK_HIP(hipEventDestroy(e0));
   QOP_CHECK_HIP(hipEventDestroy(e1));
   if (options().verbose)
@@ -147,6 +155,7 @@ BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double
   L_.get(sp_);
   L_.get(P64);
   HopShape hs = tune_hop(L_, sp_, cap_, ps_, ts_, 1 - eo_);
+  ru_shape_ = tune_hop(L_, P64, cap_, x64_, t64_, 1 - eo_);
   shape_ = hs;
   L_.get(P64);
   // allocations are cleared and links converted on the library stream
@@ -222,10 +231,11 @@ void BatchSolver::apply_A64(const int *mask) {
   a.nstride = lat_->Vh;
   a.n = cap_;
   a.active = mask;
-  // four slots per thread: slots that start together flag their updates in
-  // the same iteration, so a group shares one read of the FP64 links; a group
-  // with no flagged slot reads nothing
-  launch_hop_args(a, P64, ls.nl, 4, 256, RED_NONE, false, st);
+  // the tuned FP64 multi-RHS shape: slots that start together flag their
+  // updates in the same iteration and share the link reads of their group; a
+  // group with no flagged slot reads nothing
+  ru_shape_.apply(a);
+  launch_hop_args(a, P64, ls.nl, ru_shape_.nb, ru_shape_.block, RED_NONE, false, st);
   HopArgs b = a;
   b.parity = eo_;
   b.links = ls.parity_data(lat_, eo_);
@@ -234,7 +244,7 @@ void BatchSolver::apply_A64(const int *mask) {
   b.dst = ax64_.slot(0);
   b.x = x64_.slot(0);
   b.coef = coef_;
-  launch_hop_args(b, P64, ls.nl, 4, 256, RED_NONE, true, st);
+  launch_hop_args(b, P64, ls.nl, ru_shape_.nb, ru_shape_.block, RED_NONE, true, st);
 }
 
 void BatchSolver::load(int slot, const DevVec &src, const DevVec *x0, double mass, double tol,

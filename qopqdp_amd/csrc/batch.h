@@ -145,7 +145,8 @@ class BatchSolver {
   std::vector<double> mass_;
   hipGraphExec_t graph_ = nullptr;
   int graph_chunk_ = 0;
-  HopShape shape_;  // tuned hop launch shape
+  HopShape shape_;     // tuned sloppy hop launch shape
+  HopShape ru_shape_;  // tuned FP64 hop launch shape of the reliable updates
 };
 
 HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &dst,
