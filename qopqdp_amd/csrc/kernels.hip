@@ -101,7 +101,9 @@ __device__ __forceinline__ void vstore(void *data, float *norm, long h, long Vh,
 // Store v and replace it by the value as stored (for FP16 the rounded
 // half-precision components times the site norm, bit-identical to loading it
 // back), so reductions of the stored vector need no reload.
-template <int P>
+// NT: non-temporal (streaming) stores, so written vectors do not displace
+// the gathered source vectors from L2.
+template <int P, bool NT = false>
 __device__ __forceinline__ void vstore_q(void *data, float *norm, long h, long Vh,
                                          typename PT<P>::R v[6]) {
   if constexpr (P == P16) {
@@ -112,12 +114,27 @@ __device__ __forceinline__ void vstore_q(void *data, float *norm, long h, long V
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const __half2 q = __floats2half2_rn(v[2 * c] * inv, v[2 * c + 1] * inv);
-      ((__half2 *)data)[c * Vh + h] = q;
+      if constexpr (NT)
+        __builtin_nontemporal_store(__builtin_bit_cast(unsigned, q), (unsigned *)data + c * Vh + h);
+      else
+        ((__half2 *)data)[c * Vh + h] = q;
       const float2 f = __half22float2(q);
       v[2 * c] = f.x * m;
       v[2 * c + 1] = f.y * m;
     }
-    norm[h] = m;
+    if constexpr (NT) __builtin_nontemporal_store(m, norm + h);
+    else norm[h] = m;
+  } else if constexpr (NT && P == P32) {
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      __builtin_nontemporal_store(__builtin_bit_cast(double, make_float2(v[2 * c], v[2 * c + 1])),
+                                  (double *)data + c * Vh + h);
+  } else if constexpr (NT && P == P64) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      __builtin_nontemporal_store(v[2 * c], (double *)data + 2 * (c * Vh + h));
+      __builtin_nontemporal_store(v[2 * c + 1], (double *)data + 2 * (c * Vh + h) + 1);
+    }
   } else {
     vstore<P>(data, norm, h, Vh, v);
   }
@@ -312,7 +329,7 @@ hop_kernel(const HopArgs a) {
         vload<P>(dd, dn, h, Vh, rv);
 #pragma unroll
         for (int i = 0; i < 6; i++) rv[i] = fma(-alr, o[i], rv[i]);
-        vstore_q<P>(dd, dn, h, Vh, rv);
+        vstore_q<P, NT>(dd, dn, h, Vh, rv);
         double s = 0;
 #pragma unroll
         for (int i = 0; i < 6; i++) s += (double)rv[i] * (double)rv[i];
@@ -324,11 +341,14 @@ hop_kernel(const HopArgs a) {
           float2 t = xs[c * Vh + h];
           t.x = fmaf(alf, (float)xv[2 * c], t.x);
           t.y = fmaf(alf, (float)xv[2 * c + 1], t.y);
-          xs[c * Vh + h] = t;
+          if constexpr (NT)
+            __builtin_nontemporal_store(__builtin_bit_cast(double, t), (double *)xs + c * Vh + h);
+          else
+            xs[c * Vh + h] = t;
         }
         continue;
       }
-      vstore_q<P>(dd, dn, h, Vh, o);  // o becomes the stored value
+      vstore_q<P, NT>(dd, dn, h, Vh, o);  // o becomes the stored value
       if constexpr (RED == RED_OUT2) {
         double s = 0;
 #pragma unroll
