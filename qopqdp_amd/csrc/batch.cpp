@@ -34,16 +34,20 @@ static bool shape_valid(const Lattice *lat, int prec, int nl, int nrhs, const Ho
   if (s.nb > 1 && s.nb > nrhs) return false;
   if (s.block != 64 && s.block != 128 && s.block != 256) return false;
   if (s.tile && (s.tile % s.block != 0 || slab % s.tile != 0 || s.tile >= slab)) return false;
+  if (s.lds < 0 || s.lds > 65536) return false;
   return true;
 }
 
-HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &dst, int eo_out) {
-  static std::map<std::tuple<const Lattice *, int, int, int>, HopShape> cache;
+HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &dst, int eo_out,
+                  int red, const DevBatch *x, const DevBatch *xs) {
+  static std::map<std::tuple<const Lattice *, int, int, int, int>, HopShape> cache;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   if (g_override && shape_valid(L.lat, prec, L.nl, nrhs, g_override_shape)) return g_override_shape;
-  auto key = std::make_tuple((const Lattice *)L.lat, prec, L.nl, nrhs);
+  auto key = std::make_tuple((const Lattice *)L.lat, prec, L.nl, nrhs, red);
   auto it = cache.find(key);
   if (it != cache.end()) return it->second;
+  QOP_REQUIRE(red == RED_NONE || red == RED_OUT2 || (red == RED_CG && x && xs && prec != P64),
+              "tune_hop: RED_CG needs x and xs");
   Lattice *lat = L.lat;
   LinkSet &ls = L.get(prec);
   hipStream_t st = stream();
@@ -54,9 +58,25 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
   a.lscale = ls.parity_scale(lat, eo_out);
   a.src = src.slot(0);
   a.dst = dst.slot(0);
-  a.sstride = a.dstride = 3 * lat->Vh;
+  a.sstride = a.dstride = a.xstride = 3 * lat->Vh;
   a.nstride = lat->Vh;
   a.n = nrhs;
+  // the epilogue being tuned: scratch partials, and for the fused CG update
+  // a zero alpha (dst and x then keep their values)
+  double *scratch = nullptr;
+  if (red != RED_NONE) {
+    const size_t np = (size_t)nrhs * ((lat->Vh + 63) / 64) + nrhs;
+    QOP_CHECK_HIP(hipMalloc(&scratch, np * sizeof(double)));
+    QOP_CHECK_HIP(hipMemsetAsync(scratch, 0, np * sizeof(double), st));
+    a.partial = scratch + nrhs;
+  }
+  if (red == RED_CG) {
+    a.x = x->slot(0);
+    a.xs = xs->slot(0);
+    a.cg_alpha = scratch;
+    a.cg_stride = 1;
+  }
+  const bool hasx = red == RED_CG;
   hipEvent_t e0, e1;
   QOP_CHECK_HIP(hipEventCreate(&e0));
   QOP_CHECK_HIP(hipEventCreate(&e1));
@@ -69,9 +89,9 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
   const int reps = lat->Vh > 200000 ? 4 : 10;
   auto time_shape = [&](const HopShape &cand) {
     cand.apply(a);
-    launch_hop_args(a, prec, ls.nl, cand.nb, cand.block, RED_NONE, false, st);
+    launch_hop_args(a, prec, ls.nl, cand.nb, cand.block, red, hasx, st);
     QOP_CHECK_HIP(hipEventRecord(e0, st));
-    for (int r = 0; r < reps; r++) launch_hop_args(a, prec, ls.nl, cand.nb, cand.block, RED_NONE, false, st);
+    for (int r = 0; r < reps; r++) launch_hop_args(a, prec, ls.nl, cand.nb, cand.block, red, hasx, st);
     QOP_CHECK_HIP(hipEventRecord(e1, st));
     QOP_CHECK_HIP(hipEventSynchronize(e1));
     float ms = 0;
@@ -93,14 +113,26 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
         for (int tl : tiles) {
           if (ri && nrhs <= nb) continue;
           if (tl && (tl % blk != 0 || slab % tl != 0 || tl >= slab)) continue;
-          time_shape(HopShape{nb, blk, rm, nt, ri, tl});
+          time_shape(HopShape{nb, blk, rm, nt, ri, tl, 0});
         }
+  // stage 3: fewer blocks per CU (an LDS reservation), so fewer sites are in
+  // flight per XCD and more gathered neighbours are still in L2
+  const HopShape b2 = best;
+  for (int lds : {16384, 32768, 65536}) {
+    HopShape c = b2;
+    c.lds = lds;
+    time_shape(c);
+  }
   QOP_CHECK_HIP(hipEventDestroy(e0));
   QOP_CHECK_HIP(hipEventDestroy(e1));
+  if (scratch) {
+    QOP_CHECK_HIP(hipStreamSynchronize(st));
+    QOP_CHECK_HIP(hipFree(scratch));
+  }
   if (options().verbose)
-    printf("TUNE: prec %d nrhs %d -> nb %d block %d remap %d nt %d inner %d tile %d (%.3f ms/hop)\n",
+    printf("TUNE: prec %d nrhs %d -> nb %d block %d remap %d nt %d inner %d tile %d lds %d (%.3f ms/hop)\n",
            prec, nrhs, best.nb, best.block, best.remap, best.nt, best.rhs_inner, best.tile,
-           best_ms / reps);
+           best.lds, best_ms / reps);
   cache[key] = best;
   return best;
 }
@@ -154,9 +186,12 @@ BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double
   // a graph later, and a conversion (allocation, host sync) cannot run then.
   L_.get(sp_);
   L_.get(P64);
-  HopShape hs = tune_hop(L_, sp_, cap_, ps_, ts_, 1 - eo_);
+  // the two hops of the iteration are tuned with their own epilogues (the
+  // second on rs_ / xs_ with alpha 0, while no slot is loaded)
+  shape_ = tune_hop(L_, sp_, cap_, ps_, ts_, 1 - eo_, RED_OUT2);
+  shape2_ = tune_hop(L_, sp_, cap_, ts_, rs_, eo_, RED_CG, &ps_, &xs_);
   ru_shape_ = tune_hop(L_, P64, cap_, x64_, t64_, 1 - eo_);
-  shape_ = hs;
+
   L_.get(P64);
   // allocations are cleared and links converted on the library stream
   sync();
@@ -370,7 +405,8 @@ void BatchSolver::enqueue_iteration() {
   b.cg_stride = (int)(sizeof(BsRhs) / sizeof(double));
   b.partial = partC_;
   bs_launch_scalar1(rhs_, active_, &glob_->status, partA_, partB_, nch_, cap_, ru_cont_, st);
-  launch_hop_args(b, sp_, ls.nl, shape_.nb, shape_.block, RED_CG, true, st);
+  shape2_.apply(b);
+  launch_hop_args(b, sp_, ls.nl, shape2_.nb, shape2_.block, RED_CG, true, st);
   bs_launch_scalar2(rhs_, active_, glob_, partC_, nch_, cap_, ru_, st);
   bs_launch_pupdate(rhs_, active_, &glob_->status, sp_, (int)Vh, rs_.slot(0), ps_.slot(0), 3 * Vh, cap_, 1,
                     partB_, st);

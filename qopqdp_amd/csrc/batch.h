@@ -66,12 +66,13 @@ void bs_launch_sum(const double *part, int nch, int n, const int *sel,
 // Tuned hop launch shape for (lattice, precision, nl, nrhs), timed on first use
 // and cached (batch.cpp).
 struct HopShape {
-  int nb = 1, block = 256, remap = 1, nt = 0, rhs_inner = 0, tile = 0;
+  int nb = 1, block = 256, remap = 1, nt = 0, rhs_inner = 0, tile = 0, lds = 0;
   void apply(HopArgs &a) const {
     a.remap = remap;
     a.nt = nt;
     a.rhs_inner = rhs_inner;
     a.tile = tile;
+    a.lds_reserve = lds;
   }
 };
 struct SlotResult {
@@ -145,12 +146,16 @@ class BatchSolver {
   std::vector<double> mass_;
   hipGraphExec_t graph_ = nullptr;
   int graph_chunk_ = 0;
-  HopShape shape_;     // tuned sloppy hop launch shape
+  HopShape shape_;     // tuned launch shape of the first sloppy hop
+  HopShape shape2_;    // tuned launch shape of the second (fused CG update)
   HopShape ru_shape_;  // tuned FP64 hop launch shape of the reliable updates
 };
 
+// red: the epilogue to tune with (RED_NONE, RED_OUT2, or RED_CG with x and
+// xs given; dst and xs keep their values, alpha is 0).
 HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &dst,
-                  int eo_out);
+                  int eo_out, int red = RED_NONE, const DevBatch *x = nullptr,
+                  const DevBatch *xs = nullptr);
 void set_hop_shape_override(const HopShape *s);  // null: tune again
 
 }  // namespace qop

@@ -379,7 +379,8 @@ int hop_partial_blocks(const Lattice *lat) { return (int)((lat->Vh + 63) / 64); 
 template <int P, int NL, int NB, bool NT, bool SGL = false>
 static void hop_dispatch_red(const HopArgs &a, dim3 grid, dim3 block, int red,
                              bool hasx, hipStream_t st) {
-#define QOP_LAUNCH(RD, HX) hipLaunchKernelGGL((hop_kernel<P, NL, NB, RD, HX, NT, SGL>), grid, block, 0, st, a)
+#define QOP_LAUNCH(RD, HX) \
+  hipLaunchKernelGGL((hop_kernel<P, NL, NB, RD, HX, NT, SGL>), grid, block, a.lds_reserve, st, a)
   if (red == RED_NONE) { if (hasx) QOP_LAUNCH(RED_NONE, true); else QOP_LAUNCH(RED_NONE, false); }
   else if (red == RED_OUT2) { if (hasx) QOP_LAUNCH(RED_OUT2, true); else QOP_LAUNCH(RED_OUT2, false); }
   else if (red == RED_CG) { QOP_LAUNCH(RED_CG, true); }
@@ -431,6 +432,7 @@ void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
   QOP_REQUIRE(red == RED_NONE || hasx || red == RED_OUT2, "hop: RED_X2 / RED_CG need x");
   QOP_REQUIRE(red != RED_CG || (a.cg_alpha && a.xs.data && prec != P64), "hop: RED_CG needs alpha and xs");
   QOP_REQUIRE(a.n >= 1, "hop: n >= 1");
+  QOP_REQUIRE(a.lds_reserve >= 0 && a.lds_reserve <= 65536, "hop: LDS reserve 0..64 KiB");
   if (a.tile) {
     const int slab = a.g.Vh / a.g.L[0];
     QOP_REQUIRE(a.tile % block == 0 && slab % a.tile == 0, "hop: tile must divide the x0 slab");

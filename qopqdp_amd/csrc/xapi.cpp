@@ -18,8 +18,9 @@ extern "C" {
 // hop of a batched CG iteration), red = 2 the instance with the fused CG
 // update r -= a (m^2 p - D t), x_FP32 += a p, |r|^2 (the second hop; sloppy
 // precisions only).  Returns the mean ms per launch from
-// HIP events on the library stream; shape[6] receives (rhs per thread, block,
-// XCD remap, non-temporal links, RHS-innermost block order, x0 tile).
+// HIP events on the library stream; shape[7] receives (rhs per thread, block,
+// XCD remap, non-temporal links, RHS-innermost block order, x0 tile, LDS
+// reserve).
 double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, int nrep,
                                int red, int *shape) {
   Links &L = fla->L;
@@ -51,7 +52,17 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
     QOP_CHECK_HIP(hipFree(d));
     devvec_free(tmp);
   }
-  HopShape hs = tune_hop(L, p, nrhs, src, dst, 1);
+  QOP_REQUIRE(red >= 0 && red <= 2 && (red != 2 || p != P64), "hop_time: bad reduction");
+  DevBatch xb, xs;
+  if (red == 2) {  // operands of the fused CG update: p (x), x_FP32
+    devbatch_alloc(xb, lat, p, nrhs);
+    devbatch_alloc(xs, lat, P32, nrhs);
+    sync();
+  }
+  // the shape the batched solver tunes for this epilogue
+  const int rk = red == 2 ? RED_CG : red == 1 ? RED_OUT2 : RED_NONE;
+  HopShape hs = tune_hop(L, p, nrhs, src, dst, 1, rk, red == 2 ? &xb : nullptr,
+                         red == 2 ? &xs : nullptr);
   LinkSet &ls = L.get(p);
   HopArgs a;
   a.g = lat->g;
@@ -64,16 +75,12 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
   a.nstride = lat->Vh;
   a.n = nrhs;
   hs.apply(a);
-  QOP_REQUIRE(red >= 0 && red <= 2 && (red != 2 || p != P64), "hop_time: bad reduction");
   double *part = nullptr, *alpha = nullptr;
-  DevBatch xb, xs;
   if (red) {
     QOP_CHECK_HIP(hipMalloc(&part, (size_t)nrhs * ((lat->Vh + 63) / 64) * sizeof(double)));
     a.partial = part;
   }
-  if (red == 2) {  // operands of the fused CG update: p (x), r (dst), x_FP32
-    devbatch_alloc(xb, lat, p, nrhs);
-    devbatch_alloc(xs, lat, P32, nrhs);
+  if (red == 2) {  // the fused CG update: x = p, dst = r, alpha 0
     QOP_CHECK_HIP(hipMalloc(&alpha, nrhs * sizeof(double)));
     QOP_CHECK_HIP(hipMemsetAsync(alpha, 0, nrhs * sizeof(double), stream()));
     QOP_CHECK_HIP(hipMemcpyAsync(xb.data, src.data, (size_t)nrhs * 3 * lat->Vh * prec_cbytes(p),
@@ -90,7 +97,6 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
     a.beta = 1e-4;
     sync();
   }
-  const int rk = red == 2 ? RED_CG : red == 1 ? RED_OUT2 : RED_NONE;
   hipStream_t st = stream();
   for (int w = 0; w < 3; w++) launch_hop_args(a, p, ls.nl, hs.nb, hs.block, rk, red == 2, st);
   hipEvent_t e0, e1;
@@ -115,19 +121,20 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
   if (shape) {
     shape[0] = hs.nb; shape[1] = hs.block; shape[2] = hs.remap; shape[3] = hs.nt; shape[4] = hs.rhs_inner;
     shape[5] = hs.tile;
+    shape[6] = hs.lds;
   }
   return ms / nrep;
 }
 
 // Force the multi-RHS hop launch shape (rhs per thread, block, XCD remap,
-// non-temporal links, RHS-innermost order, x0 tile) for solvers built from
+// non-temporal links, RHS-innermost order, x0 tile, LDS reserve) for solvers built from
 // now on, wherever it is valid; null restores autotuning.
 void QOPX_hop_shape_override(const int *shape) {
   if (!shape) {
     set_hop_shape_override(nullptr);
     return;
   }
-  HopShape s{shape[0], shape[1], shape[2], shape[3], shape[4], shape[5]};
+  HopShape s{shape[0], shape[1], shape[2], shape[3], shape[4], shape[5], shape[6]};
   set_hop_shape_override(&s);
 }
 
