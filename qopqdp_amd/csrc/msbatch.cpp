@@ -216,10 +216,12 @@ extern "C" {
 
 // Multi-mass solves of nsrc sources over one shared mass list:
 // out[i*nmass + j] = m_j A_j^-1 in[i] on the EVEN or ODD subset, each to FP64
-// true relative residual sqrt(res[i*nmass+j]->rsqmin).  Sloppy multi-shift CG
-// batched over the sources to relative residual ms_tol (0: 1e-5 in FP32,
-// 2e-3 in FP16), then FP64 reliable-update refinement of every (source, mass)
-// from its multi-shift solution, `capacity` systems at a time.
+// true relative residual sqrt(res[i*nmass+j]->rsqmin).  An FP32 multi-shift CG
+// batched over the sources runs to relative residual ms_tol (0: 1e-7), then
+// every (source, mass) is refined from its multi-shift solution by the
+// reliable-update CG in sloppy precision `prec`, `capacity` systems at a
+// time.  (An FP16 multi-shift stage stalls near 1e-3 and leaves most of the
+// work to the refinement.)
 int QOPX_D3_asqtad_invert_multi_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla,
                                       QOP_invert_arg_t *inv_arg, QOP_resid_arg_t *res[],
                                       double masses[], int nmass, QOP_D3_ColorVector *out[],
@@ -232,7 +234,7 @@ int QOPX_D3_asqtad_invert_multi_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqta
   QOP_REQUIRE(nmass >= 1 && nmass <= MS_MAXSHIFT && nsrc >= 0 && capacity >= 1, "invert_multi_batch: sizes");
   if (nsrc == 0) return 0;
   const int sp = prec == 16 ? P16 : P32;
-  if (ms_tol <= 0) ms_tol = sp == P16 ? 2e-3 : 1e-5;
+  if (ms_tol <= 0) ms_tol = 1e-7;
   double t0 = wall_time();
   Lattice *lat = L.lat;
   const int n = nsrc * nmass;
@@ -241,7 +243,7 @@ int QOPX_D3_asqtad_invert_multi_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqta
   long ms_total = 0;
   {
     const int cap = std::min(capacity, nsrc);
-    MultiShiftBatch ms(L, eo, cap, sp, masses, nmass);
+    MultiShiftBatch ms(L, eo, cap, P32, masses, nmass);
     for (int s0 = 0; s0 < nsrc; s0 += cap) {
       const int cnt = std::min(cap, nsrc - s0);
       for (int b = 0; b < cnt; b++) ms.load(b, in[s0 + b]->v, ms_tol, inv_arg->max_iter);
