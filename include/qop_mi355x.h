@@ -290,9 +290,11 @@ int QOPX_D3_asqtad_invert_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla
 /* Multi-mass solves of nsrc sources over one mass list (nmass <= 16):
  * out[i*nmass+j] = m_j A_j^-1 in[i] on the EVEN or ODD subset, each to FP64
  * true relative residual sqrt(res[i*nmass+j]->rsqmin).  An FP32 multi-shift
- * CG batched over the sources reaches relative residual ms_tol (0 selects
- * 1e-7), then every (source, mass) is refined by the reliable-update CG in
- * sloppy precision prec (16 or 32), `capacity` systems at a time.
+ * CG with FP64 reliable updates, batched over the sources, reaches relative
+ * residual ms_tol (0 selects the smallest requested tolerance); every
+ * (source, mass) whose FP64 true residual is still above its tolerance is
+ * then refined by the reliable-update CG in sloppy precision prec (16 or
+ * 32), `capacity` systems at a time.
  * ms_iters[i] (optional): multi-shift iterations of source i.  Returns the
  * number of systems that converged. */
 int QOPX_D3_asqtad_invert_multi_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla,

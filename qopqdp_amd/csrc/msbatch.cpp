@@ -29,6 +29,11 @@ MultiShiftBatch::MultiShiftBatch(Links &L, int eo, int capacity, int sloppy_prec
   devbatch_alloc(ts_, lat_, sp_, cap_);
   devbatch_alloc(pm_, lat_, sp_, cap_ * ns_);
   devbatch_alloc(xm_, lat_, P32, cap_ * ns_);
+  devbatch_alloc(x64_, lat_, P64, cap_ * ns_);
+  devbatch_alloc(t64_, lat_, P64, cap_);
+  devbatch_alloc(ax64_, lat_, P64, cap_);
+  QOP_CHECK_HIP(hipMalloc(&ru_, cap_ * sizeof(int)));
+  QOP_CHECK_HIP(hipMemset(ru_, 0, cap_ * sizeof(int)));
   QOP_CHECK_HIP(hipMalloc(&slot_, cap_ * sizeof(MsSlot)));
   QOP_CHECK_HIP(hipHostMalloc(&slot_h_, cap_ * sizeof(MsSlot), hipHostMallocDefault));
   QOP_CHECK_HIP(hipMalloc(&active_, 2 * cap_ * sizeof(int)));
@@ -49,14 +54,17 @@ MultiShiftBatch::MultiShiftBatch(Links &L, int eo, int capacity, int sloppy_prec
   for (int b = 0; b < cap_; b++) { hc[2 * b] = -1.0; hc[2 * b + 1] = m0 * m0; }
   QOP_CHECK_HIP(hipMemcpy(coef_, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice));
   L_.get(sp_);  // convert the links before any graph capture
+  L_.get(P64);
   shape_ = tune_hop(L_, sp_, cap_, ps_, ts_, 1 - eo_);
+  ru_shape_ = tune_hop(L_, P64, cap_, x64_, t64_, 1 - eo_);
   sync();
 }
 
 MultiShiftBatch::~MultiShiftBatch() {
   sync();
   if (graph_) (void)hipGraphExecDestroy(graph_);
-  for (DevBatch *b : {&b64_, &rs_, &ps_, &ts_, &pm_, &xm_}) devbatch_free(*b);
+  for (DevBatch *b : {&b64_, &rs_, &ps_, &ts_, &pm_, &xm_, &x64_, &t64_, &ax64_}) devbatch_free(*b);
+  (void)hipFree(ru_);
   (void)hipFree(slot_);
   (void)hipHostFree(slot_h_);
   (void)hipFree(active_);
@@ -86,6 +94,7 @@ void MultiShiftBatch::load(int slot, const DevVec &src, double ms_tol, int max_i
   for (int i = 0; i < ns_; i++) {
     const int k = i * cap_ + slot;
     QOP_CHECK_HIP(hipMemsetAsync(xm_.slot(k).data, 0, 3 * Vh * sizeof(float2), st));
+    QOP_CHECK_HIP(hipMemsetAsync(x64_.slot(k).data, 0, 3 * Vh * sizeof(double2), st));
     if (i != imin_) copy_view(pm_.slot(k), sp_, rs_.slot(slot), sp_);
   }
   // |b|^2 (FP64), |r|^2 and |p|^2 of the stored sloppy vectors
@@ -116,6 +125,8 @@ void MultiShiftBatch::load(int slot, const DevVec &src, double ms_tol, int max_i
   s.max_iter = max_iter;
   s.b2 = h[slot];
   s.r2 = h[cap_ + slot];  // |r|^2 = |p|^2 of the stored sloppy r
+  s.r2_max = s.r2;
+  s.delta2 = options().delta * options().delta;
   const double m02 = masses_[imin_] * masses_[imin_];
   s.m2 = m02;
   for (int i = 0; i < ns_; i++) {
@@ -169,9 +180,45 @@ void MultiShiftBatch::enqueue_iteration() {
   b.cg_stride = (int)(sizeof(MsSlot) / sizeof(double));
   b.partial = partC_;
   launch_hop_args(b, sp_, ls.nl, shape_.nb, shape_.block, RED_CG, true, st);
-  ms_launch_scalarB(slot_, active_, partC_, nch_, cap_, st);
+  ms_launch_scalarB(slot_, active_, partC_, nch_, cap_, ru_, st);
   ms_launch_update(slot_, active_, sp_, (int)Vh, rs_.slot(0), ps_.slot(0), pm_.slot(0), xm_.slot(0),
                    3 * Vh, 3 * Vh * cap_, cap_, ns_, partB_, st);
+  enqueue_reliable_update();
+}
+
+// Masked by ru_: slots without a request skip every kernel (a masked hop group
+// with no flagged slot reads nothing).
+void MultiShiftBatch::enqueue_reliable_update() {
+  hipStream_t st = stream();
+  LinkSet &ls = L_.get(P64);
+  const int opp = 1 - eo_;
+  const long Vh = lat_->Vh;
+  ms_launch_ru_fold(ru_, (int)Vh, x64_.slot(0), xm_.slot(0), 3 * Vh, cap_, ns_, st);
+  HopArgs a;
+  a.g = lat_->g;
+  a.parity = opp;
+  a.links = ls.parity_data(lat_, opp);
+  a.lscale = ls.parity_scale(lat_, opp);
+  a.src = x64_.slot(imin_ * cap_);
+  a.dst = t64_.slot(0);
+  a.sstride = a.dstride = a.xstride = 3 * Vh;
+  a.nstride = Vh;
+  a.n = cap_;
+  a.active = ru_;
+  ru_shape_.apply(a);
+  launch_hop_args(a, P64, ls.nl, ru_shape_.nb, ru_shape_.block, RED_NONE, false, st);
+  HopArgs b = a;
+  b.parity = eo_;
+  b.links = ls.parity_data(lat_, eo_);
+  b.lscale = ls.parity_scale(lat_, eo_);
+  b.src = t64_.slot(0);
+  b.dst = ax64_.slot(0);
+  b.x = x64_.slot(imin_ * cap_);
+  b.coef = coef_;  // A_0 x = m_0^2 x - D t
+  launch_hop_args(b, P64, ls.nl, ru_shape_.nb, ru_shape_.block, RED_NONE, true, st);
+  ms_launch_ru_resid(ru_, sp_, (int)Vh, b64_.slot(0), ax64_.slot(0), rs_.slot(0), 3 * Vh, cap_,
+                     partC_, st);
+  ms_launch_ru_scalar(slot_, ru_, partC_, nch_, cap_, st);
 }
 
 void MultiShiftBatch::run() {
@@ -197,14 +244,19 @@ void MultiShiftBatch::run() {
 }
 
 void MultiShiftBatch::solution(int slot, int k, DevVec &out) {
+  QOP_REQUIRE(out.prec == P64, "multi-shift: solutions are FP64");
   LincombArgs a;
   a.Vh = (int)lat_->Vh;
   a.out = out.half(eo_);
-  a.v[0] = xm_.slot(k * cap_ + slot);
+  a.v[0] = x64_.slot(k * cap_ + slot);  // folded at the reliable updates
+  a.v[1] = xm_.slot(k * cap_ + slot);   // increments since the last one
   a.cr[0] = 1.0;
-  int pin[1] = {P32};
-  launch_lincomb(a, out.prec, pin, 1, false, stream());
+  a.cr[1] = 1.0;
+  int pin[2] = {P64, P32};
+  launch_lincomb(a, P64, pin, 2, false, stream());
 }
+
+int MultiShiftBatch::updates(int slot) { return slot_h_[slot].nupdate; }
 
 int MultiShiftBatch::iterations(int slot) { return slot_h_[slot].iter; }
 
@@ -217,9 +269,10 @@ extern "C" {
 // Multi-mass solves of nsrc sources over one shared mass list:
 // out[i*nmass + j] = m_j A_j^-1 in[i] on the EVEN or ODD subset, each to FP64
 // true relative residual sqrt(res[i*nmass+j]->rsqmin).  An FP32 multi-shift CG
-// batched over the sources runs to relative residual ms_tol (0: 1e-7), then
-// every (source, mass) is refined from its multi-shift solution by the
-// reliable-update CG in sloppy precision `prec`, `capacity` systems at a
+// with reliable updates, batched over the sources, runs to relative residual
+// ms_tol (0: the smallest requested tolerance); then every (source, mass) is
+// checked in FP64 and, where needed, refined from its multi-shift solution by
+// the reliable-update CG in sloppy precision `prec`, `capacity` systems at a
 // time.  (An FP16 multi-shift stage stalls near 1e-3 and leaves most of the
 // work to the refinement.)
 int QOPX_D3_asqtad_invert_multi_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla,
@@ -234,7 +287,11 @@ int QOPX_D3_asqtad_invert_multi_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqta
   QOP_REQUIRE(nmass >= 1 && nmass <= MS_MAXSHIFT && nsrc >= 0 && capacity >= 1, "invert_multi_batch: sizes");
   if (nsrc == 0) return 0;
   const int sp = prec == 16 ? P16 : P32;
-  if (ms_tol <= 0) ms_tol = 1e-7;
+  // with reliable updates the multi-shift stage can go to the requested
+  // tolerance; the refinement then only finishes shifts whose solution drifted
+  double tol_min = 1.0;
+  for (int k = 0; k < nsrc * nmass; k++) tol_min = std::min(tol_min, std::sqrt(res[k]->rsqmin));
+  if (ms_tol <= 0) ms_tol = tol_min;
   double t0 = wall_time();
   Lattice *lat = L.lat;
   const int n = nsrc * nmass;

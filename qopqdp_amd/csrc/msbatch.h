@@ -7,7 +7,11 @@
 // source.  Vectors are in a sloppy precision (FP32, or FP16 with per-site
 // norms) with FP32 solution accumulators and FP64 reductions; a shift stops
 // being updated once its estimated residual zeta_i^2 |r|^2 reaches its
-// sloppy target.  The multi-shift solutions then seed FP64 reliable-update
+// target.  Reliable updates: whenever a source's iterated residual has
+// dropped by `delta` since its last update, every shift's FP32 accumulator is
+// folded into its FP64 solution and the base residual is recomputed in FP64,
+// r = b - A_0 x_0, and replaces the sloppy one (the shifted residuals stay
+// zeta_i r).  The multi-shift solutions then seed FP64 reliable-update
 // refinements (BatchSolver), so every (source, mass) reaches its FP64 true
 // residual.
 #pragma once
@@ -26,6 +30,9 @@ struct MsSlot {
   double r2, r2_old, pAp, b2;
   double m2;         // base mass^2
   double beta0;      // b[imin] of the current iteration (the hop epilogue reads it)
+  double r2_max;     // |r|^2 at the last reliable update
+  double delta2;     // reliable-update threshold (delta^2)
+  int nupdate, pad_;
   double stop2[MS_MAXSHIFT];  // per-shift sloppy target on zeta^2 |r|^2
   double shift[MS_MAXSHIFT];
   double a[MS_MAXSHIFT], b[MS_MAXSHIFT], bo[MS_MAXSHIFT];
@@ -34,8 +41,16 @@ struct MsSlot {
 
 void ms_launch_scalarA(MsSlot *s, const int *active, const double *partA, const double *partB,
                        int nch, int n, hipStream_t st);
-void ms_launch_scalarB(MsSlot *s, int *active, const double *partC, int nch, int n,
+void ms_launch_scalarB(MsSlot *s, int *active, const double *partC, int nch, int n, int *ru,
                        hipStream_t st);
+// Reliable update of the slots flagged in ru: fold x64[k] += xs[k], xs[k] = 0
+// for every shift (k = shift * cap + slot); residual r = b - ax into rs with
+// |r|^2 partials; scalars (r2, r2_max, clear the flag).
+void ms_launch_ru_fold(const int *ru, int Vh, VecView x64, VecView xs, long stride, int cap, int ns,
+                       hipStream_t st);
+void ms_launch_ru_resid(const int *ru, int prec, int Vh, VecView b64, VecView ax64, VecView rs,
+                        long stride, int cap, double *part, hipStream_t st);
+void ms_launch_ru_scalar(MsSlot *s, int *ru, const double *part, int nch, int n, hipStream_t st);
 void ms_launch_update(const MsSlot *s, const int *active, int prec, int Vh, VecView rs,
                       VecView ps, VecView pm, VecView xm, long stride, long sstride, int cap,
                       int ns, double *partB, hipStream_t st);
@@ -54,9 +69,11 @@ class MultiShiftBatch {
   // x of (slot, mass index) into out's eo half (FP64, no mass factor)
   void solution(int slot, int k, DevVec &out);
   int iterations(int slot);
+  int updates(int slot);  // reliable updates done
 
  private:
   void enqueue_iteration();
+  void enqueue_reliable_update();
   Links &L_;
   Lattice *lat_;
   int eo_, cap_, sp_, ns_, imin_;
@@ -64,7 +81,11 @@ class MultiShiftBatch {
   std::vector<double> masses_;
   DevBatch b64_, rs_, ps_, ts_;
   DevBatch pm_;  // [ns][cap] sloppy shifted directions (index imin unused)
-  DevBatch xm_;  // [ns][cap] FP32 solutions
+  DevBatch xm_;  // [ns][cap] FP32 solution increments since the last reliable update
+  DevBatch x64_; // [ns][cap] FP64 solutions
+  DevBatch t64_, ax64_;  // [cap] FP64 hop temporaries of the reliable update
+  int *ru_ = nullptr;    // [cap] reliable update requested
+  HopShape ru_shape_;
   MsSlot *slot_ = nullptr, *slot_h_ = nullptr;
   int *active_ = nullptr, *active_h_ = nullptr;
   int *sel_ = nullptr;

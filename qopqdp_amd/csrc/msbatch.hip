@@ -83,7 +83,7 @@ ms_scalarA(MsSlot *S, int *active, const double *partA, const double *partB, int
 
 // new |r|^2, alpha, per-shift stopping, shift-over of zeta/beta
 __global__ void __launch_bounds__(1024)
-ms_scalarB(MsSlot *S, int *active, const double *partC, int nch) {
+ms_scalarB(MsSlot *S, int *active, const double *partC, int nch, int *ru) {
   __shared__ double sh[17];
   const int b = blockIdx.x;
   if (!active[b]) return;
@@ -107,6 +107,87 @@ ms_scalarB(MsSlot *S, int *active, const double *partC, int nch) {
   }
   for (int i = 0; i < s.ns; i++) { s.bo[i] = s.b[i]; s.zo[i] = s.z[i]; s.z[i] = s.zn[i]; }
   if (!any || s.iter >= s.max_iter) s.done_pending = 1;
+  // reliable update once the residual has dropped by delta since the last one
+  if (!s.done_pending && r2 < s.delta2 * s.r2_max) ru[b] = 1;
+}
+
+// x64[k] += xs[k], xs[k] = 0 for every shift of a flagged slot
+__global__ void __launch_bounds__(256)
+ms_ru_fold(const int *ru, int Vh, VecView x64, VecView xs, long stride, int cap) {
+  const int k = blockIdx.y;
+  if (!ru[k % cap]) return;
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= Vh) return;
+  double2 *x = (double2 *)x64.data + (size_t)k * stride;
+  float2 *y = (float2 *)xs.data + (size_t)k * stride;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const float2 v = y[c * Vh + h];
+    double2 t = x[c * Vh + h];
+    t.x += (double)v.x;
+    t.y += (double)v.y;
+    x[c * Vh + h] = t;
+    y[c * Vh + h] = make_float2(0.f, 0.f);
+  }
+}
+
+// r = b - ax (FP64) stored as the sloppy residual, |r|^2 partials
+template <int P>
+__global__ void __launch_bounds__(256)
+ms_ru_resid(const int *ru, int Vh, VecView b64, VecView ax64, VecView rs, long stride,
+            double *part) {
+  using R = typename MT<P>::R;
+  const int b = blockIdx.y;
+  if (!ru[b]) return;
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0;
+  if (h < Vh) {
+    const double2 *bb = (const double2 *)b64.data + (size_t)b * stride;
+    const double2 *aa = (const double2 *)ax64.data + (size_t)b * stride;
+    R r[6];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const double2 u = bb[c * Vh + h], w = aa[c * Vh + h];
+      const double xr = u.x - w.x, xi = u.y - w.y;
+      acc += xr * xr + xi * xi;
+      r[2 * c] = (R)xr;
+      r[2 * c + 1] = (R)xi;
+    }
+    const size_t cb = sizeof(typename MT<P>::S);
+    char *d = (char *)rs.data + (size_t)b * stride * cb;
+    if constexpr (P == P16) {
+      float *nm = rs.norm + (long)b * Vh;
+      float m = 0.f;
+#pragma unroll
+      for (int i = 0; i < 6; i++) m = fmaxf(m, fabsf(r[i]));
+      const float inv = m > 0.f ? 1.f / m : 0.f;
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        ((__half2 *)d)[c * Vh + h] = __floats2half2_rn(r[2 * c] * inv, r[2 * c + 1] * inv);
+      nm[h] = m;
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; c++) ((float2 *)d)[c * Vh + h] = make_float2(r[2 * c], r[2 * c + 1]);
+    }
+  }
+  const int nch = (Vh + 63) / 64;
+  acc = msum(acc);
+  if ((threadIdx.x & 63) == 0 && (h >> 6) < nch) part[(long)b * nch + (h >> 6)] = acc;
+}
+
+// the true |r|^2 replaces the iterated one
+__global__ void __launch_bounds__(1024)
+ms_ru_scalar(MsSlot *S, int *ru, const double *part, int nch) {
+  __shared__ double sh[17];
+  const int b = blockIdx.x;
+  if (!ru[b]) return;
+  const double r2 = ms_block_sum(part + (long)b * nch, nch, sh);
+  if (threadIdx.x != 0) return;
+  MsSlot &s = S[b];
+  s.r2 = r2;
+  s.r2_max = r2;
+  s.nupdate++;
+  ru[b] = 0;
 }
 
 // x_i += b_i p_i (old p_i), p_i = zn_i r + a_i p_i (live shifts), p = r + a_0 p,
@@ -198,8 +279,26 @@ void ms_launch_scalarA(MsSlot *s, const int *active, const double *partA, const 
   hipLaunchKernelGGL(ms_scalarA, dim3(n), dim3(MS_THREADS), 0, st, s, const_cast<int *>(active), partA, partB, nch);
   QOP_CHECK_HIP(hipGetLastError());
 }
-void ms_launch_scalarB(MsSlot *s, int *active, const double *partC, int nch, int n, hipStream_t st) {
-  hipLaunchKernelGGL(ms_scalarB, dim3(n), dim3(MS_THREADS), 0, st, s, active, partC, nch);
+void ms_launch_scalarB(MsSlot *s, int *active, const double *partC, int nch, int n, int *ru,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(ms_scalarB, dim3(n), dim3(MS_THREADS), 0, st, s, active, partC, nch, ru);
+  QOP_CHECK_HIP(hipGetLastError());
+}
+void ms_launch_ru_fold(const int *ru, int Vh, VecView x64, VecView xs, long stride, int cap, int ns,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(ms_ru_fold, dim3((Vh + 255) / 256, cap * ns), dim3(256), 0, st, ru, Vh, x64, xs,
+                     stride, cap);
+  QOP_CHECK_HIP(hipGetLastError());
+}
+void ms_launch_ru_resid(const int *ru, int prec, int Vh, VecView b64, VecView ax64, VecView rs,
+                        long stride, int cap, double *part, hipStream_t st) {
+  dim3 g((Vh + 255) / 256, cap), bl(256);
+  if (prec == P16) hipLaunchKernelGGL(ms_ru_resid<P16>, g, bl, 0, st, ru, Vh, b64, ax64, rs, stride, part);
+  else hipLaunchKernelGGL(ms_ru_resid<P32>, g, bl, 0, st, ru, Vh, b64, ax64, rs, stride, part);
+  QOP_CHECK_HIP(hipGetLastError());
+}
+void ms_launch_ru_scalar(MsSlot *s, int *ru, const double *part, int nch, int n, hipStream_t st) {
+  hipLaunchKernelGGL(ms_ru_scalar, dim3(n), dim3(MS_THREADS), 0, st, s, ru, part, nch);
   QOP_CHECK_HIP(hipGetLastError());
 }
 void ms_launch_update(const MsSlot *s, const int *active, int prec, int Vh, VecView rs, VecView ps,
