@@ -264,6 +264,9 @@ QOP_status_t QOP_asqtad_invert_set_opts(QOP_opt_t opts[], int nopts) {
   get("qmx_delta", delta);
   get("qmx_chunk", chunk);
   get("qmx_graph", graph);
+  int msp = o.ms_prec == P16 ? 16 : 32;
+  get("qmx_ms_prec", msp);
+  if (msp != 16 && msp != 32) return QOP_FAIL;
   if (solver < 0 || solver > 1) return QOP_FAIL;
   if (prec != 64 && prec != 32 && prec != 16) return QOP_FAIL;
   if (!(delta > 0 && delta < 1) || chunk < 1 || chunk > 1024) return QOP_FAIL;
@@ -274,6 +277,7 @@ QOP_status_t QOP_asqtad_invert_set_opts(QOP_opt_t opts[], int nopts) {
   o.delta = delta;
   o.chunk = chunk;
   o.use_graph = graph;
+  o.ms_prec = msp == 16 ? P16 : P32;
   return QOP_SUCCESS;
 }
 

@@ -268,8 +268,9 @@ extern "C" {
 
 // Multi-mass solves of nsrc sources over one shared mass list:
 // out[i*nmass + j] = m_j A_j^-1 in[i] on the EVEN or ODD subset, each to FP64
-// true relative residual sqrt(res[i*nmass+j]->rsqmin).  An FP32 multi-shift CG
-// with reliable updates, batched over the sources, runs to relative residual
+// true relative residual sqrt(res[i*nmass+j]->rsqmin).  A multi-shift CG
+// (FP32 vectors; FP16 with option qmx_ms_prec = 16) with reliable updates,
+// batched over the sources, runs to relative residual
 // ms_tol (0: the smallest requested tolerance); then every (source, mass) is
 // checked in FP64 and, where needed, refined from its multi-shift solution by
 // the reliable-update CG in sloppy precision `prec`, `capacity` systems at a
@@ -300,7 +301,7 @@ int QOPX_D3_asqtad_invert_multi_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqta
   long ms_total = 0;
   {
     const int cap = std::min(capacity, nsrc);
-    MultiShiftBatch ms(L, eo, cap, P32, masses, nmass);
+    MultiShiftBatch ms(L, eo, cap, options().ms_prec, masses, nmass);
     for (int s0 = 0; s0 < nsrc; s0 += cap) {
       const int cnt = std::min(cap, nsrc - s0);
       for (int b = 0; b < cnt; b++) ms.load(b, in[s0 + b]->v, ms_tol, inv_arg->max_iter);

@@ -51,6 +51,7 @@ struct Options {
   double delta = 0.1;    // reliable-update residual drop
   int chunk = 16;        // iterations enqueued between host status checks
   int use_graph = 1;     // capture the iteration chunk in a hipGraph
+  int ms_prec = P32;     // vector precision of the batched multi-shift stage
   int verbose = 0;
 };
 Options &options();
