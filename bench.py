@@ -103,7 +103,7 @@ def main():
     if args.roofline_only:
         res = {}
         for red in (1, 2):
-            shape = (C.c_int * 7)()
+            shape = (C.c_int * 8)()
             ms = s.lib.QOPX_D3_asqtad_hop_time(fla, args.prec, B, 50, red, shape)
             byts = hop_bytes_per_site(args.prec, B, red) * Vh
             res[f"red{red}"] = {"kernel_us": round(ms * 1e3, 1), "GBs": round(byts / ms / 1e6, 1),
@@ -191,7 +191,7 @@ def main():
 
     # dominant kernel roofline: the second hop of the batched CG iteration
     # (fused CG update); the first hop beside it
-    shape, shape1 = (C.c_int * 7)(), (C.c_int * 7)()
+    shape, shape1 = (C.c_int * 8)(), (C.c_int * 8)()
     nrep = 50
     hop_ms = s.lib.QOPX_D3_asqtad_hop_time(fla, args.prec, B, nrep, 2, shape)
     hop_bytes = hop_bytes_per_site(args.prec, B, 2) * Vh

@@ -35,6 +35,11 @@ static bool shape_valid(const Lattice *lat, int prec, int nl, int nrhs, const Ho
   if (s.block != 64 && s.block != 128 && s.block != 256) return false;
   if (s.tile && (s.tile % s.block != 0 || slab % s.tile != 0 || s.tile >= slab)) return false;
   if (s.lds < 0 || s.lds > 65536) return false;
+  if (s.walk) {
+    const int L0 = lat->L[0];
+    if (L0 < 8 || (L0 / 2) % s.walk || slab % 64 || s.nt || s.tile) return false;
+    if (s.nb != 1 && s.nb != 2 && s.nb != 4) return false;
+  }
   return true;
 }
 
@@ -123,6 +128,17 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
     c.lds = lds;
     time_shape(c);
   }
+  // stage 4: the x0-walking kernel (register ring of x0 neighbours)
+  const int L0 = lat->L[0];
+  if (L0 >= 8 && slab % 64 == 0)
+    for (int wnb : {1, 2, 4}) {
+      if (wnb > 1 && wnb > nrhs) continue;
+      for (int wblk : blocks)
+        for (int seg : {4, 8, 16}) {
+          if ((L0 / 2) % seg) continue;
+          time_shape(HopShape{wnb, wblk, 1, 0, nrhs > wnb ? 1 : 0, 0, 0, seg});
+        }
+    }
   QOP_CHECK_HIP(hipEventDestroy(e0));
   QOP_CHECK_HIP(hipEventDestroy(e1));
   if (scratch) {
@@ -130,9 +146,9 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
     QOP_CHECK_HIP(hipFree(scratch));
   }
   if (options().verbose)
-    printf("TUNE: prec %d nrhs %d -> nb %d block %d remap %d nt %d inner %d tile %d lds %d (%.3f ms/hop)\n",
-           prec, nrhs, best.nb, best.block, best.remap, best.nt, best.rhs_inner, best.tile,
-           best.lds, best_ms / reps);
+    printf("TUNE: prec %d nrhs %d -> nb %d block %d remap %d nt %d inner %d tile %d lds %d walk %d "
+           "(%.3f ms/hop)\n", prec, nrhs, best.nb, best.block, best.remap, best.nt, best.rhs_inner,
+           best.tile, best.lds, best.walk, best_ms / reps);
   cache[key] = best;
   return best;
 }

@@ -30,6 +30,7 @@ struct HopArgs {
   int nt = 0;                    // non-temporal link loads
   int rhs_inner = 0;             // RHS groups innermost in the block order
   int tile = 0;                  // x0-tiled site order: sites per tile of an x0 slab (0: plain)
+  int walk = 0;                  // x0-walking kernel: sites per thread along x0 (0: off)
   int lds_reserve = 0;           // bytes of LDS reserved per block: caps blocks per CU, so
                                  // fewer sites are in flight per L2
   // single-stored links: `links` holds the forward links of the output

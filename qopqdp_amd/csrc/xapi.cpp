@@ -18,9 +18,9 @@ extern "C" {
 // hop of a batched CG iteration), red = 2 the instance with the fused CG
 // update r -= a (m^2 p - D t), x_FP32 += a p, |r|^2 (the second hop; sloppy
 // precisions only).  Returns the mean ms per launch from
-// HIP events on the library stream; shape[7] receives (rhs per thread, block,
+// HIP events on the library stream; shape[8] receives (rhs per thread, block,
 // XCD remap, non-temporal links, RHS-innermost block order, x0 tile, LDS
-// reserve).
+// reserve, x0 walk).
 double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, int nrep,
                                int red, int *shape) {
   Links &L = fla->L;
@@ -122,19 +122,20 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
     shape[0] = hs.nb; shape[1] = hs.block; shape[2] = hs.remap; shape[3] = hs.nt; shape[4] = hs.rhs_inner;
     shape[5] = hs.tile;
     shape[6] = hs.lds;
+    shape[7] = hs.walk;
   }
   return ms / nrep;
 }
 
 // Force the multi-RHS hop launch shape (rhs per thread, block, XCD remap,
-// non-temporal links, RHS-innermost order, x0 tile, LDS reserve) for solvers built from
+// non-temporal links, RHS-innermost order, x0 tile, LDS reserve, x0 walk) for solvers built from
 // now on, wherever it is valid; null restores autotuning.
 void QOPX_hop_shape_override(const int *shape) {
   if (!shape) {
     set_hop_shape_override(nullptr);
     return;
   }
-  HopShape s{shape[0], shape[1], shape[2], shape[3], shape[4], shape[5], shape[6]};
+  HopShape s{shape[0], shape[1], shape[2], shape[3], shape[4], shape[5], shape[6], shape[7]};
   set_hop_shape_override(&s);
 }
 

@@ -1,6 +1,7 @@
 """The batched solver's result does not depend on the multi-RHS hop launch
 shape (RHS per thread, block size, XCD remap, non-temporal loads and stores,
-block order, x0-tiled site order, LDS reservation): each forced shape gives bit-identical solutions
+block order, x0-tiled site order, LDS reservation, x0-walking kernel): each
+forced shape gives bit-identical solutions
 and iteration counts, all at FP64 true residual 1e-10 (oracle operator)."""
 import ctypes as C
 
@@ -13,13 +14,17 @@ from conftest import MILC
 pytestmark = pytest.mark.gpu
 EVEN = 0
 
-# (RHS per thread, block, XCD remap, non-temporal, RHS-innermost, x0 tile, LDS reserve)
+# (RHS per thread, block, XCD remap, non-temporal, RHS-innermost, x0 tile,
+#  LDS reserve, x0 walk)
 SHAPES = [
-    (4, 128, 1, 0, 1, 0, 0),
-    (4, 128, 1, 0, 1, 512, 0),
-    (2, 64, 0, 1, 0, 1024, 16384),
-    (1, 256, 1, 0, 0, 512, 0),
-    (8, 64, 1, 1, 1, 1024, 65536),
+    (4, 128, 1, 0, 1, 0, 0, 0),
+    (4, 128, 1, 0, 1, 512, 0, 0),
+    (2, 64, 0, 1, 0, 1024, 16384, 0),
+    (1, 256, 1, 0, 0, 512, 0, 0),
+    (8, 64, 1, 1, 1, 1024, 65536, 0),
+    (2, 128, 1, 0, 1, 0, 0, 4),
+    (4, 64, 1, 0, 1, 0, 0, 4),
+    (1, 256, 0, 0, 0, 0, 0, 4),
 ]
 
 
@@ -39,7 +44,7 @@ def test_batched_solves_independent_of_launch_shape(oracle):
         for prec in (16, 32):
             ref = None
             for shape in SHAPES:
-                s.lib.QOPX_hop_shape_override((C.c_int * 7)(*shape))
+                s.lib.QOPX_hop_shape_override((C.c_int * 8)(*shape))
                 ok, outs, ras = batch_api(s, fla, srcs, masses, 1e-10, prec, capacity=4)
                 assert ok == 4, shape
                 its = [r.final_iter for r in ras]
