@@ -34,7 +34,11 @@ static bool shape_valid(const Lattice *lat, int prec, int nl, int nrhs, const Ho
   if (s.nb > 1 && s.nb > nrhs) return false;
   if (s.block != 64 && s.block != 128 && s.block != 256) return false;
   if (s.tile && (s.tile % s.block != 0 || slab % s.tile != 0 || s.tile >= slab)) return false;
-  if (s.lds < 0 || s.lds > 65536) return false;
+  if (s.stage) {
+    if (s.stage != 1 || prec == P64 || nl != 16 || s.nb > 8 || s.nt || s.tile || s.walk) return false;
+    const size_t el = prec == P16 ? 16 : 24;  // stored bytes per site and RHS
+    if ((size_t)s.nb * s.block * el > 65536) return false;
+  }
   if (s.walk) {
     const int L0 = lat->L[0];
     if (L0 < 8 || (L0 / 2) % s.walk || slab % 64 || s.nt || s.tile) return false;
@@ -120,13 +124,19 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
           if (tl && (tl % blk != 0 || slab % tl != 0 || tl >= slab)) continue;
           time_shape(HopShape{nb, blk, rm, nt, ri, tl, 0});
         }
-  // stage 3: fewer blocks per CU (an LDS reservation), so fewer sites are in
-  // flight per XCD and more gathered neighbours are still in L2
-  const HopShape b2 = best;
-  for (int lds : {16384, 32768, 65536}) {
-    HopShape c = b2;
-    c.lds = lds;
-    time_shape(c);
+  // stage 3: LDS staging of the block's source rows, for the best RHS per
+  // thread / block / order (plain site order and link loads)
+  if (prec != P64 && L.nl == 16) {
+    const HopShape b2 = best;
+    for (int sblk : blocks) {
+      HopShape c = b2;
+      c.block = sblk;
+      c.nt = 0;
+      c.tile = 0;
+      c.stage = 1;
+      if (c.nb > 8) c.nb = 8;
+      if (shape_valid(lat, prec, L.nl, nrhs, c)) time_shape(c);
+    }
   }
   // stage 4: the x0-walking kernel (register ring of x0 neighbours)
   const int L0 = lat->L[0];
@@ -146,9 +156,9 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
     QOP_CHECK_HIP(hipFree(scratch));
   }
   if (options().verbose)
-    printf("TUNE: prec %d nrhs %d -> nb %d block %d remap %d nt %d inner %d tile %d lds %d walk %d "
+    printf("TUNE: prec %d nrhs %d -> nb %d block %d remap %d nt %d inner %d tile %d stage %d walk %d "
            "(%.3f ms/hop)\n", prec, nrhs, best.nb, best.block, best.remap, best.nt, best.rhs_inner,
-           best.tile, best.lds, best.walk, best_ms / reps);
+           best.tile, best.stage, best.walk, best_ms / reps);
   cache[key] = best;
   return best;
 }

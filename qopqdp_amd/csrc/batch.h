@@ -66,13 +66,13 @@ void bs_launch_sum(const double *part, int nch, int n, const int *sel,
 // Tuned hop launch shape for (lattice, precision, nl, nrhs), timed on first use
 // and cached (batch.cpp).
 struct HopShape {
-  int nb = 1, block = 256, remap = 1, nt = 0, rhs_inner = 0, tile = 0, lds = 0, walk = 0;
+  int nb = 1, block = 256, remap = 1, nt = 0, rhs_inner = 0, tile = 0, stage = 0, walk = 0;
   void apply(HopArgs &a) const {
     a.remap = remap;
     a.nt = nt;
     a.rhs_inner = rhs_inner;
     a.tile = tile;
-    a.lds_reserve = lds;
+    a.stage = stage;
     a.walk = walk;
   }
 };

@@ -31,8 +31,9 @@ struct HopArgs {
   int rhs_inner = 0;             // RHS groups innermost in the block order
   int tile = 0;                  // x0-tiled site order: sites per tile of an x0 slab (0: plain)
   int walk = 0;                  // x0-walking kernel: sites per thread along x0 (0: off)
-  int lds_reserve = 0;           // bytes of LDS reserved per block: caps blocks per CU, so
-                                 // fewer sites are in flight per L2
+  int stage = 0;                 // LDS staging: the block's own source-parity sites are
+                                 // loaded once into LDS and the neighbours inside that
+                                 // range (x3 and most x2 hops) are read from there
   // single-stored links: `links` holds the forward links of the output
   // parity, links_bck those of the source parity ([nl/2][9][Vh] each)
   int single = 0;

@@ -53,6 +53,26 @@ __device__ __forceinline__ void ld_c(const void *base, long i, float scl,
   }
 }
 
+// Colour vector from an LDS tile holding the stored words ([3][bs] complex
+// elements, FP16 norms [bs]): the same conversion as vload, so the values are
+// bit-identical to a load from memory.
+template <int P>
+__device__ __forceinline__ void tload(const typename PT<P>::S *td, const float *tn, int o, int bs,
+                                      typename PT<P>::R v[6]) {
+  float n = 1.0f;
+  if constexpr (P == P16) n = tn[o];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if constexpr (P == P16) {
+      float2 t = __half22float2(td[c * bs + o]);
+      v[2 * c] = t.x * n; v[2 * c + 1] = t.y * n;
+    } else {
+      const auto t = td[c * bs + o];
+      v[2 * c] = t.x; v[2 * c + 1] = t.y;
+    }
+  }
+}
+
 // Load the 3 colour components of site h from a parity-half view.
 template <int P>
 __device__ __forceinline__ void vload(const void *data, const float *norm, long h,
@@ -267,7 +287,12 @@ __device__ __forceinline__ void hop_epilogue(const HopArgs &a, int h, int Vh, in
 
 // ---------------------------------------------------------------------------
 // hop kernel
-template <int P, int NL, int NB, int RED, bool HASX, bool NT, bool SGL>
+// STG: LDS staging.  The block first loads the stored source words of the
+// source-parity sites [h0, h0 + blockDim) -- the same (x0, x1, x2) rows as its
+// output sites -- into LDS; a neighbour inside that range (all x3 hops, and
+// the x2 hops that stay inside the block's rows) is then read from LDS, the
+// rest from memory.  Plain site order only.
+template <int P, int NL, int NB, int RED, bool HASX, bool NT, bool SGL, bool STG>
 __global__ void __launch_bounds__(256)
 hop_kernel(const HopArgs a) {
   using R = typename PT<P>::R;
@@ -292,6 +317,28 @@ hop_kernel(const HopArgs a) {
   for (int b = 0; b < NB; b++) {
     act[b] = (b0 + b < a.n) && (!a.active || a.active[b0 + b]);
     any |= act[b];
+  }
+  using S = typename PT<P>::S;
+  const int bs = blockDim.x;
+  const int h0 = sb * bs;
+  S *td = nullptr;     // [NB][3][bs]
+  float *tn = nullptr;  // FP16 norms [NB][bs]
+  if constexpr (STG) {
+    extern __shared__ __align__(16) unsigned char stg_mem[];
+    td = (S *)stg_mem;
+    tn = (float *)(td + NB * 3 * bs);
+    const int hs = h0 + (int)threadIdx.x;
+    if (any && hs < Vh) {
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        if (!act[b]) continue;
+        const S *sd = (const S *)a.src.data + (long)(b0 + b) * a.sstride;
+#pragma unroll
+        for (int c = 0; c < 3; c++) td[(b * 3 + c) * bs + threadIdx.x] = sd[(long)c * Vh + hs];
+        if constexpr (P == P16) tn[b * bs + threadIdx.x] = a.src.norm[(long)(b0 + b) * a.nstride + hs];
+      }
+    }
+    __syncthreads();
   }
   // a group with no active slot reads nothing (masked launches are cheap)
   if (h < Vh && any) {
@@ -352,7 +399,13 @@ hop_kernel(const HopArgs a) {
           const char *sd = (const char *)a.src.data + (long)(b0 + b) * a.sstride * sizeof(typename PT<P>::S);
           const float *sn = a.src.norm ? a.src.norm + (long)(b0 + b) * a.nstride : nullptr;
           R v[6];
-          vload<P>(sd, sn, nb, Vh, v);
+          if constexpr (STG) {
+            const unsigned o = (unsigned)(nb - h0);
+            if (o < (unsigned)bs) tload<P>(td + b * 3 * bs, tn + b * bs, (int)o, bs, v);
+            else vload<P>(sd, sn, nb, Vh, v);
+          } else {
+            vload<P>(sd, sn, nb, Vh, v);
+          }
 #pragma unroll
           for (int r = 0; r < 3; r++) {
             R re = acc[b][2 * r], im = acc[b][2 * r + 1];
@@ -534,11 +587,19 @@ hop_walk_kernel(const HopArgs a) {
 
 int hop_partial_blocks(const Lattice *lat) { return (int)((lat->Vh + 63) / 64); }
 
-template <int P, int NL, int NB, bool NT, bool SGL = false>
+// LDS bytes of a staged block: NB RHS x blockDim sites x (3 stored complex
+// elements + the FP16 norm)
+static size_t stage_bytes(int prec, int nb, int block) {
+  const size_t el = prec == P64 ? 16 : 8;  // double2 or float2
+  return prec == P16 ? (size_t)nb * block * (3 * 4 + 4) : (size_t)nb * block * 3 * el;
+}
+
+template <int P, int NL, int NB, bool NT, bool SGL = false, bool STG = false>
 static void hop_dispatch_red(const HopArgs &a, dim3 grid, dim3 block, int red,
                              bool hasx, hipStream_t st) {
+  const size_t lds = STG ? stage_bytes(P, NB, (int)block.x) : 0;
 #define QOP_LAUNCH(RD, HX) \
-  hipLaunchKernelGGL((hop_kernel<P, NL, NB, RD, HX, NT, SGL>), grid, block, a.lds_reserve, st, a)
+  hipLaunchKernelGGL((hop_kernel<P, NL, NB, RD, HX, NT, SGL, STG>), grid, block, lds, st, a)
   if (red == RED_NONE) { if (hasx) QOP_LAUNCH(RED_NONE, true); else QOP_LAUNCH(RED_NONE, false); }
   else if (red == RED_OUT2) { if (hasx) QOP_LAUNCH(RED_OUT2, true); else QOP_LAUNCH(RED_OUT2, false); }
   else if (red == RED_CG) { QOP_LAUNCH(RED_CG, true); }
@@ -555,6 +616,19 @@ static void hop_dispatch_nb(const HopArgs &a, int nb, dim3 grid, dim3 block,
       hop_dispatch_red<P, NL, 1, false, true>(a, grid, block, red, hasx, st);
     } else {
       fatal(__FILE__, __LINE__, "hop: single-stored links exist in FP64 and FP32 only");
+    }
+    return;
+  }
+  if (a.stage) {
+    if constexpr (P != P64 && NL == 16) {
+      switch (nb) {
+        case 1: hop_dispatch_red<P, NL, 1, false, false, true>(a, grid, block, red, hasx, st); break;
+        case 2: hop_dispatch_red<P, NL, 2, false, false, true>(a, grid, block, red, hasx, st); break;
+        case 4: hop_dispatch_red<P, NL, 4, false, false, true>(a, grid, block, red, hasx, st); break;
+        default: hop_dispatch_red<P, NL, 8, false, false, true>(a, grid, block, red, hasx, st); break;
+      }
+    } else {
+      fatal(__FILE__, __LINE__, "hop: LDS staging exists for FP32/FP16 with Naik links");
     }
     return;
   }
@@ -587,7 +661,7 @@ template <int P, int NL>
 static void hop_walk_dispatch(const HopArgs &a, int nb, dim3 grid, dim3 block, int red, bool hasx,
                               hipStream_t st) {
 #define QOP_WLAUNCH(NB_, RD, HX) \
-  hipLaunchKernelGGL((hop_walk_kernel<P, NL, NB_, RD, HX>), grid, block, a.lds_reserve, st, a)
+  hipLaunchKernelGGL((hop_walk_kernel<P, NL, NB_, RD, HX>), grid, block, 0, st, a)
 #define QOP_WRED(NB_)                                                                  \
   if (red == RED_NONE) { if (hasx) QOP_WLAUNCH(NB_, RED_NONE, true); else QOP_WLAUNCH(NB_, RED_NONE, false); } \
   else if (red == RED_OUT2) { if (hasx) QOP_WLAUNCH(NB_, RED_OUT2, true); else QOP_WLAUNCH(NB_, RED_OUT2, false); } \
@@ -607,7 +681,12 @@ void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
   QOP_REQUIRE(red == RED_NONE || hasx || red == RED_OUT2, "hop: RED_X2 / RED_CG need x");
   QOP_REQUIRE(red != RED_CG || (a.cg_alpha && a.xs.data && prec != P64), "hop: RED_CG needs alpha and xs");
   QOP_REQUIRE(a.n >= 1, "hop: n >= 1");
-  QOP_REQUIRE(a.lds_reserve >= 0 && a.lds_reserve <= 65536, "hop: LDS reserve 0..64 KiB");
+  if (a.stage) {
+    QOP_REQUIRE(prec != P64 && nl == 16 && nb <= 8 && !a.single && !a.nt && !a.tile && !a.walk,
+                "hop: LDS staging needs FP32/FP16 Naik links, at most 8 RHS per thread, plain "
+                "site order and plain link loads");
+    QOP_REQUIRE(stage_bytes(prec, nb, block) <= 65536, "hop: LDS staging tile above 64 KiB");
+  }
   if (a.tile) {
     const int slab = a.g.Vh / a.g.L[0];
     QOP_REQUIRE(a.tile % block == 0 && slab % a.tile == 0, "hop: tile must divide the x0 slab");

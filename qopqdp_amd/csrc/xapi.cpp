@@ -20,7 +20,7 @@ extern "C" {
 // precisions only).  Returns the mean ms per launch from
 // HIP events on the library stream; shape[8] receives (rhs per thread, block,
 // XCD remap, non-temporal links, RHS-innermost block order, x0 tile, LDS
-// reserve, x0 walk).
+// staging, x0 walk).
 double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, int nrep,
                                int red, int *shape) {
   Links &L = fla->L;
@@ -121,7 +121,7 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
   if (shape) {
     shape[0] = hs.nb; shape[1] = hs.block; shape[2] = hs.remap; shape[3] = hs.nt; shape[4] = hs.rhs_inner;
     shape[5] = hs.tile;
-    shape[6] = hs.lds;
+    shape[6] = hs.stage;
     shape[7] = hs.walk;
   }
   return ms / nrep;

@@ -1,8 +1,10 @@
 """The batched solver's result does not depend on the multi-RHS hop launch
 shape (RHS per thread, block size, XCD remap, non-temporal loads and stores,
-block order, x0-tiled site order, LDS reservation, x0-walking kernel): each
-forced shape gives bit-identical solutions
-and iteration counts, all at FP64 true residual 1e-10 (oracle operator)."""
+block order, x0-tiled site order, LDS staging of the block's source rows,
+x0-walking kernel): each forced shape gives bit-identical solutions and
+iteration counts, all at FP64 true residual 1e-10 (oracle operator).  The
+second lattice has x3 rows of 5 sites, so staged blocks end inside a row and
+gathers switch between the LDS tile and memory within a wavefront."""
 import ctypes as C
 
 import numpy as np
@@ -15,22 +17,27 @@ pytestmark = pytest.mark.gpu
 EVEN = 0
 
 # (RHS per thread, block, XCD remap, non-temporal, RHS-innermost, x0 tile,
-#  LDS reserve, x0 walk)
+#  LDS staging, x0 walk)
 SHAPES = [
     (4, 128, 1, 0, 1, 0, 0, 0),
     (4, 128, 1, 0, 1, 512, 0, 0),
-    (2, 64, 0, 1, 0, 1024, 16384, 0),
+    (2, 64, 0, 1, 0, 1024, 0, 0),
     (1, 256, 1, 0, 0, 512, 0, 0),
-    (8, 64, 1, 1, 1, 1024, 65536, 0),
+    (8, 64, 1, 1, 1, 1024, 0, 0),
+    (4, 128, 1, 0, 1, 0, 1, 0),
+    (2, 64, 0, 0, 0, 0, 1, 0),
+    (8, 256, 1, 0, 1, 0, 1, 0),
+    (1, 64, 1, 0, 1, 0, 1, 0),
     (2, 128, 1, 0, 1, 0, 0, 4),
     (4, 64, 1, 0, 1, 0, 0, 4),
     (1, 256, 0, 0, 0, 0, 0, 4),
 ]
 
 
-def test_batched_solves_independent_of_launch_shape(oracle):
+# x0 slab of 2048 sites (tiles of 512 and 1024 apply); x3 rows of 5 sites
+@pytest.mark.parametrize("L", [[8, 16, 16, 16], [6, 6, 6, 10]])
+def test_batched_solves_independent_of_launch_shape(oracle, L):
     from test_gpu_invert import batch_api
-    L = [8, 16, 16, 16]  # x0 slab of 2048 sites: tiles of 512 and 1024 apply
     U = q.gen_gauge(L, 5001, 0.3)
     oracle.rephase_milc(L, U)
     fat, lng = oracle.make_imp_links(L, U, MILC)
