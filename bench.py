@@ -199,7 +199,7 @@ def main():
     hop1_ms = s.lib.QOPX_D3_asqtad_hop_time(fla, args.prec, B, nrep, 1, shape1)
     hop1_bytes = hop_bytes_per_site(args.prec, B, 1) * Vh
     kname = f"hop_kernel fp{args.prec} x{B} RHS, fused CG update"
-    traffic = measured_traffic(kname, L)
+    traffic = measured_traffic(kname, L, list(shape))
     # single-RHS bandwidth baseline (FP64 / FP32 / FP16 even->odd Dslash)
     single = {}
     for prec, sl, sv in ((64, 144, 48), (32, 72, 24), (16, 40, 16)):
@@ -282,14 +282,15 @@ def hop_bytes_per_site(prec, nrhs, red):
     return 16 * s_link + 2 * nrhs * s_vec
 
 
-def measured_traffic(kernel, L):
+def measured_traffic(kernel, L, shape):
     """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE runs summarised in profiles/traffic.json (null if absent)."""
+    WRITE_SIZE runs summarised in profiles/traffic.json; null if absent or
+    measured for another lattice or launch shape."""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
             t = json.load(f)
         e = t[kernel]
-        if list(e["lattice"]) != list(L):
+        if list(e["lattice"]) != list(L) or list(e["shape"]) != list(shape):
             return None
         return e["bytes_per_launch"]
     except (OSError, KeyError, ValueError):
