@@ -35,7 +35,7 @@ static bool shape_valid(const Lattice *lat, int prec, int nl, int nrhs, const Ho
   if (s.block != 64 && s.block != 128 && s.block != 256) return false;
   if (s.tile && (s.tile % s.block != 0 || slab % s.tile != 0 || s.tile >= slab)) return false;
   if (s.stage) {
-    if (s.stage != 1 || prec == P64 || nl != 16 || s.nb > 8 || s.nt || s.tile || s.walk) return false;
+    if (s.stage != 1 || prec == P64 || nl != 16 || s.nb > 8 || s.nt || s.walk) return false;
     const size_t el = prec == P16 ? 16 : 24;  // stored bytes per site and RHS
     if ((size_t)s.nb * s.block * el > 65536) return false;
   }
@@ -125,18 +125,19 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
           time_shape(HopShape{nb, blk, rm, nt, ri, tl, 0});
         }
   // stage 3: LDS staging of the block's source rows, for the best RHS per
-  // thread / block / order (plain site order and link loads)
+  // thread / order, every block size and x0 tile (plain link loads)
   if (prec != P64 && L.nl == 16) {
     const HopShape b2 = best;
-    for (int sblk : blocks) {
-      HopShape c = b2;
-      c.block = sblk;
-      c.nt = 0;
-      c.tile = 0;
-      c.stage = 1;
-      if (c.nb > 8) c.nb = 8;
-      if (shape_valid(lat, prec, L.nl, nrhs, c)) time_shape(c);
-    }
+    for (int sblk : blocks)
+      for (int tl : tiles) {
+        HopShape c = b2;
+        c.block = sblk;
+        c.nt = 0;
+        c.tile = tl;
+        c.stage = 1;
+        if (c.nb > 8) c.nb = 8;
+        if (shape_valid(lat, prec, L.nl, nrhs, c)) time_shape(c);
+      }
   }
   // stage 4: the x0-walking kernel (register ring of x0 neighbours)
   const int L0 = lat->L[0];

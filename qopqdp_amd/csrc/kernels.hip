@@ -291,7 +291,8 @@ __device__ __forceinline__ void hop_epilogue(const HopArgs &a, int h, int Vh, in
 // source-parity sites [h0, h0 + blockDim) -- the same (x0, x1, x2) rows as its
 // output sites -- into LDS; a neighbour inside that range (all x3 hops, and
 // the x2 hops that stay inside the block's rows) is then read from LDS, the
-// rest from memory.  Plain site order only.
+// rest from memory.  The block's sites are contiguous in both site orders
+// (an x0 tile is a whole number of blocks), from h0 = h - threadIdx.x.
 template <int P, int NL, int NB, int RED, bool HASX, bool NT, bool SGL, bool STG>
 __global__ void __launch_bounds__(256)
 hop_kernel(const HopArgs a) {
@@ -320,7 +321,7 @@ hop_kernel(const HopArgs a) {
   }
   using S = typename PT<P>::S;
   const int bs = blockDim.x;
-  const int h0 = sb * bs;
+  const int h0 = h - (int)threadIdx.x;
   S *td = nullptr;     // [NB][3][bs]
   float *tn = nullptr;  // FP16 norms [NB][bs]
   if constexpr (STG) {
@@ -682,9 +683,9 @@ void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
   QOP_REQUIRE(red != RED_CG || (a.cg_alpha && a.xs.data && prec != P64), "hop: RED_CG needs alpha and xs");
   QOP_REQUIRE(a.n >= 1, "hop: n >= 1");
   if (a.stage) {
-    QOP_REQUIRE(prec != P64 && nl == 16 && nb <= 8 && !a.single && !a.nt && !a.tile && !a.walk,
-                "hop: LDS staging needs FP32/FP16 Naik links, at most 8 RHS per thread, plain "
-                "site order and plain link loads");
+    QOP_REQUIRE(prec != P64 && nl == 16 && nb <= 8 && !a.single && !a.nt && !a.walk,
+                "hop: LDS staging needs FP32/FP16 Naik links, at most 8 RHS per thread and plain "
+                "link loads");
     QOP_REQUIRE(stage_bytes(prec, nb, block) <= 65536, "hop: LDS staging tile above 64 KiB");
   }
   if (a.tile) {

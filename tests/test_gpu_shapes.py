@@ -28,6 +28,8 @@ SHAPES = [
     (2, 64, 0, 0, 0, 0, 1, 0),
     (8, 256, 1, 0, 1, 0, 1, 0),
     (1, 64, 1, 0, 1, 0, 1, 0),
+    (4, 128, 1, 0, 1, 512, 1, 0),
+    (2, 256, 0, 0, 0, 1024, 1, 0),
     (2, 128, 1, 0, 1, 0, 0, 4),
     (4, 64, 1, 0, 1, 0, 0, 4),
     (1, 256, 0, 0, 0, 0, 0, 4),
