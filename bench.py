@@ -58,6 +58,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only time the dominant hop kernel (for the rocprofv3 cross-check)")
+    ap.add_argument("--shape", default=None,
+                    help="--roofline-only: force the hop launch shape (8 comma-separated ints: "
+                         "RHS per thread, block, XCD remap, non-temporal, RHS-innermost, x0 tile, "
+                         "LDS staging, x0 walk) instead of the autotuned one")
     ap.add_argument("--serve", action="store_true",
                     help="serving mix instead: Poisson arrivals over several lattices")
     ap.add_argument("--serve-lattices", default="16,16,16,16;24,24,24,48;32,32,32,64")
@@ -102,6 +106,10 @@ def main():
 
     if args.roofline_only:
         res = {}
+        if args.shape:
+            forced = [int(x) for x in args.shape.split(",")]
+            assert len(forced) == 8, "--shape takes 8 integers"
+            s.lib.QOPX_hop_shape_override((C.c_int * 8)(*forced))
         for red in (1, 2):
             shape = (C.c_int * 8)()
             ms = s.lib.QOPX_D3_asqtad_hop_time(fla, args.prec, B, 50, red, shape)
@@ -283,16 +291,16 @@ def hop_bytes_per_site(prec, nrhs, red):
 
 
 def measured_traffic(kernel, L, shape):
-    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE runs summarised in profiles/traffic.json; null if absent or
-    measured for another lattice or launch shape."""
+    """HBM bytes per launch of `kernel` with launch `shape` from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs summarised in profiles/traffic.json; null if
+    that lattice and shape were not measured."""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
             t = json.load(f)
         e = t[kernel]
-        if list(e["lattice"]) != list(L) or list(e["shape"]) != list(shape):
+        if list(e["lattice"]) != list(L):
             return None
-        return e["bytes_per_launch"]
+        return e["shapes"][",".join(str(int(v)) for v in shape)]["bytes_per_launch"]
     except (OSError, KeyError, ValueError):
         return None
 
