@@ -19,6 +19,12 @@
  *    A = m^2 - D_eo D_oe, and res_arg->final_rsq is |r|^2/|in|^2
  *    (README:24-48, lib/asqtad_invert_p.c:1-40, 603-610).
  *
+ * Device memory exhaustion is not fatal: an entry point with a QOP_info_t
+ * argument sets info->status = QOP_MEM_ERROR (include/qop_int.h:19) and
+ * returns without a result, the create_* functions return NULL, and the
+ * memory they had taken is given back; other failures print the HIP error
+ * and abort, as the reference exits on its fatal conditions.
+ *
  * The QDP-typed entry points (*_qdp) of the reference are not provided:
  * there is no QDP/C here.  All compute runs on the GPU (hand-written HIP
  * kernels for gfx950); there is no CPU path.
@@ -281,7 +287,9 @@ void QOPX_gen_gaussian_V(const int L[4], unsigned long long seed, int evenodd, d
  * finished slot takes the next system (continuous batching).  prec: sloppy
  * precision 32 or 16 (FP16 vectors with per-site norms, FP16 links with
  * per-link scales).  latency[i] (optional): seconds from the call's start to
- * system i's completion.  Returns the number of systems that converged. */
+ * system i's completion.  Returns the number of systems that converged, or
+ * -1 with info->status = QOP_MEM_ERROR when the batch workspace does not fit
+ * in device memory (nothing is kept allocated then). */
 int QOPX_D3_asqtad_invert_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla,
                                 QOP_invert_arg_t *inv_arg, QOP_resid_arg_t *res[], double masses[],
                                 QOP_D3_ColorVector *out[], QOP_D3_ColorVector *in[], int n,
@@ -296,7 +304,8 @@ int QOPX_D3_asqtad_invert_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla
  * then refined by the reliable-update CG in sloppy precision prec (16 or
  * 32), `capacity` systems at a time.
  * ms_iters[i] (optional): multi-shift iterations of source i.  Returns the
- * number of systems that converged. */
+ * number of systems that converged, or -1 with info->status = QOP_MEM_ERROR
+ * when the device memory is exhausted. */
 int QOPX_D3_asqtad_invert_multi_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla,
                                       QOP_invert_arg_t *inv_arg, QOP_resid_arg_t *res[],
                                       double masses[], int nmass, QOP_D3_ColorVector *out[],
@@ -305,11 +314,11 @@ int QOPX_D3_asqtad_invert_multi_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqta
 
 /* Mean ms per launch of the even->odd hop kernel (precision 64/32/16, nrhs
  * right-hand sides, autotuned shape (rhs per thread, block, XCD remap,
- * non-temporal links, RHS-innermost order, x0 tile, LDS staging, x0 walk) returned in shape[8]),
- * from HIP
- * events.  red = 0: plain hop; 1: with the fused |out|^2 reduction (first hop
- * of a batched CG iteration); 2: with the fused CG update
- * r -= a (m^2 p - D t), x += a p, |r|^2 (second hop; precision 32/16). */
+ * non-temporal links, RHS-innermost order, x0 tile, LDS staging, x0 walk)
+ * returned in shape[8]), from HIP events.  red = 0: plain hop; 1: with the
+ * fused |out|^2 reduction (first hop of a batched CG iteration); 2: with the
+ * fused CG update r -= a (m^2 p - D t), x += a p, |r|^2 (second hop;
+ * precision 32/16).  -1 if the operands do not fit in device memory. */
 double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, int nrep,
                                int red, int *shape);
 
@@ -320,7 +329,8 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
 void QOPX_hop_shape_override(const int *shape);
 
 /* Mean ms per launch of a plain device copy of `bytes` (multiple of 16):
- * the measured copy bandwidth and the calibration kernel for HBM counters. */
+ * the measured copy bandwidth and the calibration kernel for HBM counters.
+ * -1 if the buffers do not fit (as QOPX_D3_asqtad_hop_time). */
 double QOPX_copy_time(unsigned long long bytes, int nrep);
 
 /* ---- request server: continuous batching over resident configurations ----
@@ -344,7 +354,8 @@ typedef struct QOPX_Server_struct QOPX_Server;
 QOPX_Server *QOPX_serve_create(int capacity, int prec, unsigned long long max_bytes);
 /* Add a configuration from a raw FP64 gauge field (host): rephased with r0,
  * bc, sign (as QOP_D3_rephase_G; null pointers skip) and smeared with coeffs
- * on the device.  Returns the configuration index, or -1 if over budget. */
+ * on the device.  Returns the configuration index, or -1 if over budget or
+ * out of device memory. */
 int QOPX_serve_add_gauge(QOPX_Server *s, const int L[4], QOP_D_Real *U[4],
                          QOP_asqtad_coeffs_t *coeffs, int *r0, QOP_bc_t *bc,
                          QOP_staggered_sign_t *sign);

@@ -2,6 +2,7 @@
 // Each entry point follows the reference function cited beside it.
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "handles.h"
@@ -40,13 +41,12 @@ void upload_vec(DevVec &v, const T *raw, int eo) {
   Lattice *lat = v.lat;
   long b, e;
   eo_range(lat, eo, b, e);
-  T *d;
-  QOP_CHECK_HIP(hipMalloc(&d, (size_t)lat->V * 6 * sizeof(T)));
+  DevBuf<T> buf((size_t)lat->V * 6 * sizeof(T));
+  T *d = buf.p;
   QOP_CHECK_HIP(hipMemcpyAsync(d + 6 * b, raw + 6 * b, (e - b) * 6 * sizeof(T), hipMemcpyHostToDevice, stream()));
   if constexpr (sizeof(T) == 8) launch_raw_to_vec(lat, (const double *)d, v, eo, stream());
   else launch_rawf_to_vec(lat, (const float *)d, v, eo, stream());
   sync();
-  QOP_CHECK_HIP(hipFree(d));
 }
 
 template <typename T>
@@ -54,13 +54,12 @@ void download_vec(T *raw, const DevVec &v, int eo) {
   Lattice *lat = v.lat;
   long b, e;
   eo_range(lat, eo, b, e);
-  T *d;
-  QOP_CHECK_HIP(hipMalloc(&d, (size_t)lat->V * 6 * sizeof(T)));
+  DevBuf<T> buf((size_t)lat->V * 6 * sizeof(T));
+  T *d = buf.p;
   if constexpr (sizeof(T) == 8) launch_vec_to_raw(lat, v, (double *)d, eo, stream());
   else launch_vec_to_rawf(lat, v, (float *)d, eo, stream());
   QOP_CHECK_HIP(hipMemcpyAsync(raw + 6 * b, d + 6 * b, (e - b) * 6 * sizeof(T), hipMemcpyDeviceToHost, stream()));
   sync();
-  QOP_CHECK_HIP(hipFree(d));
 }
 
 // Host raw matrices (T) -> device FP64 raw matrices.
@@ -70,7 +69,7 @@ double *upload_mat(Lattice *lat, const T *raw, int eo) {
   eo_range(lat, eo, b, e);
   size_t n = (size_t)lat->V * 18;
   double *d;
-  QOP_CHECK_HIP(hipMalloc(&d, n * sizeof(double)));
+  dev_alloc(&d, n * sizeof(double));
   QOP_CHECK_HIP(hipMemsetAsync(d, 0, n * sizeof(double), stream()));
   if constexpr (sizeof(T) == 8) {
     QOP_CHECK_HIP(hipMemcpyAsync(d + 18 * b, raw + 18 * b, (e - b) * 18 * sizeof(double), hipMemcpyHostToDevice, stream()));
@@ -96,29 +95,27 @@ void download_mat(T *raw, const double *d, Lattice *lat, int eo) {
 template <typename T>
 void build_links_from_raw(Links &L, T *fat[], T *lng[], int eo) {
   const double *f[4], *l[4];
-  double *fd[4], *ld[4] = {nullptr, nullptr, nullptr, nullptr};
+  DevBuf<double> fd[4], ld[4];
   for (int i = 0; i < 4; i++) {
-    fd[i] = upload_mat(L.lat, fat[i], eo);
-    f[i] = fd[i];
-    if (lng) { ld[i] = upload_mat(L.lat, lng[i], eo); l[i] = ld[i]; }
+    fd[i].p = upload_mat(L.lat, fat[i], eo);
+    f[i] = fd[i].p;
+    if (lng) { ld[i].p = upload_mat(L.lat, lng[i], eo); l[i] = ld[i].p; }
   }
   build_links64(L, f, lng ? l : nullptr, stream());
-  for (int i = 0; i < 4; i++) {
-    QOP_CHECK_HIP(hipFree(fd[i]));
-    if (ld[i]) QOP_CHECK_HIP(hipFree(ld[i]));
-  }
 }
 
 void fatten(Links &L, Lattice *lat, const double *const gf[4], const double *const gl[4],
             const QOP_asqtad_coeffs_t *c, QOP_info_t *info) {
   double coef[6] = {c->one_link, c->three_staple, c->five_staple, c->seven_staple, c->lepage, c->naik};
   bool naik = c->naik != 0;
+  DevBuf<double> fbuf[4], lbuf[4];
   double *fat[4], *lng[4];
   size_t n = (size_t)lat->V * 18 * sizeof(double);
   for (int i = 0; i < 4; i++) {
-    QOP_CHECK_HIP(hipMalloc(&fat[i], n));
-    lng[i] = nullptr;
-    if (naik) QOP_CHECK_HIP(hipMalloc(&lng[i], n));
+    dev_alloc(&fbuf[i].p, n);
+    if (naik) dev_alloc(&lbuf[i].p, n);
+    fat[i] = fbuf[i].p;
+    lng[i] = lbuf[i].p;
   }
   const double *g[4] = {gf[0], gf[1], gf[2], gf[3]}, *gL[4] = {gl[0], gl[1], gl[2], gl[3]};
   double t0 = wall_time();
@@ -127,10 +124,6 @@ void fatten(Links &L, Lattice *lat, const double *const gf[4], const double *con
   L.lat = lat;
   const double *fc[4] = {fat[0], fat[1], fat[2], fat[3]}, *lc[4] = {lng[0], lng[1], lng[2], lng[3]};
   build_links64(L, fc, naik ? lc : nullptr, stream());
-  for (int i = 0; i < 4; i++) {
-    QOP_CHECK_HIP(hipFree(fat[i]));
-    if (lng[i]) QOP_CHECK_HIP(hipFree(lng[i]));
-  }
   bool have3 = c->three_staple != 0 || c->five_staple != 0 || c->seven_staple != 0 || c->lepage != 0;
   if (info) {  // asqtad_dslash_p.c:536, 624-626
     info->final_sec = dt;
@@ -285,24 +278,36 @@ QOP_status_t QOP_asqtad_invert_set_opts(QOP_opt_t opts[], int nopts) {
 QOP_D3_ColorVector *QOP_D3_create_V_from_raw(QOP_D_Real *src, QOP_evenodd_t eo) {
   CHECK_INIT;
   auto *v = new QOP_D3_ColorVector;
-  devvec_alloc(v->v, G.lat, P64);
-  upload_vec(v->v, src, eo);
+  if (!guard_oom(__func__, nullptr, [&] {
+        devvec_alloc(v->v, G.lat, P64);
+        upload_vec(v->v, src, eo);
+      })) {
+    devvec_free(v->v);
+    delete v;
+    return nullptr;
+  }
   return v;
 }
 QOP_F3_ColorVector *QOP_F3_create_V_from_raw(QOP_F_Real *src, QOP_evenodd_t eo) {
   CHECK_INIT;
   auto *v = new QOP_F3_ColorVector;
-  devvec_alloc(v->v, G.lat, P32);
-  upload_vec(v->v, src, eo);
+  if (!guard_oom(__func__, nullptr, [&] {
+        devvec_alloc(v->v, G.lat, P32);
+        upload_vec(v->v, src, eo);
+      })) {
+    devvec_free(v->v);
+    delete v;
+    return nullptr;
+  }
   return v;
 }
 void QOP_D3_extract_V_to_raw(QOP_D_Real *dest, QOP_D3_ColorVector *src, QOP_evenodd_t eo) {
   CHECK_INIT;
-  download_vec(dest, src->v, eo);
+  guard_oom(__func__, nullptr, [&] { download_vec(dest, src->v, eo); });
 }
 void QOP_F3_extract_V_to_raw(QOP_F_Real *dest, QOP_F3_ColorVector *src, QOP_evenodd_t eo) {
   CHECK_INIT;
-  download_vec(dest, src->v, eo);
+  guard_oom(__func__, nullptr, [&] { download_vec(dest, src->v, eo); });
 }
 void QOP_D3_destroy_V(QOP_D3_ColorVector *f) {
   if (!f) return;
@@ -318,12 +323,12 @@ void QOP_F3_destroy_V(QOP_F3_ColorVector *f) {
 }
 QOP_D3_ColorVector *QOP_D3_convert_V_from_raw(QOP_D_Real *src, QOP_evenodd_t eo) {
   QOP_D3_ColorVector *v = QOP_D3_create_V_from_raw(src, eo);
-  v->raw = src;
+  if (v) v->raw = src;
   return v;
 }
 QOP_F3_ColorVector *QOP_F3_convert_V_from_raw(QOP_F_Real *src, QOP_evenodd_t eo) {
   QOP_F3_ColorVector *v = QOP_F3_create_V_from_raw(src, eo);
-  v->raw = src;
+  if (v) v->raw = src;
   return v;
 }
 // common_p.c:225-240; the raw array is sized for all sites (the reference
@@ -331,16 +336,24 @@ QOP_F3_ColorVector *QOP_F3_convert_V_from_raw(QOP_F_Real *src, QOP_evenodd_t eo)
 QOP_D_Real *QOP_D3_convert_V_to_raw(QOP_D3_ColorVector *src, QOP_evenodd_t eo) {
   CHECK_INIT;
   double *r = src->raw;
-  if (!r) r = (double *)calloc((size_t)src->v.lat->V * 6, sizeof(double));
-  download_vec(r, src->v, eo);
+  const bool own = !r;
+  if (own) r = (double *)calloc((size_t)src->v.lat->V * 6, sizeof(double));
+  if (!guard_oom(__func__, nullptr, [&] { download_vec(r, src->v, eo); })) {
+    if (own) free(r);
+    return nullptr;  // src stays valid
+  }
   QOP_D3_destroy_V(src);
   return r;
 }
 QOP_F_Real *QOP_F3_convert_V_to_raw(QOP_F3_ColorVector *src, QOP_evenodd_t eo) {
   CHECK_INIT;
   float *r = src->raw;
-  if (!r) r = (float *)calloc((size_t)src->v.lat->V * 6, sizeof(float));
-  download_vec(r, src->v, eo);
+  const bool own = !r;
+  if (own) r = (float *)calloc((size_t)src->v.lat->V * 6, sizeof(float));
+  if (!guard_oom(__func__, nullptr, [&] { download_vec(r, src->v, eo); })) {
+    if (own) free(r);
+    return nullptr;  // src stays valid
+  }
   QOP_F3_destroy_V(src);
   return r;
 }
@@ -350,14 +363,24 @@ QOP_D3_GaugeField *QOP_D3_create_G_from_raw(QOP_D_Real *links[], QOP_evenodd_t e
   CHECK_INIT;
   auto *g = new QOP_D3_GaugeField;
   g->lat = G.lat;
-  for (int i = 0; i < 4; i++) g->U[i] = upload_mat(G.lat, links[i], eo);
+  if (!guard_oom(__func__, nullptr, [&] {
+        for (int i = 0; i < 4; i++) g->U[i] = upload_mat(G.lat, links[i], eo);
+      })) {
+    QOP_D3_destroy_G(g);
+    return nullptr;
+  }
   return g;
 }
 QOP_F3_GaugeField *QOP_F3_create_G_from_raw(QOP_F_Real *links[], QOP_evenodd_t eo) {
   CHECK_INIT;
   auto *g = new QOP_F3_GaugeField;
   g->lat = G.lat;
-  for (int i = 0; i < 4; i++) g->U[i] = upload_mat(G.lat, links[i], eo);
+  if (!guard_oom(__func__, nullptr, [&] {
+        for (int i = 0; i < 4; i++) g->U[i] = upload_mat(G.lat, links[i], eo);
+      })) {
+    QOP_F3_destroy_G(g);
+    return nullptr;
+  }
   return g;
 }
 void QOP_D3_extract_G_to_raw(QOP_D_Real *dest[], QOP_D3_GaugeField *src, QOP_evenodd_t eo) {
@@ -393,42 +416,60 @@ QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_raw(QOP_D_Real *fat[], QO
   CHECK_INIT;
   auto *f = new QOP_D3_FermionLinksAsqtad;
   f->L.lat = G.lat;
-  build_links_from_raw(f->L, fat, lng, eo);
+  if (!guard_oom(__func__, nullptr, [&] { build_links_from_raw(f->L, fat, lng, eo); })) {
+    QOP_D3_asqtad_destroy_L(f);
+    return nullptr;
+  }
   return f;
 }
 QOP_F3_FermionLinksAsqtad *QOP_F3_asqtad_create_L_from_raw(QOP_F_Real *fat[], QOP_F_Real *lng[], QOP_evenodd_t eo) {
   CHECK_INIT;
   auto *f = new QOP_F3_FermionLinksAsqtad;
   f->L.lat = G.lat;
-  build_links_from_raw(f->L, fat, lng, eo);
+  if (!guard_oom(__func__, nullptr, [&] { build_links_from_raw(f->L, fat, lng, eo); })) {
+    QOP_F3_asqtad_destroy_L(f);
+    return nullptr;
+  }
   return f;
 }
 QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_G(QOP_info_t *info, QOP_asqtad_coeffs_t *c, QOP_D3_GaugeField *g) {
   CHECK_INIT;
   auto *f = new QOP_D3_FermionLinksAsqtad;
-  fatten(f->L, g->lat, g->U, g->U, c, info);
+  if (!guard_oom(__func__, info, [&] { fatten(f->L, g->lat, g->U, g->U, c, info); })) {
+    QOP_D3_asqtad_destroy_L(f);
+    return nullptr;
+  }
   return f;
 }
 QOP_F3_FermionLinksAsqtad *QOP_F3_asqtad_create_L_from_G(QOP_info_t *info, QOP_asqtad_coeffs_t *c, QOP_F3_GaugeField *g) {
   CHECK_INIT;
   auto *f = new QOP_F3_FermionLinksAsqtad;
-  fatten(f->L, g->lat, g->U, g->U, c, info);
+  if (!guard_oom(__func__, info, [&] { fatten(f->L, g->lat, g->U, g->U, c, info); })) {
+    QOP_F3_asqtad_destroy_L(f);
+    return nullptr;
+  }
   return f;
 }
 QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_G2(QOP_info_t *info, QOP_asqtad_coeffs_t *c, QOP_D3_GaugeField *gf, QOP_D3_GaugeField *gl) {
   CHECK_INIT;
   auto *f = new QOP_D3_FermionLinksAsqtad;
-  fatten(f->L, gf->lat, gf->U, gl->U, c, info);
+  if (!guard_oom(__func__, info, [&] { fatten(f->L, gf->lat, gf->U, gl->U, c, info); })) {
+    QOP_D3_asqtad_destroy_L(f);
+    return nullptr;
+  }
   return f;
 }
 static void extract_links(Links &L, double *fat[], double *lng[], int eo, float *ffat[], float *flng[]) {
   Lattice *lat = L.lat;
-  double *fd[4], *ld[4] = {nullptr, nullptr, nullptr, nullptr};
+  DevBuf<double> fb[4], lb[4];
+  double *fd[4], *ld[4];
   size_t n = (size_t)lat->V * 18 * sizeof(double);
   bool want_long = (lng || flng) && L.nl == 16;
   for (int i = 0; i < 4; i++) {
-    QOP_CHECK_HIP(hipMalloc(&fd[i], n));
-    if (want_long) QOP_CHECK_HIP(hipMalloc(&ld[i], n));
+    dev_alloc(&fb[i].p, n);
+    if (want_long) dev_alloc(&lb[i].p, n);
+    fd[i] = fb[i].p;
+    ld[i] = lb[i].p;
   }
   extract_links64(L, fd, want_long ? ld : nullptr, stream());
   for (int i = 0; i < 4; i++) {
@@ -436,25 +477,23 @@ static void extract_links(Links &L, double *fat[], double *lng[], int eo, float 
     if (ffat) download_mat(ffat[i], fd[i], lat, eo);
     if (want_long && lng) download_mat(lng[i], ld[i], lat, eo);
     if (want_long && flng) download_mat(flng[i], ld[i], lat, eo);
-    QOP_CHECK_HIP(hipFree(fd[i]));
-    if (ld[i]) QOP_CHECK_HIP(hipFree(ld[i]));
   }
 }
 void QOP_D3_asqtad_extract_L_to_raw(QOP_D_Real *fat[], QOP_D_Real *lng[], QOP_D3_FermionLinksAsqtad *src, QOP_evenodd_t eo) {
   CHECK_INIT;
-  extract_links(src->L, fat, lng, eo, nullptr, nullptr);
+  guard_oom(__func__, nullptr, [&] { extract_links(src->L, fat, lng, eo, nullptr, nullptr); });
 }
 void QOP_F3_asqtad_extract_L_to_raw(QOP_F_Real *fat[], QOP_F_Real *lng[], QOP_F3_FermionLinksAsqtad *src, QOP_evenodd_t eo) {
   CHECK_INIT;
-  extract_links(src->L, nullptr, nullptr, eo, fat, lng);
+  guard_oom(__func__, nullptr, [&] { extract_links(src->L, nullptr, nullptr, eo, fat, lng); });
 }
 void QOP_D3_asqtad_load_L_from_raw(QOP_D3_FermionLinksAsqtad *f, QOP_D_Real *fat[], QOP_D_Real *lng[], QOP_evenodd_t eo) {
   CHECK_INIT;
-  build_links_from_raw(f->L, fat, lng, eo);
+  guard_oom(__func__, nullptr, [&] { build_links_from_raw(f->L, fat, lng, eo); });
 }
 void QOP_D3_asqtad_load_L_from_G(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *f, QOP_asqtad_coeffs_t *c, QOP_D3_GaugeField *g) {
   CHECK_INIT;
-  fatten(f->L, g->lat, g->U, g->U, c, info);
+  guard_oom(__func__, info, [&] { fatten(f->L, g->lat, g->U, g->U, c, info); });
 }
 static void destroy_links(Links &L) {
   sync();
@@ -476,20 +515,21 @@ void QOP_D3_asqtad_rephase_L(QOP_D3_FermionLinksAsqtad *f, int *r0, QOP_bc_t *bc
   CHECK_INIT;
   Links &L = f->L;
   Lattice *lat = L.lat;
-  double *fd[4], *ld[4] = {nullptr, nullptr, nullptr, nullptr};
-  size_t n = (size_t)lat->V * 18 * sizeof(double);
-  for (int i = 0; i < 4; i++) {
-    QOP_CHECK_HIP(hipMalloc(&fd[i], n));
-    if (L.nl == 16) QOP_CHECK_HIP(hipMalloc(&ld[i], n));
-  }
-  extract_links64(L, fd, L.nl == 16 ? ld : nullptr, stream());
-  rephase_dev(lat, fd, r0, bc, sign, 1, 3, L.nl == 16 ? ld : nullptr, 1);
-  const double *fc[4] = {fd[0], fd[1], fd[2], fd[3]}, *lc[4] = {ld[0], ld[1], ld[2], ld[3]};
-  build_links64(L, fc, L.nl == 16 ? lc : nullptr, stream());
-  for (int i = 0; i < 4; i++) {
-    QOP_CHECK_HIP(hipFree(fd[i]));
-    if (ld[i]) QOP_CHECK_HIP(hipFree(ld[i]));
-  }
+  guard_oom(__func__, nullptr, [&] {
+    DevBuf<double> fb[4], lb[4];
+    double *fd[4], *ld[4];
+    size_t n = (size_t)lat->V * 18 * sizeof(double);
+    for (int i = 0; i < 4; i++) {
+      dev_alloc(&fb[i].p, n);
+      if (L.nl == 16) dev_alloc(&lb[i].p, n);
+      fd[i] = fb[i].p;
+      ld[i] = lb[i].p;
+    }
+    extract_links64(L, fd, L.nl == 16 ? ld : nullptr, stream());
+    rephase_dev(lat, fd, r0, bc, sign, 1, 3, L.nl == 16 ? ld : nullptr, 1);
+    const double *fc[4] = {fd[0], fd[1], fd[2], fd[3]}, *lc[4] = {ld[0], ld[1], ld[2], ld[3]};
+    build_links64(L, fc, L.nl == 16 ? lc : nullptr, stream());
+  });
 }
 // asqtad_dslash_p.c:742-790: the single-precision copy of double links
 QOP_F3_FermionLinksAsqtad *QOP_FD3_asqtad_create_L_from_L(QOP_D3_FermionLinksAsqtad *src) {
@@ -498,13 +538,14 @@ QOP_F3_FermionLinksAsqtad *QOP_FD3_asqtad_create_L_from_L(QOP_D3_FermionLinksAsq
   f->L.lat = src->L.lat;
   f->L.nl = src->L.nl;
   // keep an FP64 master with the FP32-rounded values so both views agree
-  LinkSet tmp;
-  convert_links(src->L.lat, src->L.l64, tmp, P32, stream());
-  f->L.l64.prec = P64;
-  f->L.l64.nl = src->L.nl;
-  size_t n = (size_t)2 * src->L.nl * 9 * src->L.lat->Vh;
-  QOP_CHECK_HIP(hipMalloc(&f->L.l64.data, n * sizeof(double2)));
-  {
+  const bool ok = guard_oom(__func__, nullptr, [&] {
+    LinkSet tmp;
+    convert_links(src->L.lat, src->L.l64, tmp, P32, stream());
+    f->L.l32 = tmp;  // owned by f from here on
+    size_t n = (size_t)2 * src->L.nl * 9 * src->L.lat->Vh;
+    dev_alloc(&f->L.l64.data, n * sizeof(double2));
+    f->L.l64.prec = P64;
+    f->L.l64.nl = src->L.nl;
     // widen FP32 -> FP64 through the vector pack kernels' lincomb path:
     // treat the link array as a batch of n/(3 Vh) parity halves.
     LincombArgs a;
@@ -513,14 +554,17 @@ QOP_F3_FermionLinksAsqtad *QOP_FD3_asqtad_create_L_from_L(QOP_D3_FermionLinksAsq
     a.n = (int)(n / (3 * Vh));
     a.out.data = f->L.l64.data;
     a.ostride = 3 * Vh;
-    a.v[0].data = tmp.data;
+    a.v[0].data = f->L.l32.data;
     a.stride[0] = 3 * Vh;
     a.cr[0] = 1.0;
     int pin[1] = {P32};
     launch_lincomb(a, P64, pin, 1, false, stream());
+    sync();
+  });
+  if (!ok) {
+    QOP_F3_asqtad_destroy_L(f);
+    return nullptr;
   }
-  f->L.l32 = tmp;
-  sync();
   return f;
 }
 QOP_D3_FermionLinksAsqtad *QOP_DF3_asqtad_create_L_from_L(QOP_F3_FermionLinksAsqtad *src) {
@@ -528,12 +572,18 @@ QOP_D3_FermionLinksAsqtad *QOP_DF3_asqtad_create_L_from_L(QOP_F3_FermionLinksAsq
   auto *f = new QOP_D3_FermionLinksAsqtad;
   f->L.lat = src->L.lat;
   f->L.nl = src->L.nl;
-  f->L.l64.prec = P64;
-  f->L.l64.nl = src->L.nl;
-  size_t n = (size_t)2 * src->L.nl * 9 * src->L.lat->Vh * sizeof(double2);
-  QOP_CHECK_HIP(hipMalloc(&f->L.l64.data, n));
-  QOP_CHECK_HIP(hipMemcpyAsync(f->L.l64.data, src->L.l64.data, n, hipMemcpyDeviceToDevice, stream()));
-  sync();
+  const bool ok = guard_oom(__func__, nullptr, [&] {
+    size_t n = (size_t)2 * src->L.nl * 9 * src->L.lat->Vh * sizeof(double2);
+    dev_alloc(&f->L.l64.data, n);
+    f->L.l64.prec = P64;
+    f->L.l64.nl = src->L.nl;
+    QOP_CHECK_HIP(hipMemcpyAsync(f->L.l64.data, src->L.l64.data, n, hipMemcpyDeviceToDevice, stream()));
+    sync();
+  });
+  if (!ok) {
+    QOP_D3_asqtad_destroy_L(f);
+    return nullptr;
+  }
   return f;
 }
 
@@ -542,17 +592,19 @@ void QOP_D3_asqtad_dslash(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *f, QOP_D_
                           QOP_D3_ColorVector *out, QOP_D3_ColorVector *in, QOP_evenodd_t eo_out,
                           QOP_evenodd_t eo_in) {
   CHECK_INIT;
-  (void)info;
-  dslash(f->L, mass, out->v, in->v, eo_out, eo_in);
-  sync();
+  guard_oom(__func__, info, [&] {
+    dslash(f->L, mass, out->v, in->v, eo_out, eo_in);
+    sync();
+  });
 }
 void QOP_F3_asqtad_dslash(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *f, QOP_F_Real mass,
                           QOP_F3_ColorVector *out, QOP_F3_ColorVector *in, QOP_evenodd_t eo_out,
                           QOP_evenodd_t eo_in) {
   CHECK_INIT;
-  (void)info;
-  dslash(f->L, mass, out->v, in->v, eo_out, eo_in);
-  sync();
+  guard_oom(__func__, info, [&] {
+    dslash(f->L, mass, out->v, in->v, eo_out, eo_in);
+    sync();
+  });
 }
 void QOP_D3_asqtad_diaginv(QOP_info_t *, QOP_D3_FermionLinksAsqtad *, QOP_D_Real mass,
                            QOP_D3_ColorVector *out, QOP_D3_ColorVector *in, QOP_evenodd_t eo) {
@@ -599,7 +651,8 @@ void QOP_D3_asqtad_invert(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *f, QOP_in
   InvArg ia = to_ia(inv_arg);
   ResArg ra = to_ra(res_arg);
   double flop = 0, sec = 0;
-  invert_dispatch(f->L, &ia, &ra, mass, out->v, in->v, &flop, &sec);
+  if (!guard_oom(__func__, info, [&] { invert_dispatch(f->L, &ia, &ra, mass, out->v, in->v, &flop, &sec); }))
+    return;
   from_ra(res_arg, ra);
   if (info) { info->final_sec = sec; info->final_flop = flop; info->status = QOP_SUCCESS; }
 }
@@ -612,13 +665,16 @@ void QOP_F3_asqtad_invert(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *f, QOP_in
   CHECK_INIT;
   InvArg ia = to_ia(inv_arg);
   ResArg ra = to_ra(res_arg);
-  TmpVec o(out->v.lat, P64), i(in->v.lat, P64);
-  copy(o.v, out->v, 2);
-  copy(i.v, in->v, 2);
   double flop = 0, sec = 0;
-  invert_dispatch(f->L, &ia, &ra, mass, o.v, i.v, &flop, &sec);
-  copy(out->v, o.v, inv_arg->evenodd);
-  sync();
+  const bool ok = guard_oom(__func__, info, [&] {
+    TmpVec o(out->v.lat, P64), i(in->v.lat, P64);
+    copy(o.v, out->v, 2);
+    copy(i.v, in->v, 2);
+    invert_dispatch(f->L, &ia, &ra, mass, o.v, i.v, &flop, &sec);
+    copy(out->v, o.v, inv_arg->evenodd);
+    sync();
+  });
+  if (!ok) return;
   from_ra(res_arg, ra);
   if (info) { info->final_sec = sec; info->final_flop = flop; info->status = QOP_SUCCESS; }
 }
@@ -645,7 +701,11 @@ void QOP_D3_asqtad_invert_multi(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *f, 
     }
   }
   double flop = 0, sec = 0;
-  invert_multi_dispatch(f->L, &ia, rap.data(), ms.data(), nmass, out.data(), in.data(), nsrc, &flop, &sec);
+  if (!guard_oom(__func__, info, [&] {
+        invert_multi_dispatch(f->L, &ia, rap.data(), ms.data(), nmass, out.data(), in.data(), nsrc, &flop,
+                              &sec);
+      }))
+    return;
   k = 0;
   for (int i = 0; i < nsrc; i++)
     for (int j = 0; j < nmass[i]; j++, k++) from_ra(res_arg[i][j], ra[k]);
@@ -662,34 +722,35 @@ void QOP_F3_asqtad_invert_multi(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *f, 
   std::vector<ResArg> ra(nsolve);
   std::vector<ResArg *> rap(nsolve);
   std::vector<double> ms(nsolve);
-  std::vector<TmpVec *> ov(nsolve), iv(nsrc);
+  std::vector<std::unique_ptr<TmpVec>> ov(nsolve), iv(nsrc);
   std::vector<DevVec *> out(nsolve), in(nsrc);
-  int k = 0;
-  for (int i = 0; i < nsrc; i++) {
-    iv[i] = new TmpVec(in_pt[i]->v.lat, P64);
-    copy(iv[i]->v, in_pt[i]->v, 2);
-    in[i] = &iv[i]->v;
-    for (int j = 0; j < nmass[i]; j++, k++) {
-      ra[k] = to_ra(res_arg[i][j]);
-      rap[k] = &ra[k];
-      ms[k] = masses[i][j];
-      ov[k] = new TmpVec(out_pt[i][j]->v.lat, P64);
-      copy(ov[k]->v, out_pt[i][j]->v, 2);
-      out[k] = &ov[k]->v;
-    }
-  }
   double flop = 0, sec = 0;
-  invert_multi_dispatch(f->L, &ia, rap.data(), ms.data(), nmass, out.data(), in.data(), nsrc, &flop, &sec);
-  int cg = inv_arg->evenodd == QOP_EVENODD ? QOP_EVEN : inv_arg->evenodd;
-  k = 0;
-  for (int i = 0; i < nsrc; i++)
-    for (int j = 0; j < nmass[i]; j++, k++) {
-      copy(out_pt[i][j]->v, ov[k]->v, cg);
-      from_ra(res_arg[i][j], ra[k]);
+  const bool ok = guard_oom(__func__, info, [&] {
+    int k = 0;
+    for (int i = 0; i < nsrc; i++) {
+      iv[i].reset(new TmpVec(in_pt[i]->v.lat, P64));
+      copy(iv[i]->v, in_pt[i]->v, 2);
+      in[i] = &iv[i]->v;
+      for (int j = 0; j < nmass[i]; j++, k++) {
+        ra[k] = to_ra(res_arg[i][j]);
+        rap[k] = &ra[k];
+        ms[k] = masses[i][j];
+        ov[k].reset(new TmpVec(out_pt[i][j]->v.lat, P64));
+        copy(ov[k]->v, out_pt[i][j]->v, 2);
+        out[k] = &ov[k]->v;
+      }
     }
-  sync();
-  for (auto *p : ov) delete p;
-  for (auto *p : iv) delete p;
+    invert_multi_dispatch(f->L, &ia, rap.data(), ms.data(), nmass, out.data(), in.data(), nsrc, &flop, &sec);
+    int cg = inv_arg->evenodd == QOP_EVENODD ? QOP_EVEN : inv_arg->evenodd;
+    k = 0;
+    for (int i = 0; i < nsrc; i++)
+      for (int j = 0; j < nmass[i]; j++, k++) {
+        copy(out_pt[i][j]->v, ov[k]->v, cg);
+        from_ra(res_arg[i][j], ra[k]);
+      }
+    sync();
+  });
+  if (!ok) return;
   if (info) { info->final_sec = sec; info->final_flop = flop; info->status = QOP_SUCCESS; }
 }
 

@@ -75,7 +75,7 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
   double *scratch = nullptr;
   if (red != RED_NONE) {
     const size_t np = (size_t)nrhs * ((lat->Vh + 63) / 64) + nrhs;
-    QOP_CHECK_HIP(hipMalloc(&scratch, np * sizeof(double)));
+    dev_alloc(&scratch, np * sizeof(double));
     QOP_CHECK_HIP(hipMemsetAsync(scratch, 0, np * sizeof(double), st));
     a.partial = scratch + nrhs;
   }
@@ -169,63 +169,70 @@ BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double
                          hipStream_t st)
     : L_(L), lat_(L.lat), eo_(eo), cap_(capacity), sp_(sloppy_prec), delta_(delta),
       st_(st ? st : stream()) {
-  QOP_REQUIRE(eo == 0 || eo == 1, "batch solver: subset must be even or odd");
-  QOP_REQUIRE(capacity >= 1 && capacity <= 256, "batch solver: capacity 1..256");
-  QOP_REQUIRE(sp_ == P32 || sp_ == P16, "batch solver: sloppy precision must be FP32 or FP16");
-  nch_ = (int)((lat_->Vh + 63) / 64);
-  devbatch_alloc(x64_, lat_, P64, cap_);
-  devbatch_alloc(r64_, lat_, P64, cap_);
-  devbatch_alloc(b64_, lat_, P64, cap_);
-  devbatch_alloc(ax64_, lat_, P64, cap_);
-  devbatch_alloc(t64_, lat_, P64, cap_);
-  devbatch_alloc(rs_, lat_, sp_, cap_);
-  devbatch_alloc(ps_, lat_, sp_, cap_);
-  devbatch_alloc(ts_, lat_, sp_, cap_);
-  devbatch_alloc(xs_, lat_, P32, cap_);
-  QOP_CHECK_HIP(hipMalloc(&rhs_, cap_ * sizeof(BsRhs)));
-  QOP_CHECK_HIP(hipHostMalloc(&rhs_h_, cap_ * sizeof(BsRhs), hipHostMallocDefault));
-  QOP_CHECK_HIP(hipMalloc(&active_, 2 * cap_ * sizeof(int)));
-  sel_ = active_ + cap_;
-  QOP_CHECK_HIP(hipHostMalloc(&active_h_, 2 * cap_ * sizeof(int), hipHostMallocDefault));
-  sel_h_ = active_h_ + cap_;
-  QOP_CHECK_HIP(hipMalloc(&ru_, 2 * cap_ * sizeof(int)));
-  ru_cont_ = ru_ + cap_;
-  QOP_CHECK_HIP(hipMemsetAsync(ru_, 0, 2 * cap_ * sizeof(int), st_));
-  QOP_CHECK_HIP(hipMalloc(&glob_, sizeof(BsGlobal)));
-  QOP_CHECK_HIP(hipHostMalloc(&glob_h_, sizeof(BsGlobal), hipHostMallocDefault));
-  QOP_CHECK_HIP(hipMalloc(&partA_, 3 * (size_t)cap_ * nch_ * sizeof(double)));
-  partB_ = partA_ + (size_t)cap_ * nch_;
-  partC_ = partB_ + (size_t)cap_ * nch_;
-  QOP_CHECK_HIP(hipMalloc(&coef_, (2 + 6) * (size_t)cap_ * sizeof(double)));
-  lcoef_ = coef_ + 2 * cap_;
-  QOP_CHECK_HIP(hipMalloc(&sums_, 2 * (size_t)cap_ * sizeof(double)));
-  QOP_CHECK_HIP(hipHostMalloc(&sums_h_, 2 * (size_t)cap_ * sizeof(double), hipHostMallocDefault));
-  memset(rhs_h_, 0, cap_ * sizeof(BsRhs));
-  memset(active_h_, 0, 2 * cap_ * sizeof(int));
-  glob_h_->status = 0;
-  glob_h_->pending = 0;  // the status word stays 0: slots never pause the batch
-  QOP_CHECK_HIP(hipMemcpyAsync(rhs_, rhs_h_, cap_ * sizeof(BsRhs), hipMemcpyHostToDevice, st_));
-  QOP_CHECK_HIP(hipMemcpyAsync(active_, active_h_, 2 * cap_ * sizeof(int), hipMemcpyHostToDevice, st_));
-  QOP_CHECK_HIP(hipMemcpyAsync(glob_, glob_h_, sizeof(BsGlobal), hipMemcpyHostToDevice, st_));
-  loaded_.assign(cap_, 0);
-  mass_.assign(cap_, 0.0);
-  // Build the sloppy and FP64 link sets now: the iteration is captured into
-  // a graph later, and a conversion (allocation, host sync) cannot run then.
-  L_.get(sp_);
-  L_.get(P64);
-  // the two hops of the iteration are tuned with their own epilogues (the
-  // second on rs_ / xs_ with alpha 0, while no slot is loaded)
-  shape_ = tune_hop(L_, sp_, cap_, ps_, ts_, 1 - eo_, RED_OUT2);
-  shape2_ = tune_hop(L_, sp_, cap_, ts_, rs_, eo_, RED_CG, &ps_, &xs_);
-  ru_shape_ = tune_hop(L_, P64, cap_, x64_, t64_, 1 - eo_);
+  try {
+    QOP_REQUIRE(eo == 0 || eo == 1, "batch solver: subset must be even or odd");
+    QOP_REQUIRE(capacity >= 1 && capacity <= 256, "batch solver: capacity 1..256");
+    QOP_REQUIRE(sp_ == P32 || sp_ == P16, "batch solver: sloppy precision must be FP32 or FP16");
+    nch_ = (int)((lat_->Vh + 63) / 64);
+    devbatch_alloc(x64_, lat_, P64, cap_);
+    devbatch_alloc(r64_, lat_, P64, cap_);
+    devbatch_alloc(b64_, lat_, P64, cap_);
+    devbatch_alloc(ax64_, lat_, P64, cap_);
+    devbatch_alloc(t64_, lat_, P64, cap_);
+    devbatch_alloc(rs_, lat_, sp_, cap_);
+    devbatch_alloc(ps_, lat_, sp_, cap_);
+    devbatch_alloc(ts_, lat_, sp_, cap_);
+    devbatch_alloc(xs_, lat_, P32, cap_);
+    dev_alloc(&rhs_, cap_ * sizeof(BsRhs));
+    QOP_CHECK_HIP(hipHostMalloc(&rhs_h_, cap_ * sizeof(BsRhs), hipHostMallocDefault));
+    dev_alloc(&active_, 2 * cap_ * sizeof(int));
+    sel_ = active_ + cap_;
+    QOP_CHECK_HIP(hipHostMalloc(&active_h_, 2 * cap_ * sizeof(int), hipHostMallocDefault));
+    sel_h_ = active_h_ + cap_;
+    dev_alloc(&ru_, 2 * cap_ * sizeof(int));
+    ru_cont_ = ru_ + cap_;
+    QOP_CHECK_HIP(hipMemsetAsync(ru_, 0, 2 * cap_ * sizeof(int), st_));
+    dev_alloc(&glob_, sizeof(BsGlobal));
+    QOP_CHECK_HIP(hipHostMalloc(&glob_h_, sizeof(BsGlobal), hipHostMallocDefault));
+    dev_alloc(&partA_, 3 * (size_t)cap_ * nch_ * sizeof(double));
+    partB_ = partA_ + (size_t)cap_ * nch_;
+    partC_ = partB_ + (size_t)cap_ * nch_;
+    dev_alloc(&coef_, (2 + 6) * (size_t)cap_ * sizeof(double));
+    lcoef_ = coef_ + 2 * cap_;
+    dev_alloc(&sums_, 2 * (size_t)cap_ * sizeof(double));
+    QOP_CHECK_HIP(hipHostMalloc(&sums_h_, 2 * (size_t)cap_ * sizeof(double), hipHostMallocDefault));
+    memset(rhs_h_, 0, cap_ * sizeof(BsRhs));
+    memset(active_h_, 0, 2 * cap_ * sizeof(int));
+    glob_h_->status = 0;
+    glob_h_->pending = 0;  // the status word stays 0: slots never pause the batch
+    QOP_CHECK_HIP(hipMemcpyAsync(rhs_, rhs_h_, cap_ * sizeof(BsRhs), hipMemcpyHostToDevice, st_));
+    QOP_CHECK_HIP(hipMemcpyAsync(active_, active_h_, 2 * cap_ * sizeof(int), hipMemcpyHostToDevice, st_));
+    QOP_CHECK_HIP(hipMemcpyAsync(glob_, glob_h_, sizeof(BsGlobal), hipMemcpyHostToDevice, st_));
+    loaded_.assign(cap_, 0);
+    mass_.assign(cap_, 0.0);
+    // Build the sloppy and FP64 link sets now: the iteration is captured into
+    // a graph later, and a conversion (allocation, host sync) cannot run then.
+    L_.get(sp_);
+    L_.get(P64);
+    // the two hops of the iteration are tuned with their own epilogues (the
+    // second on rs_ / xs_ with alpha 0, while no slot is loaded)
+    shape_ = tune_hop(L_, sp_, cap_, ps_, ts_, 1 - eo_, RED_OUT2);
+    shape2_ = tune_hop(L_, sp_, cap_, ts_, rs_, eo_, RED_CG, &ps_, &xs_);
+    ru_shape_ = tune_hop(L_, P64, cap_, x64_, t64_, 1 - eo_);
 
-  L_.get(P64);
-  // allocations are cleared and links converted on the library stream
-  sync();
-  QOP_CHECK_HIP(hipStreamSynchronize(st_));
+    L_.get(P64);
+    // allocations are cleared and links converted on the library stream
+    sync();
+    QOP_CHECK_HIP(hipStreamSynchronize(st_));
+  } catch (const DeviceOutOfMemory &) {
+    release();  // a failed allocation frees what was built
+    throw;
+  }
 }
 
-BatchSolver::~BatchSolver() {
+BatchSolver::~BatchSolver() { release(); }
+
+void BatchSolver::release() {
   QOP_CHECK_HIP(hipStreamSynchronize(st_));
   if (ev_) (void)hipEventDestroy(ev_);
   if (graph_) (void)hipGraphExecDestroy(graph_);

@@ -87,6 +87,11 @@ class BatchSolver {
   BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double delta,
               hipStream_t st = nullptr);
   ~BatchSolver();
+
+ private:
+  void release();  // frees everything allocated so far
+
+ public:
   BatchSolver(const BatchSolver &) = delete;
   BatchSolver &operator=(const BatchSolver &) = delete;
 

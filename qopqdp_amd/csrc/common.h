@@ -40,6 +40,36 @@ inline size_t prec_cbytes(int p) { return p == P64 ? 16 : p == P32 ? 8 : 4; }
     if (!(c)) ::qop::fatal(__FILE__, __LINE__, msg);                       \
   } while (0)
 
+// Device memory exhausted (hipErrorOutOfMemory).  Thrown by dev_alloc; the
+// API entry points catch it, release what they allocated and report
+// QOP_MEM_ERROR (or return null) instead of exiting.
+struct DeviceOutOfMemory {
+  size_t bytes;
+};
+void dev_alloc_bytes(void **p, size_t bytes);
+template <class T>
+inline void dev_alloc(T **p, size_t bytes) {
+  dev_alloc_bytes((void **)p, bytes);
+}
+// Owning device buffer for temporaries: freed when it goes out of scope,
+// also while a DeviceOutOfMemory unwinds.
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  DevBuf() = default;
+  explicit DevBuf(size_t bytes) { dev_alloc(&p, bytes); }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  T *release() {
+    T *q = p;
+    p = nullptr;
+    return q;
+  }
+};
+
 // Lattice geometry passed by value to kernels.
 struct Geom {
   int L[4];

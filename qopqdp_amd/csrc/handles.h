@@ -1,5 +1,7 @@
 // Definitions of the opaque QOP handle types of include/qop_mi355x.h.
 #pragma once
+#include <cstdio>
+
 #include "../../include/qop_mi355x.h"
 #include "common.h"
 
@@ -37,4 +39,23 @@ namespace qop {
 void links_from_raw64(Links &L, double *fat[4], double *lng[4]);
 void links_from_gauge64(Links &L, double *const U_host[4], const QOP_asqtad_coeffs_t *c, int *r0,
                         QOP_bc_t *bc, QOP_staggered_sign_t *sign);
+
+// Runs body(); a device allocation failure inside it is reported on stderr
+// and, when info is given, as info->status = QOP_MEM_ERROR (the QOP status
+// for it, include/qop_int.h:19) instead of ending the process.  Objects built
+// inside release their memory while the failure unwinds.  Returns false on
+// such a failure.
+template <class F>
+bool guard_oom(const char *where, QOP_info_t *info, F &&body) {
+  try {
+    body();
+    return true;
+  } catch (const DeviceOutOfMemory &e) {
+    fprintf(stderr, "QOP-MI355X: %s: device out of memory (an allocation of %zu bytes failed)\n",
+            where, e.bytes);
+    fflush(stderr);
+    if (info) info->status = QOP_MEM_ERROR;
+    return false;
+  }
+}
 }  // namespace qop

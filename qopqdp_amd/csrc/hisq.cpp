@@ -43,11 +43,15 @@ double now() {
   return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
-// raw FP64 device matrices [4][V][18]
+// raw FP64 device matrices [4][V][18], freed on scope exit
 struct Raw4 {
   double *m[4] = {nullptr, nullptr, nullptr, nullptr};
+  Raw4() = default;
+  Raw4(const Raw4 &) = delete;
+  Raw4 &operator=(const Raw4 &) = delete;
+  ~Raw4() { free(); }
   void alloc(const Lattice *lat) {
-    for (int i = 0; i < 4; i++) QOP_CHECK_HIP(hipMalloc(&m[i], (size_t)lat->V * 18 * sizeof(double)));
+    for (int i = 0; i < 4; i++) dev_alloc(&m[i], (size_t)lat->V * 18 * sizeof(double));
   }
   void free() {
     for (int i = 0; i < 4; i++)
@@ -78,7 +82,13 @@ LinksT *links_from_raw_dev(Lattice *lat, const Raw4 &fat, const Raw4 *lng) {
   const double *lp[4] = {nullptr, nullptr, nullptr, nullptr};
   if (lng)
     for (int i = 0; i < 4; i++) lp[i] = lng->m[i];
-  build_links64(f->L, fp, lng ? lp : nullptr, stream());
+  try {
+    build_links64(f->L, fp, lng ? lp : nullptr, stream());
+  } catch (const DeviceOutOfMemory &) {
+    f->L.free_all();
+    delete f;
+    throw;
+  }
   sync();
   return f;
 }
@@ -129,7 +139,7 @@ HisqT *hisq_create(QOP_info_t *info, QOP_hisq_coeffs_t *c, GaugeT *gauge) {
     u.rel_error = g_hisq.reunit_svd_rel_error;
     u.abs_error = g_hisq.reunit_svd_abs_error;
     const long nw = u3_nwaves(lat);
-    QOP_CHECK_HIP(hipMalloc(&u.counts, 2 * nw * sizeof(int)));
+    dev_alloc(&u.counts, 2 * nw * sizeof(int));
     launch_u3_project(u, st);
     std::vector<int> cnt(2 * nw);
     QOP_CHECK_HIP(hipMemcpyAsync(cnt.data(), u.counts, 2 * nw * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -145,6 +155,18 @@ HisqT *hisq_create(QOP_info_t *info, QOP_hisq_coeffs_t *c, GaugeT *gauge) {
     flops += 795.0 * 4 * lat->V + 202.0 * nsvd;
   }
   auto *h = new HisqT;
+  // frees h and the link sets built so far if a device allocation fails
+  struct Owner {
+    HisqT *h;
+    ~Owner() {
+      if (!h) return;
+      sync();
+      for (auto *f : h->fn)
+        if (f) { f->L.free_all(); delete f; }
+      if (h->fn_deps) { h->fn_deps->L.free_all(); delete h->fn_deps; }
+      delete h;
+    }
+  } owner{h};
   h->n_naiks = n_naiks;
   h->fn.assign(n_naiks, nullptr);
   const double *w[4] = {W.m[0], W.m[1], W.m[2], W.m[3]};
@@ -200,6 +222,7 @@ HisqT *hisq_create(QOP_info_t *info, QOP_hisq_coeffs_t *c, GaugeT *gauge) {
     info->status = QOP_SUCCESS;
     info->count1 += nsvd;  // QOP_info_hisq_svd_counter
   }
+  owner.h = nullptr;
   return h;
 }
 
@@ -235,11 +258,19 @@ QOP_status_t QOP_hisq_links_set_opts(QOP_opt_t opts[], int nopts) {
 
 QOP_D3_FermionLinksHisq *QOP_D3_hisq_create_L_from_G(QOP_info_t *info, QOP_hisq_coeffs_t *coeffs,
                                                      QOP_D3_GaugeField *gauge) {
-  return hisq_create<QOP_D3_FermionLinksHisq, QOP_D3_FermionLinksAsqtad>(info, coeffs, gauge);
+  QOP_D3_FermionLinksHisq *h = nullptr;
+  guard_oom(__func__, info, [&] {
+    h = hisq_create<QOP_D3_FermionLinksHisq, QOP_D3_FermionLinksAsqtad>(info, coeffs, gauge);
+  });
+  return h;
 }
 QOP_F3_FermionLinksHisq *QOP_F3_hisq_create_L_from_G(QOP_info_t *info, QOP_hisq_coeffs_t *coeffs,
                                                      QOP_F3_GaugeField *gauge) {
-  return hisq_create<QOP_F3_FermionLinksHisq, QOP_F3_FermionLinksAsqtad>(info, coeffs, gauge);
+  QOP_F3_FermionLinksHisq *h = nullptr;
+  guard_oom(__func__, info, [&] {
+    h = hisq_create<QOP_F3_FermionLinksHisq, QOP_F3_FermionLinksAsqtad>(info, coeffs, gauge);
+  });
+  return h;
 }
 void QOP_D3_hisq_destroy_L(QOP_D3_FermionLinksHisq *h) { hisq_destroy(h, QOP_D3_asqtad_destroy_L); }
 void QOP_F3_hisq_destroy_L(QOP_F3_FermionLinksHisq *h) { hisq_destroy(h, QOP_F3_asqtad_destroy_L); }

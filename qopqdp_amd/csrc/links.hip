@@ -77,11 +77,12 @@ void build_links64(Links &L, const double *fat_dev[4], const double *lng_dev[4],
   L.nl = lng_dev ? 16 : 8;
   LinkSet &s = L.l64;
   linkset_free(s);
+  L.drop_derived();
+  DevBuf<const double *> pbuf(8 * sizeof(double *));
+  const double **ptrs = pbuf.p;
+  dev_alloc(&s.data, (size_t)2 * L.nl * 9 * lat->Vh * sizeof(double2));
   s.prec = P64;
   s.nl = L.nl;
-  QOP_CHECK_HIP(hipMalloc(&s.data, (size_t)2 * s.nl * 9 * lat->Vh * sizeof(double2)));
-  const double **ptrs;
-  QOP_CHECK_HIP(hipMalloc(&ptrs, 8 * sizeof(double *)));
   const double *hp[8];
   for (int i = 0; i < 4; i++) { hp[i] = fat_dev[i]; hp[4 + i] = lng_dev ? lng_dev[i] : nullptr; }
   QOP_CHECK_HIP(hipMemcpyAsync(ptrs, hp, sizeof hp, hipMemcpyHostToDevice, st));
@@ -89,8 +90,6 @@ void build_links64(Links &L, const double *fat_dev[4], const double *lng_dev[4],
   hipLaunchKernelGGL(dblstore_kernel, g, b, 0, st, lat->g, ptrs, ptrs + 4, s.nl, (double2 *)s.data);
   QOP_CHECK_HIP(hipGetLastError());
   QOP_CHECK_HIP(hipStreamSynchronize(st));
-  QOP_CHECK_HIP(hipFree(ptrs));
-  L.drop_derived();
 }
 
 // ---------------------------------------------------------------------------
@@ -131,16 +130,22 @@ __global__ void to16_kernel(const double2 *in, __half2 *out, float *scale,
 void convert_links(const Lattice *lat, const LinkSet &src, LinkSet &dst, int prec,
                    hipStream_t st) {
   QOP_REQUIRE(src.prec == P64, "convert_links: source must be FP64");
+  QOP_REQUIRE(prec == P32 || prec == P16, "convert_links: bad precision");
   linkset_free(dst);
+  long nlinks = 2L * src.nl * lat->Vh;
+  // allocate both parts before the set is marked built, so that a failed
+  // allocation leaves it empty
+  DevBuf<void> data((size_t)nlinks * 9 * prec_cbytes(prec));
+  DevBuf<float> scale;
+  if (prec == P16) dev_alloc(&scale.p, (size_t)nlinks * sizeof(float));
+  dst.data = data.release();
+  dst.scale = scale.release();
   dst.prec = prec;
   dst.nl = src.nl;
-  long nlinks = 2L * src.nl * lat->Vh;
-  QOP_CHECK_HIP(hipMalloc(&dst.data, (size_t)nlinks * 9 * prec_cbytes(prec)));
   dim3 g((nlinks + 255) / 256), b(256);
   if (prec == P32) {
     hipLaunchKernelGGL(to32_kernel, g, b, 0, st, (const double2 *)src.data, (float2 *)dst.data, nlinks, lat->Vh);
   } else if (prec == P16) {
-    QOP_CHECK_HIP(hipMalloc(&dst.scale, (size_t)nlinks * sizeof(float)));
     hipLaunchKernelGGL(to16_kernel, g, b, 0, st, (const double2 *)src.data, (__half2 *)dst.data, dst.scale, nlinks, lat->Vh);
   } else {
     fatal(__FILE__, __LINE__, "convert_links: bad precision");
@@ -175,8 +180,8 @@ __global__ void extract_kernel(Geom g, const double2 *links, int nl, double *con
 
 void extract_links64(const Links &L, double *fat_dev[4], double *lng_dev[4], hipStream_t st) {
   const Lattice *lat = L.lat;
-  double **ptrs;
-  QOP_CHECK_HIP(hipMalloc(&ptrs, 8 * sizeof(double *)));
+  DevBuf<double *> pbuf(8 * sizeof(double *));
+  double **ptrs = pbuf.p;
   double *hp[8];
   for (int i = 0; i < 4; i++) { hp[i] = fat_dev[i]; hp[4 + i] = lng_dev ? lng_dev[i] : nullptr; }
   QOP_CHECK_HIP(hipMemcpyAsync(ptrs, hp, sizeof hp, hipMemcpyHostToDevice, st));
@@ -184,7 +189,6 @@ void extract_links64(const Links &L, double *fat_dev[4], double *lng_dev[4], hip
   hipLaunchKernelGGL(extract_kernel, g, b, 0, st, lat->g, (const double2 *)L.l64.data, L.nl, ptrs, ptrs + 4);
   QOP_CHECK_HIP(hipGetLastError());
   QOP_CHECK_HIP(hipStreamSynchronize(st));
-  QOP_CHECK_HIP(hipFree(ptrs));
 }
 
 // ---------------------------------------------------------------------------
@@ -330,11 +334,12 @@ void launch_fatten(const Lattice *lat, const double *gf[4], const double *gl[4],
   coef1 -= 6 * coefL;  // lib/asqtad_dslash_p.c:531
   bool have5 = coef5 != 0 || coef7 != 0 || coefL != 0;
   bool have3 = coef3 != 0 || have5;
-  double *staple = nullptr, *temp = nullptr;
+  DevBuf<double> sbuf, tbuf;
   if (have3) {
-    QOP_CHECK_HIP(hipMalloc(&staple, V * 18 * sizeof(double)));
-    QOP_CHECK_HIP(hipMalloc(&temp, V * 18 * sizeof(double)));
+    dev_alloc(&sbuf.p, V * 18 * sizeof(double));
+    dev_alloc(&tbuf.p, V * 18 * sizeof(double));
   }
+  double *staple = sbuf.p, *temp = tbuf.p;
   for (int dir = 0; dir < 4; dir++) {
     dim3 gs((18 * V + 255) / 256);
     hipLaunchKernelGGL(scale_kernel, gs, b, 0, st, fat[dir], gf[dir], coef1, 18 * V);
@@ -361,8 +366,6 @@ void launch_fatten(const Lattice *lat, const double *gf[4], const double *gl[4],
       hipLaunchKernelGGL(naik_kernel, g, b, 0, st, lat->g, dir, gl[dir], c[5], lng[dir]);
   QOP_CHECK_HIP(hipGetLastError());
   QOP_CHECK_HIP(hipStreamSynchronize(st));
-  if (staple) QOP_CHECK_HIP(hipFree(staple));
-  if (temp) QOP_CHECK_HIP(hipFree(temp));
 }
 
 }  // namespace qop

@@ -14,53 +14,60 @@ namespace qop {
 MultiShiftBatch::MultiShiftBatch(Links &L, int eo, int capacity, int sloppy_prec,
                                  const double *masses, int ns)
     : L_(L), lat_(L.lat), eo_(eo), cap_(capacity), sp_(sloppy_prec), ns_(ns) {
-  QOP_REQUIRE(eo == 0 || eo == 1, "multi-shift: subset must be even or odd");
-  QOP_REQUIRE(ns >= 1 && ns <= MS_MAXSHIFT, "multi-shift: 1..16 masses");
-  QOP_REQUIRE(capacity >= 1 && capacity <= 64, "multi-shift: capacity 1..64");
-  QOP_REQUIRE(sp_ == P32 || sp_ == P16, "multi-shift: sloppy precision must be FP32 or FP16");
-  masses_.assign(masses, masses + ns);
-  imin_ = 0;
-  for (int i = 1; i < ns; i++)
-    if (masses[i] * masses[i] < masses[imin_] * masses[imin_]) imin_ = i;
-  nch_ = (int)((lat_->Vh + 63) / 64);
-  devbatch_alloc(b64_, lat_, P64, cap_);
-  devbatch_alloc(rs_, lat_, sp_, cap_);
-  devbatch_alloc(ps_, lat_, sp_, cap_);
-  devbatch_alloc(ts_, lat_, sp_, cap_);
-  devbatch_alloc(pm_, lat_, sp_, cap_ * ns_);
-  devbatch_alloc(xm_, lat_, P32, cap_ * ns_);
-  devbatch_alloc(x64_, lat_, P64, cap_ * ns_);
-  devbatch_alloc(t64_, lat_, P64, cap_);
-  devbatch_alloc(ax64_, lat_, P64, cap_);
-  QOP_CHECK_HIP(hipMalloc(&ru_, cap_ * sizeof(int)));
-  QOP_CHECK_HIP(hipMemset(ru_, 0, cap_ * sizeof(int)));
-  QOP_CHECK_HIP(hipMalloc(&slot_, cap_ * sizeof(MsSlot)));
-  QOP_CHECK_HIP(hipHostMalloc(&slot_h_, cap_ * sizeof(MsSlot), hipHostMallocDefault));
-  QOP_CHECK_HIP(hipMalloc(&active_, 2 * cap_ * sizeof(int)));
-  sel_ = active_ + cap_;
-  QOP_CHECK_HIP(hipHostMalloc(&active_h_, cap_ * sizeof(int), hipHostMallocDefault));
-  QOP_CHECK_HIP(hipMalloc(&partA_, 3 * (size_t)cap_ * nch_ * sizeof(double)));
-  partB_ = partA_ + (size_t)cap_ * nch_;
-  partC_ = partB_ + (size_t)cap_ * nch_;
-  QOP_CHECK_HIP(hipMalloc(&coef_, 2 * (size_t)cap_ * sizeof(double)));
-  QOP_CHECK_HIP(hipMalloc(&sums_, 2 * (size_t)cap_ * sizeof(double)));
-  QOP_CHECK_HIP(hipMalloc(&zero_status_, sizeof(int)));
-  QOP_CHECK_HIP(hipMemset(zero_status_, 0, sizeof(int)));
-  memset(slot_h_, 0, cap_ * sizeof(MsSlot));
-  memset(active_h_, 0, cap_ * sizeof(int));
-  QOP_CHECK_HIP(hipMemset(active_, 0, 2 * cap_ * sizeof(int)));
-  const double m0 = masses[imin_];
-  std::vector<double> hc(2 * cap_);
-  for (int b = 0; b < cap_; b++) { hc[2 * b] = -1.0; hc[2 * b + 1] = m0 * m0; }
-  QOP_CHECK_HIP(hipMemcpy(coef_, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice));
-  L_.get(sp_);  // convert the links before any graph capture
-  L_.get(P64);
-  shape_ = tune_hop(L_, sp_, cap_, ps_, ts_, 1 - eo_);
-  ru_shape_ = tune_hop(L_, P64, cap_, x64_, t64_, 1 - eo_);
-  sync();
+  try {
+    QOP_REQUIRE(eo == 0 || eo == 1, "multi-shift: subset must be even or odd");
+    QOP_REQUIRE(ns >= 1 && ns <= MS_MAXSHIFT, "multi-shift: 1..16 masses");
+    QOP_REQUIRE(capacity >= 1 && capacity <= 64, "multi-shift: capacity 1..64");
+    QOP_REQUIRE(sp_ == P32 || sp_ == P16, "multi-shift: sloppy precision must be FP32 or FP16");
+    masses_.assign(masses, masses + ns);
+    imin_ = 0;
+    for (int i = 1; i < ns; i++)
+      if (masses[i] * masses[i] < masses[imin_] * masses[imin_]) imin_ = i;
+    nch_ = (int)((lat_->Vh + 63) / 64);
+    devbatch_alloc(b64_, lat_, P64, cap_);
+    devbatch_alloc(rs_, lat_, sp_, cap_);
+    devbatch_alloc(ps_, lat_, sp_, cap_);
+    devbatch_alloc(ts_, lat_, sp_, cap_);
+    devbatch_alloc(pm_, lat_, sp_, cap_ * ns_);
+    devbatch_alloc(xm_, lat_, P32, cap_ * ns_);
+    devbatch_alloc(x64_, lat_, P64, cap_ * ns_);
+    devbatch_alloc(t64_, lat_, P64, cap_);
+    devbatch_alloc(ax64_, lat_, P64, cap_);
+    dev_alloc(&ru_, cap_ * sizeof(int));
+    QOP_CHECK_HIP(hipMemset(ru_, 0, cap_ * sizeof(int)));
+    dev_alloc(&slot_, cap_ * sizeof(MsSlot));
+    QOP_CHECK_HIP(hipHostMalloc(&slot_h_, cap_ * sizeof(MsSlot), hipHostMallocDefault));
+    dev_alloc(&active_, 2 * cap_ * sizeof(int));
+    sel_ = active_ + cap_;
+    QOP_CHECK_HIP(hipHostMalloc(&active_h_, cap_ * sizeof(int), hipHostMallocDefault));
+    dev_alloc(&partA_, 3 * (size_t)cap_ * nch_ * sizeof(double));
+    partB_ = partA_ + (size_t)cap_ * nch_;
+    partC_ = partB_ + (size_t)cap_ * nch_;
+    dev_alloc(&coef_, 2 * (size_t)cap_ * sizeof(double));
+    dev_alloc(&sums_, 2 * (size_t)cap_ * sizeof(double));
+    dev_alloc(&zero_status_, sizeof(int));
+    QOP_CHECK_HIP(hipMemset(zero_status_, 0, sizeof(int)));
+    memset(slot_h_, 0, cap_ * sizeof(MsSlot));
+    memset(active_h_, 0, cap_ * sizeof(int));
+    QOP_CHECK_HIP(hipMemset(active_, 0, 2 * cap_ * sizeof(int)));
+    const double m0 = masses[imin_];
+    std::vector<double> hc(2 * cap_);
+    for (int b = 0; b < cap_; b++) { hc[2 * b] = -1.0; hc[2 * b + 1] = m0 * m0; }
+    QOP_CHECK_HIP(hipMemcpy(coef_, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice));
+    L_.get(sp_);  // convert the links before any graph capture
+    L_.get(P64);
+    shape_ = tune_hop(L_, sp_, cap_, ps_, ts_, 1 - eo_);
+    ru_shape_ = tune_hop(L_, P64, cap_, x64_, t64_, 1 - eo_);
+    sync();
+  } catch (const DeviceOutOfMemory &) {
+    release();  // a failed allocation frees what was built
+    throw;
+  }
 }
 
-MultiShiftBatch::~MultiShiftBatch() {
+MultiShiftBatch::~MultiShiftBatch() { release(); }
+
+void MultiShiftBatch::release() {
   sync();
   if (graph_) (void)hipGraphExecDestroy(graph_);
   for (DevBatch *b : {&b64_, &rs_, &ps_, &ts_, &pm_, &xm_, &x64_, &t64_, &ax64_}) devbatch_free(*b);
@@ -296,58 +303,69 @@ int QOPX_D3_asqtad_invert_multi_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqta
   double t0 = wall_time();
   Lattice *lat = L.lat;
   const int n = nsrc * nmass;
-  // stage 1: multi-shift, sources in groups of up to `capacity`
   std::vector<DevVec> guess(n);
-  long ms_total = 0;
-  {
-    const int cap = std::min(capacity, nsrc);
-    MultiShiftBatch ms(L, eo, cap, options().ms_prec, masses, nmass);
-    for (int s0 = 0; s0 < nsrc; s0 += cap) {
-      const int cnt = std::min(cap, nsrc - s0);
-      for (int b = 0; b < cnt; b++) ms.load(b, in[s0 + b]->v, ms_tol, inv_arg->max_iter);
-      ms.run();
-      for (int b = 0; b < cnt; b++) {
-        if (ms_iters) ms_iters[s0 + b] = ms.iterations(b);
-        ms_total += ms.iterations(b);
-        for (int j = 0; j < nmass; j++) {
-          DevVec &g = guess[(s0 + b) * nmass + j];
-          devvec_alloc(g, lat, P64);
-          ms.solution(b, j, g);
+  // the multi-shift solutions are freed on every exit, also when a device
+  // allocation fails
+  struct FreeGuesses {
+    std::vector<DevVec> &g;
+    ~FreeGuesses() {
+      for (DevVec &v : g) devvec_free(v);
+    }
+  } free_guesses{guess};
+  long ms_total = 0, iters = 0;
+  int ok = 0;
+  const bool fit = guard_oom(__func__, info, [&] {
+    // stage 1: multi-shift, sources in groups of up to `capacity`
+    {
+      const int cap = std::min(capacity, nsrc);
+      MultiShiftBatch ms(L, eo, cap, options().ms_prec, masses, nmass);
+      for (int s0 = 0; s0 < nsrc; s0 += cap) {
+        const int cnt = std::min(cap, nsrc - s0);
+        for (int b = 0; b < cnt; b++) ms.load(b, in[s0 + b]->v, ms_tol, inv_arg->max_iter);
+        ms.run();
+        for (int b = 0; b < cnt; b++) {
+          if (ms_iters) ms_iters[s0 + b] = ms.iterations(b);
+          ms_total += ms.iterations(b);
+          for (int j = 0; j < nmass; j++) {
+            DevVec &g = guess[(s0 + b) * nmass + j];
+            devvec_alloc(g, lat, P64);
+            ms.solution(b, j, g);
+          }
         }
+        sync();
       }
-      sync();
     }
-  }
-  // stage 2: FP64 reliable-update refinement of every (source, mass)
-  const int cap = std::min(capacity, n);
-  BatchSolver bs(L, eo, cap, sp, options().delta);
-  std::vector<int> slot_of(cap, -1);
-  int next = 0, finished = 0, ok = 0;
-  long iters = 0;
-  while (finished < n) {
-    int s;
-    while (next < n && (s = bs.free_slot()) >= 0) {
-      const int src = next / nmass, j = next % nmass;
-      bs.load(s, in[src]->v, &guess[next], masses[j], std::sqrt(res[next]->rsqmin), inv_arg->max_iter);
-      slot_of[s] = next++;
+    // stage 2: FP64 reliable-update refinement of every (source, mass)
+    const int cap = std::min(capacity, n);
+    BatchSolver bs(L, eo, cap, sp, options().delta);
+    std::vector<int> slot_of(cap, -1);
+    int next = 0, finished = 0;
+    while (finished < n) {
+      int s;
+      while (next < n && (s = bs.free_slot()) >= 0) {
+        const int src = next / nmass, j = next % nmass;
+        bs.load(s, in[src]->v, &guess[next], masses[j], std::sqrt(res[next]->rsqmin), inv_arg->max_iter);
+        slot_of[s] = next++;
+      }
+      std::vector<int> done;
+      bs.step(done);
+      for (int d : done) {
+        const int k = slot_of[d];
+        SlotResult r;
+        bs.result(d, out[k]->v, true, &r);
+        res[k]->final_iter = r.iter;
+        res[k]->final_rsq = r.true_rsq;
+        res[k]->final_rel = 0;
+        res[k]->final_restart = r.nupdate;
+        ok += r.reason == 1;
+        iters += r.iter;
+        bs.release(d);
+        devvec_free(guess[k]);
+        finished++;
+      }
     }
-    std::vector<int> done;
-    bs.step(done);
-    for (int d : done) {
-      const int k = slot_of[d];
-      SlotResult r;
-      bs.result(d, out[k]->v, true, &r);
-      res[k]->final_iter = r.iter;
-      res[k]->final_rsq = r.true_rsq;
-      res[k]->final_rel = 0;
-      res[k]->final_restart = r.nupdate;
-      ok += r.reason == 1;
-      iters += r.iter;
-      bs.release(d);
-      devvec_free(guess[k]);
-      finished++;
-    }
-  }
+  });
+  if (!fit) return -1;
   if (info) {
     info->final_sec = wall_time() - t0;
     const double nflop = 0.5 * (20 * 3 + 2 * 8 * 9 * L.nl), nflopm = 0.5 * 10 * 3;

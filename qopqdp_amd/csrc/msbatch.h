@@ -60,6 +60,11 @@ class MultiShiftBatch {
   // sloppy_prec: P32 or P16.  masses: ns >= 1 masses shared by all sources.
   MultiShiftBatch(Links &L, int eo, int capacity, int sloppy_prec, const double *masses, int ns);
   ~MultiShiftBatch();
+
+ private:
+  void release();  // frees everything allocated so far
+
+ public:
   MultiShiftBatch(const MultiShiftBatch &) = delete;
   MultiShiftBatch &operator=(const MultiShiftBatch &) = delete;
   // load source b (full FP64 vector; its eo half is used) with a sloppy

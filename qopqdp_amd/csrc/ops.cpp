@@ -1,4 +1,6 @@
 // Field operations on device vectors (see ops.h).
+#include <memory>
+
 #include "ops.h"
 #include "solver.h"
 
@@ -62,10 +64,10 @@ void dslash(Links &L, double mass, DevVec &out, const DevVec &in, int eo_out, in
   const int prec = out.prec;
   // The hop reads `in` on the opposite parity while writing `out`; when they
   // alias, use a copy of the source (the reference copies it too, :1003-1007).
-  TmpVec *tcopy = nullptr;
+  std::unique_ptr<TmpVec> tcopy;
   const DevVec *src = &in;
   if (out.data == in.data) {
-    tcopy = new TmpVec(in.lat, in.prec);
+    tcopy.reset(new TmpVec(in.lat, in.prec));
     copy(tcopy->v, in, 2);
     src = &tcopy->v;
   }
@@ -92,7 +94,7 @@ void dslash(Links &L, double mass, DevVec &out, const DevVec &in, int eo_out, in
   }
   if (tcopy) {
     sync();
-    delete tcopy;
+    tcopy.reset();
   }
 }
 

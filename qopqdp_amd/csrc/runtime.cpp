@@ -13,6 +13,17 @@ namespace qop {
   abort();
 }
 
+void dev_alloc_bytes(void **p, size_t bytes) {
+  *p = nullptr;
+  const hipError_t e = hipMalloc(p, bytes);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();  // clear the error state of this thread
+    *p = nullptr;
+    throw DeviceOutOfMemory{bytes};
+  }
+  if (e != hipSuccess) fatal(__FILE__, __LINE__, hipGetErrorString(e));
+}
+
 static std::mutex g_mu;
 
 Lattice *lattice_get(const int L[4]) {
@@ -43,11 +54,16 @@ Lattice *lattice_get(const int L[4]) {
 void devvec_alloc(DevVec &v, Lattice *lat, int prec) {
   v.lat = lat;
   v.prec = prec;
-  QOP_CHECK_HIP(hipMalloc(&v.data, (size_t)2 * 3 * lat->Vh * prec_cbytes(prec)));
-  QOP_CHECK_HIP(hipMemsetAsync(v.data, 0, (size_t)2 * 3 * lat->Vh * prec_cbytes(prec), stream()));
   v.norm = nullptr;
+  dev_alloc(&v.data, (size_t)2 * 3 * lat->Vh * prec_cbytes(prec));
+  QOP_CHECK_HIP(hipMemsetAsync(v.data, 0, (size_t)2 * 3 * lat->Vh * prec_cbytes(prec), stream()));
   if (prec == P16) {
-    QOP_CHECK_HIP(hipMalloc(&v.norm, (size_t)2 * lat->Vh * sizeof(float)));
+    try {
+      dev_alloc(&v.norm, (size_t)2 * lat->Vh * sizeof(float));
+    } catch (const DeviceOutOfMemory &) {
+      devvec_free(v);
+      throw;
+    }
     QOP_CHECK_HIP(hipMemsetAsync(v.norm, 0, (size_t)2 * lat->Vh * sizeof(float), stream()));
   }
 }
@@ -64,11 +80,16 @@ void devbatch_alloc(DevBatch &v, Lattice *lat, int prec, int n) {
   v.prec = prec;
   v.n = n;
   size_t bytes = (size_t)n * 3 * lat->Vh * prec_cbytes(prec);
-  QOP_CHECK_HIP(hipMalloc(&v.data, bytes));
-  QOP_CHECK_HIP(hipMemsetAsync(v.data, 0, bytes, stream()));
   v.norm = nullptr;
+  dev_alloc(&v.data, bytes);
+  QOP_CHECK_HIP(hipMemsetAsync(v.data, 0, bytes, stream()));
   if (prec == P16) {
-    QOP_CHECK_HIP(hipMalloc(&v.norm, (size_t)n * lat->Vh * sizeof(float)));
+    try {
+      dev_alloc(&v.norm, (size_t)n * lat->Vh * sizeof(float));
+    } catch (const DeviceOutOfMemory &) {
+      devbatch_free(v);
+      throw;
+    }
     QOP_CHECK_HIP(hipMemsetAsync(v.norm, 0, (size_t)n * lat->Vh * sizeof(float), stream()));
   }
 }
@@ -107,9 +128,9 @@ LinkSet &Links::get_single(int prec) {
     LinkSet &src = get(prec);
     const int nls = nl / 2, per = nl / 4;
     const size_t blk = (size_t)9 * lat->Vh * prec_cbytes(prec);
+    dev_alloc(&d.data, (size_t)2 * nls * blk);
     d.prec = prec;
     d.nl = nls;
-    QOP_CHECK_HIP(hipMalloc(&d.data, (size_t)2 * nls * blk));
     for (int p = 0; p < 2; p++)
       for (int mu = 0; mu < 4; mu++)
         for (int j = 0; j < per / 2; j++) {
@@ -156,9 +177,12 @@ struct Scratch {
       QOP_CHECK_HIP(hipStreamSynchronize(stream()));
       if (host) QOP_CHECK_HIP(hipHostFree(p)); else QOP_CHECK_HIP(hipFree(p));
     }
-    n = bytes < (1u << 20) ? (1u << 20) : bytes * 2;
-    if (host) QOP_CHECK_HIP(hipHostMalloc(&p, n, hipHostMallocDefault));
-    else QOP_CHECK_HIP(hipMalloc(&p, n));
+    p = nullptr;
+    n = 0;
+    const size_t want = bytes < (1u << 20) ? (1u << 20) : bytes * 2;
+    if (host) QOP_CHECK_HIP(hipHostMalloc(&p, want, hipHostMallocDefault));
+    else dev_alloc(&p, want);
+    n = want;
     return p;
   }
 };

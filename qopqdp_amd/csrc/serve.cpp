@@ -127,9 +127,18 @@ class Server {
       used_ += cf->bytes;
     }
     cf->links.lat = cf->lat;
-    if (U) links_from_gauge64(cf->links, U, c, r0, bc, sign);
-    else links_from_raw64(cf->links, fat, lng);
-    if (prec_ != P64) cf->links.get(prec_);
+    const bool fit = guard_oom("QOPX_serve_add_config", nullptr, [&] {
+      if (U) links_from_gauge64(cf->links, U, c, r0, bc, sign);
+      else links_from_raw64(cf->links, fat, lng);
+      if (prec_ != P64) cf->links.get(prec_);
+    });
+    if (!fit) {
+      sync();
+      cf->links.free_all();
+      std::lock_guard<std::mutex> lk(mu_);
+      used_ -= cf->bytes;
+      return -1;
+    }
     std::lock_guard<std::mutex> lk(mu_);
     cfgs_.push_back(std::move(cf));
     return (int)cfgs_.size() - 1;
@@ -337,16 +346,30 @@ class Server {
       std::lock_guard<std::mutex> lk(mu_);
       used_ += need;
     }
-    devvec_alloc(e->stage, c.lat, P64);
-    QOP_CHECK_HIP(hipMalloc(&e->raw, (size_t)c.lat->V * 6 * sizeof(double)));
-    if (prec_ != P64) {
-      QOP_CHECK_HIP(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
-      e->own_stream = true;
-      e->bs = std::make_unique<BatchSolver>(c.links, eo, cap_, prec_, 0.1, e->st);
-      e->slot_req.assign(cap_, 0);
-    } else {
-      e->st = stream();  // the reference solver runs on the library stream
-      e->slot_req.assign(1, 0);
+    try {
+      devvec_alloc(e->stage, c.lat, P64);
+      dev_alloc(&e->raw, (size_t)c.lat->V * 6 * sizeof(double));
+      if (prec_ != P64) {
+        QOP_CHECK_HIP(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking));
+        e->own_stream = true;
+        e->bs = std::make_unique<BatchSolver>(c.links, eo, cap_, prec_, 0.1, e->st);
+        e->slot_req.assign(cap_, 0);
+      } else {
+        e->st = stream();  // the reference solver runs on the library stream
+        e->slot_req.assign(1, 0);
+      }
+    } catch (const DeviceOutOfMemory &) {
+      // the device holds less than the budget promised (another process, or
+      // fragmentation): give the memory back; the next admission pass evicts
+      // idle engines first, and a request with no engine of ours to evict
+      // fails
+      drop_engine(*e);
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        used_ -= need;
+      }
+      *never = engines_.empty();
+      return nullptr;
     }
     sync();
     e->last_used = now();
@@ -421,16 +444,20 @@ class Server {
     QOP_CHECK_HIP(hipMemcpyAsync(e.raw + 6 * b, q.src + 6 * b, (size_t)Vh * 6 * sizeof(double),
                                  hipMemcpyHostToDevice, st));
     launch_raw_to_vec(lat, e.raw, e.stage, q.eo, st);
-    DevVec out;
-    devvec_alloc(out, c.lat, P64);
     InvArg ia{1e9, q.max_iter, q.max_iter, 5, q.eo};
     ResArg ra{q.tol * q.tol, 0, 0, 0, 0, 0};
-    invert(c.links, &ia, &ra, q.mass, out, e.stage, nullptr, nullptr);
-    launch_vec_to_raw(lat, out, e.raw, q.eo, st);
-    QOP_CHECK_HIP(hipMemcpyAsync(q.out + 6 * b, e.raw + 6 * b, (size_t)Vh * 6 * sizeof(double),
-                                 hipMemcpyDeviceToHost, st));
-    sync();
-    devvec_free(out);
+    const bool fit = guard_oom("QOPX_serve (FP64 solve)", nullptr, [&] {
+      TmpVec out(c.lat, P64);
+      invert(c.links, &ia, &ra, q.mass, out.v, e.stage, nullptr, nullptr);
+      launch_vec_to_raw(lat, out.v, e.raw, q.eo, st);
+      QOP_CHECK_HIP(hipMemcpyAsync(q.out + 6 * b, e.raw + 6 * b, (size_t)Vh * 6 * sizeof(double),
+                                   hipMemcpyDeviceToHost, st));
+      sync();
+    });
+    if (!fit) {
+      fail(id);
+      return;
+    }
     std::lock_guard<std::mutex> lk(mu_);
     Request &r = reqs_[id];
     r.iter = ra.final_iter;

@@ -8,6 +8,7 @@
 // Vectors are FP64 device vectors; FP64 scalars throughout.  The FP32 branch
 // of solve_multi (mixed_rsq) runs the same multi-shift CG on FP32 vectors and
 // FP32 links, as the reference does.
+#include <memory>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -50,20 +51,30 @@ struct CgmsBuffers {
   int nch = 0;
   CgmsBuffers(Lattice *lat, int ns) {
     nch = (int)((lat->Vh + 63) / 64);
-    QOP_CHECK_HIP(hipMalloc(&d.st, sizeof(CgmsState)));
-    QOP_CHECK_HIP(hipMalloc(&d.partA, 4 * (size_t)nch * sizeof(double)));
-    d.partB = d.partA + nch;
-    d.partC = d.partB + nch;
-    d.partD = d.partC + nch;
-    QOP_CHECK_HIP(hipMalloc(&d.xs, 2 * (size_t)ns * sizeof(VecView)));
-    d.pms = d.xs + ns;
+    try {
+      dev_alloc(&d.st, sizeof(CgmsState));
+      dev_alloc(&d.partA, 4 * (size_t)nch * sizeof(double));
+      d.partB = d.partA + nch;
+      d.partC = d.partB + nch;
+      d.partD = d.partC + nch;
+      dev_alloc(&d.xs, 2 * (size_t)ns * sizeof(VecView));
+      d.pms = d.xs + ns;
+    } catch (const DeviceOutOfMemory &) {
+      release();
+      throw;
+    }
     QOP_CHECK_HIP(hipHostMalloc(&host, sizeof(CgmsState), hipHostMallocDefault));
   }
-  ~CgmsBuffers() {
-    (void)hipFree(d.st);
-    (void)hipFree(d.partA);
-    (void)hipFree(d.xs);
-    (void)hipHostFree(host);
+  ~CgmsBuffers() { release(); }
+  void release() {
+    if (d.st) (void)hipFree(d.st);
+    if (d.partA) (void)hipFree(d.partA);
+    if (d.xs) (void)hipFree(d.xs);
+    if (host) (void)hipHostFree(host);
+    d.st = nullptr;
+    d.partA = nullptr;
+    d.xs = nullptr;
+    host = nullptr;
   }
   void pull(hipStream_t st) {
     QOP_CHECK_HIP(hipMemcpyAsync(host, d.st, sizeof(CgmsState), hipMemcpyDeviceToHost, st));
@@ -102,10 +113,10 @@ void invert_cgms(Links &L, int prec, double mass, int eo, InvArg *ia, ResArg **r
 
   TmpVec r(lat, prec), Mp(lat, prec), tmp(lat, prec), p(lat, prec);
   TmpVec y(lat, prec), Ay(lat, prec), ry(lat, prec);
-  std::vector<TmpVec *> pm(ns, nullptr);
+  std::vector<std::unique_ptr<TmpVec>> pm(ns);
   copy(r.v, in, eo);
   for (int i = 0; i < ns; i++) {
-    if (i != imin) pm[i] = new TmpVec(lat, prec);
+    if (i != imin) pm[i].reset(new TmpVec(lat, prec));
     DevVec &pv = i == imin ? p.v : pm[i]->v;
     zero(*out[i], eo);
     copy(pv, r.v, eo);
@@ -257,7 +268,6 @@ void invert_cgms(Links &L, int prec, double mass, int eo, InvArg *ia, ResArg **r
     ra[i]->final_rel = s * S.trueRelFac * S.relnorm2;
     ra[i]->final_iter = S.iteration;
     ra[i]->final_restart = 0;
-    if (i != imin) delete pm[i];
   }
 }
 
@@ -388,7 +398,7 @@ void solve_multi(Links &L, InvArg *ia, ResArg **ra, const double *masses, DevVec
   const int max_iter = ia->max_iter, max_restarts = ia->max_restarts;
   const double mixedrsq = ia->mixed_rsq;
   double dtime = 0;
-  std::vector<TmpVec *> rv(nsolve), av(nsolve);
+  std::vector<std::unique_ptr<TmpVec>> rv(nsolve), av(nsolve);
   TmpVec tv(lat, P64);
   for (int i = 0; i < nsolve; i++) {
     in2[i] = norm2(*in[i], ineo);
@@ -402,8 +412,8 @@ void solve_multi(Links &L, InvArg *ia, ResArg **ra, const double *masses, DevVec
       m2s[j] = mass2[i];
       nm++;
     }
-    rv[i] = new TmpVec(lat, P64);
-    av[i] = new TmpVec(lat, P64);
+    rv[i].reset(new TmpVec(lat, P64));
+    av[i].reset(new TmpVec(lat, P64));
   }
   for (int i = 0; i < nsolve; i++) {
     int j = 0;
@@ -412,13 +422,13 @@ void solve_multi(Links &L, InvArg *ia, ResArg **ra, const double *masses, DevVec
   }
   ResArg resarg0{0, 0, 0, 0, 0, 0};
   std::vector<ResArg *> xresarg(nm);
-  std::vector<TmpVec *> xv(nm), axv(nm);
+  std::vector<std::unique_ptr<TmpVec>> xv(nm), axv(nm);
   double m20 = m2s[0];
   double gl_mass = std::sqrt(m20);
   for (int i = 0; i < nm; i++) {
     m2s[i] -= m20;
-    xv[i] = new TmpVec(lat, P64);
-    axv[i] = new TmpVec(lat, P64);
+    xv[i].reset(new TmpVec(lat, P64));
+    axv[i].reset(new TmpVec(lat, P64));
   }
   std::vector<DevVec *> r(nsolve), Aout(nsolve), x(nm), Ax(nm);
   for (int i = 0; i < nsolve; i++) { r[i] = &rv[i]->v; Aout[i] = &av[i]->v; }
@@ -465,11 +475,11 @@ void solve_multi(Links &L, InvArg *ia, ResArg **ra, const double *masses, DevVec
       for (int i = 0; i < nm; i++) m2sf[i] = (float)m2s[i];
       TmpVec rf(lat, P32);
       copy(rf.v, *r[imax], ineo);
-      std::vector<TmpVec *> xf(nm);
+      std::vector<std::unique_ptr<TmpVec>> xf(nm);
       std::vector<DevVec *> xfp(nm);
-      for (int i = 0; i < nm; i++) { xf[i] = new TmpVec(lat, P32); xfp[i] = &xf[i]->v; }
+      for (int i = 0; i < nm; i++) { xf[i].reset(new TmpVec(lat, P32)); xfp[i] = &xf[i]->v; }
       invert_cgms(L, P32, gl_mass, ineo, ia, xresarg.data(), m2sf.data(), nm, xfp.data(), rf.v);
-      for (int i = 0; i < nm; i++) { copy(*x[i], xf[i]->v, ineo); sync(); delete xf[i]; }
+      for (int i = 0; i < nm; i++) { copy(*x[i], xf[i]->v, ineo); sync(); xf[i].reset(); }
     } else {
       invert_cgms(L, P64, gl_mass, ineo, ia, xresarg.data(), m2s.data(), nm, x.data(), *r[imax]);
     }
@@ -504,8 +514,6 @@ void solve_multi(Links &L, InvArg *ia, ResArg **ra, const double *masses, DevVec
   double nflop = 0.5 * (20 * 3 + 2 * 8 * 9 * L.nl), nflopm = 0.5 * 10 * 3;
   if (final_flop) *final_flop = (nflop + nflopm * (nm - 1)) * iter * (double)lat->V;
   if (final_sec) *final_sec = dtime;
-  for (int i = 0; i < nm; i++) { delete xv[i]; delete axv[i]; }
-  for (int i = 0; i < nsolve; i++) { delete rv[i]; delete av[i]; }
 }
 
 // ---------------------------------------------------------------------------

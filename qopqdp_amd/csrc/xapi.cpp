@@ -10,33 +10,32 @@
 
 using namespace qop;
 
-extern "C" {
-
-// Time `nrep` launches of the even->odd hop kernel in precision prec
-// (64/32/16) on nrhs right-hand sides with the autotuned launch shape.
-// red = 1 selects the instance with the fused |out|^2 reduction (the first
-// hop of a batched CG iteration), red = 2 the instance with the fused CG
-// update r -= a (m^2 p - D t), x_FP32 += a p, |r|^2 (the second hop; sloppy
-// precisions only).  Returns the mean ms per launch from
-// HIP events on the library stream; shape[8] receives (rhs per thread, block,
-// XCD remap, non-temporal links, RHS-innermost block order, x0 tile, LDS
-// staging, x0 walk).
-double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, int nrep,
-                               int red, int *shape) {
+// the body of QOPX_D3_asqtad_hop_time (below)
+static double hop_time_impl(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, int nrep,
+                            int red, int *shape) {
   Links &L = fla->L;
   Lattice *lat = L.lat;
   int p = prec == 64 ? P64 : prec == 32 ? P32 : P16;
   QOP_REQUIRE(nrhs >= 1 && nrhs <= 64 && nrep >= 1, "hop_time: bad arguments");
-  DevBatch src, dst;
+  DevBatch src, dst, xb, xs;
+  DevVec tmp;
+  // everything is freed on return, also when an allocation fails
+  struct Free {
+    DevBatch *b[4];
+    DevVec *v;
+    ~Free() {
+      for (DevBatch *x : b) devbatch_free(*x);
+      devvec_free(*v);
+    }
+  } fr{{&src, &dst, &xb, &xs}, &tmp};
   devbatch_alloc(src, lat, p, nrhs);
   devbatch_alloc(dst, lat, p, nrhs);
   // fill the source with a constant so FP16 norms are nonzero
   {
     std::vector<double> one((size_t)lat->V * 6, 0.5);
-    DevVec tmp;
     devvec_alloc(tmp, lat, P64);
-    double *d;
-    QOP_CHECK_HIP(hipMalloc(&d, one.size() * sizeof(double)));
+    DevBuf<double> dbuf(one.size() * sizeof(double));
+    double *d = dbuf.p;
     QOP_CHECK_HIP(hipMemcpy(d, one.data(), one.size() * sizeof(double), hipMemcpyHostToDevice));
     launch_raw_to_vec(lat, d, tmp, 2, stream());
     for (int b = 0; b < nrhs; b++) {
@@ -49,11 +48,8 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
       launch_lincomb(a, p, pin, 1, false, stream());
     }
     sync();
-    QOP_CHECK_HIP(hipFree(d));
-    devvec_free(tmp);
   }
   QOP_REQUIRE(red >= 0 && red <= 2 && (red != 2 || p != P64), "hop_time: bad reduction");
-  DevBatch xb, xs;
   if (red == 2) {  // operands of the fused CG update: p (x), x_FP32
     devbatch_alloc(xb, lat, p, nrhs);
     devbatch_alloc(xs, lat, P32, nrhs);
@@ -75,13 +71,15 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
   a.nstride = lat->Vh;
   a.n = nrhs;
   hs.apply(a);
-  double *part = nullptr, *alpha = nullptr;
+  DevBuf<double> part, alpha_buf;
   if (red) {
-    QOP_CHECK_HIP(hipMalloc(&part, (size_t)nrhs * ((lat->Vh + 63) / 64) * sizeof(double)));
-    a.partial = part;
+    dev_alloc(&part.p, (size_t)nrhs * ((lat->Vh + 63) / 64) * sizeof(double));
+    a.partial = part.p;
   }
+  double *alpha = nullptr;
   if (red == 2) {  // the fused CG update: x = p, dst = r, alpha 0
-    QOP_CHECK_HIP(hipMalloc(&alpha, nrhs * sizeof(double)));
+    dev_alloc(&alpha_buf.p, nrhs * sizeof(double));
+    alpha = alpha_buf.p;
     QOP_CHECK_HIP(hipMemsetAsync(alpha, 0, nrhs * sizeof(double), stream()));
     QOP_CHECK_HIP(hipMemcpyAsync(xb.data, src.data, (size_t)nrhs * 3 * lat->Vh * prec_cbytes(p),
                                  hipMemcpyDeviceToDevice, stream()));
@@ -110,14 +108,7 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
   QOP_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
   QOP_CHECK_HIP(hipEventDestroy(e0));
   QOP_CHECK_HIP(hipEventDestroy(e1));
-  if (part) QOP_CHECK_HIP(hipFree(part));
-  if (alpha) QOP_CHECK_HIP(hipFree(alpha));
-  if (red == 2) {
-    devbatch_free(xb);
-    devbatch_free(xs);
-  }
-  devbatch_free(src);
-  devbatch_free(dst);
+  sync();
   if (shape) {
     shape[0] = hs.nb; shape[1] = hs.block; shape[2] = hs.remap; shape[3] = hs.nt; shape[4] = hs.rhs_inner;
     shape[5] = hs.tile;
@@ -127,9 +118,27 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
   return ms / nrep;
 }
 
+extern "C" {
+
+// Time `nrep` launches of the even->odd hop kernel in precision prec
+// (64/32/16) on nrhs right-hand sides with the autotuned launch shape.
+// red = 1 selects the instance with the fused |out|^2 reduction (the first
+// hop of a batched CG iteration), red = 2 the instance with the fused CG
+// update r -= a (m^2 p - D t), x_FP32 += a p, |r|^2 (the second hop; sloppy
+// precisions only).  Returns the mean ms per launch from
+// HIP events on the library stream; shape[8] receives (rhs per thread, block,
+// XCD remap, non-temporal links, RHS-innermost block order, x0 tile, LDS
+// staging, x0 walk).
+double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, int nrep,
+                               int red, int *shape) {
+  double ms = -1.0;  // -1: the device memory is exhausted
+  guard_oom(__func__, nullptr, [&] { ms = hop_time_impl(fla, prec, nrhs, nrep, red, shape); });
+  return ms;
+}
+
 // Force the multi-RHS hop launch shape (rhs per thread, block, XCD remap,
-// non-temporal links, RHS-innermost order, x0 tile, LDS reserve, x0 walk) for solvers built from
-// now on, wherever it is valid; null restores autotuning.
+// non-temporal links, RHS-innermost order, x0 tile, LDS staging, x0 walk) for solvers built
+// from now on, wherever it is valid; null restores autotuning.
 void QOPX_hop_shape_override(const int *shape) {
   if (!shape) {
     set_hop_shape_override(nullptr);
@@ -159,33 +168,37 @@ int QOPX_D3_asqtad_invert_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla
   const int sp = prec == 16 ? P16 : P32;
   double t0 = wall_time();
   const int cap = std::min(capacity, n);
-  BatchSolver bs(L, inv_arg->evenodd, cap, sp, options().delta);
-  std::vector<int> slot_of(cap, -1);
-  int next = 0, finished = 0, ok = 0;
+  int ok = 0;
   long iters = 0;
-  while (finished < n) {
-    int s;
-    while (next < n && (s = bs.free_slot()) >= 0) {
-      bs.load(s, in[next]->v, nullptr, masses[next], std::sqrt(res[next]->rsqmin), inv_arg->max_iter);
-      slot_of[s] = next++;
+  const bool fit = guard_oom(__func__, info, [&] {
+    BatchSolver bs(L, inv_arg->evenodd, cap, sp, options().delta);
+    std::vector<int> slot_of(cap, -1);
+    int next = 0, finished = 0;
+    while (finished < n) {
+      int s;
+      while (next < n && (s = bs.free_slot()) >= 0) {
+        bs.load(s, in[next]->v, nullptr, masses[next], std::sqrt(res[next]->rsqmin), inv_arg->max_iter);
+        slot_of[s] = next++;
+      }
+      std::vector<int> done;
+      bs.step(done);
+      for (int d : done) {
+        int k = slot_of[d];
+        SlotResult r;
+        bs.result(d, out[k]->v, true, &r);
+        res[k]->final_iter = r.iter;
+        res[k]->final_rsq = r.true_rsq;
+        res[k]->final_rel = 0;
+        res[k]->final_restart = r.nupdate;
+        ok += r.reason == 1;
+        iters += r.iter;
+        if (latency) latency[k] = wall_time() - t0;
+        bs.release(d);
+        finished++;
+      }
     }
-    std::vector<int> done;
-    bs.step(done);
-    for (int d : done) {
-      int k = slot_of[d];
-      SlotResult r;
-      bs.result(d, out[k]->v, true, &r);
-      res[k]->final_iter = r.iter;
-      res[k]->final_rsq = r.true_rsq;
-      res[k]->final_rel = 0;
-      res[k]->final_restart = r.nupdate;
-      ok += r.reason == 1;
-      iters += r.iter;
-      if (latency) latency[k] = wall_time() - t0;
-      bs.release(d);
-      finished++;
-    }
-  }
+  });
+  if (!fit) return -1;
   if (info) {
     info->final_sec = wall_time() - t0;
     info->final_flop = 0.5 * (20 * 3 + 2 * 8 * 9 * L.nl) * (double)iters * (double)L.lat->V;
@@ -199,9 +212,13 @@ int QOPX_D3_asqtad_invert_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla
 // copy bandwidth next to the nominal peak.
 double QOPX_copy_time(unsigned long long bytes, int nrep) {
   QOP_REQUIRE(bytes >= 16 && bytes % 16 == 0 && nrep >= 1, "copy_time: bytes must be a multiple of 16");
-  void *a, *b;
-  QOP_CHECK_HIP(hipMalloc(&a, bytes));
-  QOP_CHECK_HIP(hipMalloc(&b, bytes));
+  DevBuf<void> abuf, bbuf;
+  if (!guard_oom(__func__, nullptr, [&] {
+        dev_alloc(&abuf.p, bytes);
+        dev_alloc(&bbuf.p, bytes);
+      }))
+    return -1.0;
+  void *a = abuf.p, *b = bbuf.p;
   QOP_CHECK_HIP(hipMemsetAsync(a, 0, bytes, stream()));
   hipStream_t st = stream();
   for (int w = 0; w < 2; w++) launch_copy(a, b, bytes, st);
@@ -216,8 +233,6 @@ double QOPX_copy_time(unsigned long long bytes, int nrep) {
   QOP_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
   QOP_CHECK_HIP(hipEventDestroy(e0));
   QOP_CHECK_HIP(hipEventDestroy(e1));
-  QOP_CHECK_HIP(hipFree(a));
-  QOP_CHECK_HIP(hipFree(b));
   return ms / nrep;
 }
 
