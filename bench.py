@@ -54,6 +54,10 @@ def main():
     ap.add_argument("--mass", type=float, default=0.01)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--fp64-solves", type=int, default=2)
+    ap.add_argument("--ru-join", type=float, default=0.3,
+                    help="qmx_ru_join of the batched solves: a slot whose residual dropped by this "
+                         "factor joins another slot's reliable update (0: off, bitwise "
+                         "batch-invariant results)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-only", action="store_true",
@@ -128,6 +132,7 @@ def main():
     vins = [s.vec(x) for x in srcs]
     vouts = [s.zeros() for _ in range(B)]
     P = C.c_void_p
+    assert s.set_opts(qmx_ru_join=args.ru_join) == 0
 
     def batch_step():
         ras = [q.resid_arg_default() for _ in range(B)]
@@ -244,6 +249,7 @@ def main():
                         f"{B} Gaussian sources per step, FP64 true residual {args.tol}",
             "lattice": L, "batch": B, "mass": args.mass, "tol": args.tol,
             "inner_prec": args.prec, "gauge": "reunit(exp(0.3iH)), MILC phases, asqtad smearing",
+            "reliable_update_delta": 0.1, "ru_join": args.ru_join,
         },
         "p50_latency_ms": round(float(np.percentile(lat_ms, 50)), 1),
         "p99_latency_ms": round(float(np.percentile(lat_ms, 99)), 1),
@@ -341,6 +347,7 @@ def multi_bench(args):
     log(f"setup {time.time() - t0:.1f}s, links {prep}")
     srcs = [q.gen_source(L, 2000 + 100 * 4 + j, q.QOP_EVEN) for j in range(nsrc)]
     vins = [s.vec(x) for x in srcs]
+    assert s.set_opts(qmx_ru_join=args.ru_join) == 0  # the refinement's batched CG
     nm = len(masses)
     n = nsrc * nm
     P = C.c_void_p
@@ -424,7 +431,8 @@ def multi_bench(args):
         "dtype": f"fp{args.prec} storage, fp32 arithmetic, fp64 reductions/recurrences/true residual",
         "config": {"workload": ("HISQ" if args.hisq else "asqtad") + f" {'x'.join(map(str, L))}, "
                    f"{nsrc} sources x {nm} masses", "lattice": L, "sources": nsrc,
-                   "masses": [round(m, 8) for m in masses], "tol": args.tol},
+                   "masses": [round(m, 8) for m in masses], "tol": args.tol,
+                   "ru_join": args.ru_join},
         "links": prep, "seconds": round(dt, 3), "converged": ok, "solves": n,
         "multi_shift_iters": ms_it, "max_fp64_true_resid_per_mass": per_shift,
         "fp64_sequential": {"solves_per_s": round(fp_rate, 3), "sources_timed": len(fp_t),

@@ -259,6 +259,9 @@ QOP_status_t QOP_asqtad_invert_set_opts(QOP_opt_t opts[], int nopts) {
   get("qmx_graph", graph);
   int msp = o.ms_prec == P16 ? 16 : 32;
   get("qmx_ms_prec", msp);
+  double join = o.ru_join;
+  get("qmx_ru_join", join);
+  if (!(join >= 0 && join < 1)) return QOP_FAIL;
   if (msp != 16 && msp != 32) return QOP_FAIL;
   if (solver < 0 || solver > 1) return QOP_FAIL;
   if (prec != 64 && prec != 32 && prec != 16) return QOP_FAIL;
@@ -271,6 +274,7 @@ QOP_status_t QOP_asqtad_invert_set_opts(QOP_opt_t opts[], int nopts) {
   o.chunk = chunk;
   o.use_graph = graph;
   o.ms_prec = msp == 16 ? P16 : P32;
+  o.ru_join = join;
   return QOP_SUCCESS;
 }
 

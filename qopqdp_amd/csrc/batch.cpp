@@ -168,6 +168,7 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
 BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double delta,
                          hipStream_t st)
     : L_(L), lat_(L.lat), eo_(eo), cap_(capacity), sp_(sloppy_prec), delta_(delta),
+      join_(options().ru_join),
       st_(st ? st : stream()) {
   try {
     QOP_REQUIRE(eo == 0 || eo == 1, "batch solver: subset must be even or odd");
@@ -395,6 +396,7 @@ void BatchSolver::load(int slot, const DevVec &src, const DevVec *x0, double mas
   R.stop2 = tol * tol * R.b2;
   R.max_iter = max_iter;
   R.delta2 = delta_ * delta_;
+  R.join2 = join_ * join_;
   mass_[slot] = mass;
   loaded_[slot] = 1;
   const bool done = R.r2 <= R.stop2;
@@ -442,6 +444,7 @@ void BatchSolver::enqueue_iteration() {
   shape2_.apply(b);
   launch_hop_args(b, sp_, ls.nl, shape2_.nb, shape2_.block, RED_CG, true, st);
   bs_launch_scalar2(rhs_, active_, glob_, partC_, nch_, cap_, ru_, st);
+  if (join_ > 0) bs_launch_ru_join(rhs_, active_, ru_, cap_, st);
   bs_launch_pupdate(rhs_, active_, &glob_->status, sp_, (int)Vh, rs_.slot(0), ps_.slot(0), 3 * Vh, cap_, 1,
                     partB_, st);
   // reliable update of the flagged slots, all on the device

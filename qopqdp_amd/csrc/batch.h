@@ -34,6 +34,7 @@ struct BsRhs {
   double stop2;     // tol^2 |src|^2
   double b2;        // |src|^2
   double delta2;    // reliable-update threshold (delta^2)
+  double join2;     // joins another slot's update below join2 * r2_rel (0: never)
 };
 
 struct BsGlobal {
@@ -56,6 +57,9 @@ void bs_launch_ru(BsRhs *rhs, int *active, int *ru, int *ru_cont, double *lcoef,
                   int Vh, VecView x64, VecView xs, VecView b64, VecView ax64, VecView rs,
                   VecView ps, long stride, double *pr2, double *pre, double *pim, int nch,
                   int n, int stage, double *ppB, hipStream_t st);
+// A slot whose residual has dropped to join2 * r2_rel joins the reliable
+// update of this iteration when another slot is flagged for one.
+void bs_launch_ru_join(BsRhs *rhs, const int *active, int *ru, int n, hipStream_t st);
 void bs_launch_commit(BsGlobal *glob, hipStream_t st);
 void bs_launch_pupdate(const BsRhs *rhs, const int *active, const int *status,
                        int prec, int Vh, VecView rs, VecView ps, long stride, int n,
@@ -131,6 +135,7 @@ class BatchSolver {
   Lattice *lat_;
   int eo_, cap_, sp_;
   double delta_;
+  double join_;  // reliable-update joining threshold (residual drop), 0: off
   hipStream_t st_;
   hipEvent_t ev_ = nullptr;
   bool inflight_ = false;

@@ -363,6 +363,23 @@ bs_ru_pfix(const int *ru_cont, const double *lcoef, int Vh, VecView rs, VecView 
   if ((threadIdx.x & 63) == 0 && (h >> 6) < nch) partB[(long)b * nch + (h >> 6)] = acc;
 }
 
+// One thread: if any slot is flagged for a reliable update in this
+// iteration, flag every running slot whose iterated residual has already
+// dropped to join2 * r2_rel, so that the updates share the FP64 link reads.
+// (A joined slot's update then depends on its batch-mates.)
+__global__ void bs_ru_join(BsRhs *rhs, const int *active, int *ru, int n) {
+  if (threadIdx.x != 0) return;
+  bool any = false;
+  for (int b = 0; b < n; b++) any |= active[b] && ru[b];
+  if (!any) return;
+  for (int b = 0; b < n; b++) {
+    BsRhs &R = rhs[b];
+    if (!active[b] || ru[b] || !(R.join2 > 0) || !(R.r2 <= R.join2 * R.r2_rel)) continue;
+    R.need_update = 1;
+    ru[b] = 1;
+  }
+}
+
 // Last kernel of an iteration: raise the status the host polls.  Kernels of
 // one iteration all see the same status because only this kernel changes it.
 __global__ void bs_commit(BsGlobal *glob) {
@@ -396,6 +413,10 @@ void bs_launch_update(BsRhs *rhs, const int *active, const int *status, int prec
 void bs_launch_scalar2(BsRhs *rhs, const int *active, BsGlobal *glob, const double *partC, int nch,
                        int n, int *ru, hipStream_t st) {
   hipLaunchKernelGGL(bs_scalar2, dim3(n), dim3(BS_THREADS), 0, st, rhs, active, glob, partC, nch, ru);
+  QOP_CHECK_HIP(hipGetLastError());
+}
+void bs_launch_ru_join(BsRhs *rhs, const int *active, int *ru, int n, hipStream_t st) {
+  hipLaunchKernelGGL(bs_ru_join, dim3(1), dim3(64), 0, st, rhs, active, ru, n);
   QOP_CHECK_HIP(hipGetLastError());
 }
 void bs_launch_commit(BsGlobal *glob, hipStream_t st) {

@@ -51,6 +51,8 @@ struct Options {
   double delta = 0.1;    // reliable-update residual drop
   int chunk = 16;        // iterations enqueued between host status checks
   int use_graph = 1;     // capture the iteration chunk in a hipGraph
+  double ru_join = 0.0;  // batched CG: a slot whose residual dropped by this factor since its
+                         // last reliable update joins another slot's update (0: never)
   int ms_prec = P32;     // vector precision of the batched multi-shift stage
   int verbose = 0;
 };

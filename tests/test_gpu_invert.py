@@ -274,3 +274,29 @@ def test_batch_api_matches_qop_invert(setup, oracle):
             assert ra.final_iter == r.final_iter
     finally:
         s.set_opts(qmx_solver=0)
+
+
+def test_batch_reliable_update_joining(setup, oracle):
+    """qmx_ru_join: a slot whose residual has dropped by the given factor joins
+    another slot's reliable update.  Every system still reaches the FP64 true
+    residual; the solutions agree with the unjoined (bitwise batch-invariant)
+    solves within the condition number times the tolerance (here 1e-5
+    relative for m = 0.01), not bitwise."""
+    s, L, h = setup["s"], setup["L"], setup["h"]
+    srcs = [q.gen_source(L, 2090 + i, EVEN) for i in range(4)]
+    masses = [0.01, 0.02, 0.05, 0.01]
+    tol = 1e-10
+    ok0, outs0, _ = batch_api(s, setup["fla"], srcs, masses, tol, 16, capacity=4)
+    assert s.set_opts(qmx_ru_join=0.3) == 0
+    try:
+        ok, outs, ras = batch_api(s, setup["fla"], srcs, masses, tol, 16, capacity=4)
+    finally:
+        assert s.set_opts(qmx_ru_join=0) == 0
+    assert ok == ok0 == 4
+    V = s.V
+    for b, m, o, o0, r in zip(srcs, masses, outs, outs0, ras):
+        assert oracle.true_resid_even(h, m, o, b) <= tol
+        d = np.linalg.norm(o[: V // 2] - o0[: V // 2]) / np.linalg.norm(o0[: V // 2])
+        assert d < 1e-5, d
+    assert s.set_opts(qmx_ru_join=1.0) != 0
+    assert s.set_opts(qmx_ru_join=-0.1) != 0
