@@ -385,6 +385,12 @@ void QOPX_serve_destroy(QOPX_Server *s);
 
 /* Library status: 1 when a HIP device is usable. */
 int QOPX_device_ok(void);
+/* Page-locked host memory for raw arrays (vectors, links, gauge fields):
+ * create/extract/convert calls and request-server transfers on such buffers
+ * run at the full host-link rate (direct DMA) instead of the staged pageable
+ * rate.  NULL if the allocation fails; free with QOPX_host_free. */
+void *QOPX_host_alloc(unsigned long long bytes);
+void QOPX_host_free(void *p);
 /* Last error message (empty when none). */
 const char *QOPX_last_error(void);
 

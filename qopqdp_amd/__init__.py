@@ -203,6 +203,8 @@ SIGNATURES = [
     ("QOPX_serve_stats", None, [_VP, C.POINTER(C.c_double)]),
     ("QOPX_serve_destroy", None, [_VP]),
     ("QOPX_device_ok", C.c_int, []),
+    ("QOPX_host_alloc", C.c_void_p, [C.c_ulonglong]),
+    ("QOPX_host_free", None, [C.c_void_p]),
     ("QOPX_last_error", C.c_char_p, []),
 ]
 

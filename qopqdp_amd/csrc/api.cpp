@@ -758,6 +758,18 @@ void QOP_F3_asqtad_invert_multi(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *f, 
   if (info) { info->final_sec = sec; info->final_flop = flop; info->status = QOP_SUCCESS; }
 }
 
+void *QOPX_host_alloc(unsigned long long bytes) {
+  void *p = nullptr;
+  if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return p;
+}
+void QOPX_host_free(void *p) {
+  if (p) (void)hipHostFree(p);
+}
+
 int QOPX_device_ok(void) {
   int nd = 0;
   return hipGetDeviceCount(&nd) == hipSuccess && nd > 0;
