@@ -102,6 +102,13 @@ int or_hisq_W(const int L[4], const or_asqtad_coeffs_t *fat7, double *const U[4]
               const or_reunit_opts_t *o, double *flops);
 
 /* Number of iterations done by the last inner solver calls (diagnostics). */
+/* Asqtad fermion force, fnmat form (lib/asqtad_force_fnmat_p.c:474-546):
+ * force[mu] += -2 antiherm(U_mu deriv_mu^+), deriv the derivative of the fat
+ * and Naik links of sum_i eps_i x_i x_i^+ terms (force_oracle.c). */
+void or_asqtad_force_multi(const int L[4], double *const gauge[4], double *force[4],
+                           const or_asqtad_coeffs_t *coef, const double eps[],
+                           double *const x[], int nterms);
+
 int or_set_verbose(int v);
 int or_num_threads(void);
 

@@ -85,6 +85,8 @@ class Oracle:
             L.or_hisq_W.restype = C.c_int
             L.or_hisq_W.argtypes = [_IV, C.POINTER(Coeffs), _PP, _PP, C.POINTER(ReunitOpts),
                                     C.POINTER(C.c_double)]
+            L.or_asqtad_force_multi.argtypes = [_IV, _PP, _PP, C.POINTER(Coeffs),
+                                                C.POINTER(C.c_double), _PP, C.c_int]
             L.or_set_verbose.restype = C.c_int
             L.or_num_threads.restype = C.c_int
             Oracle._lib = L
@@ -116,6 +118,17 @@ class Oracle:
         self.lib.or_make_imp_links(_iv(L), C.byref(c), _pa(U), _pa(Ulong if Ulong else U),
                                    _pa(fat), _pa(lng) if lng else None)
         return fat, lng
+
+    def asqtad_force(self, L, U, coeffs, eps, xs, force=None):
+        """force[mu] += -2 antiherm(U deriv^+) for sum_i eps_i x_i x_i^+
+        (lib/asqtad_force_fnmat_p.c:515-546); xs are full-lattice vectors."""
+        V = int(np.prod(L))
+        if force is None:
+            force = [np.zeros((V, 3, 3, 2)) for _ in range(4)]
+        xs = [np.ascontiguousarray(x, dtype=np.float64) for x in xs]
+        self.lib.or_asqtad_force_multi(_iv(L), _pa(U), _pa(force), C.byref(Coeffs(**coeffs)),
+                                       (C.c_double * len(eps))(*eps), _pa(xs), len(xs))
+        return force
 
     def plaquette(self, L, U):
         return self.lib.or_plaquette(_iv(L), _pa(U))
