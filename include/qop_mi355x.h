@@ -170,6 +170,8 @@ typedef struct QOP_D3_FermionLinksAsqtad_struct QOP_D3_FermionLinksAsqtad;
 typedef struct QOP_F3_FermionLinksAsqtad_struct QOP_F3_FermionLinksAsqtad;
 typedef struct QOP_D3_FermionLinksHisq_struct QOP_D3_FermionLinksHisq;
 typedef struct QOP_F3_FermionLinksHisq_struct QOP_F3_FermionLinksHisq;
+typedef struct QOP_D3_Force_struct QOP_D3_Force;
+typedef struct QOP_F3_Force_struct QOP_F3_Force;
 
 /* ---- general routines (include/qop_int.h:230-251; lib/common.c) ------ */
 QOP_status_t QOP_init(QOP_layout_t *layout);         /* lib/common.c:40-83 */
@@ -209,6 +211,15 @@ void QOP_F3_destroy_G(QOP_F3_GaugeField *field);
 /* staggered signs and boundary phases in place (common_p.c:464-526) */
 void QOP_D3_rephase_G(QOP_D3_GaugeField *links, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign);
 void QOP_F3_rephase_G(QOP_F3_GaugeField *links, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign);
+
+/* Force fields: one colour matrix per site and direction, raw layout as the
+ * gauge field.  Sites outside evenodd are zero after create. */
+QOP_D3_Force *QOP_D3_create_F_from_raw(QOP_D_Real *force[], QOP_evenodd_t evenodd); /* common_p.c:80-96 */
+QOP_F3_Force *QOP_F3_create_F_from_raw(QOP_F_Real *force[], QOP_evenodd_t evenodd);
+void QOP_D3_extract_F_to_raw(QOP_D_Real *dest[], QOP_D3_Force *src, QOP_evenodd_t evenodd); /* common_p.c:123-130 */
+void QOP_F3_extract_F_to_raw(QOP_F_Real *dest[], QOP_F3_Force *src, QOP_evenodd_t evenodd);
+void QOP_D3_destroy_F(QOP_D3_Force *field); /* common_p.c:160-169 */
+void QOP_F3_destroy_F(QOP_F3_Force *field);
 
 /* ---- asqtad links (include/qop_pc.h:217-271; lib/asqtad_dslash_p.c) -- */
 QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_raw(QOP_D_Real *fatlinks[], QOP_D_Real *longlinks[], QOP_evenodd_t evenodd); /* :307-329 */
@@ -333,6 +344,29 @@ void QOPX_hop_shape_override(const int *shape);
  * -1 if the buffers do not fit (as QOPX_D3_asqtad_hop_time). */
 double QOPX_copy_time(unsigned long long bytes, int nrep);
 
+/* ---- asqtad fermion force (include/qop_pc.h:427-465; lib/asqtad_force_p.c) --
+ * force[mu] += -2 TA(U_mu deriv_mu^+) with deriv the derivative of
+ * sum_i eps[i] x_i x_i^+ through the fat7 + Lepage + Naik smearing
+ * (lib/asqtad_force_fnmat_p.c:474-546); in_pt[i] are full-lattice vectors,
+ * x_i on even sites and D_oe x_i on odd sites as the caller prepares them.
+ * gauge is the rephased gauge field the links were smeared from.  Every nsrc
+ * takes the fnmat form on the device (the reference's ASVEC form for
+ * nsrc < fnmat_src_min evaluates the same derivative).  FP64 arithmetic for
+ * both precisions.  info->status = QOP_MEM_ERROR if the workspace does not
+ * fit in device memory (force unchanged then). */
+QOP_status_t QOP_asqtad_force_set_opts(QOP_opt_t opts[], int nopts); /* lib/asqtad_force.c:22-43 */
+void QOP_D3_asqtad_force(QOP_info_t *info, QOP_D3_GaugeField *gauge, QOP_D3_Force *force,
+                         QOP_asqtad_coeffs_t *coeffs, QOP_D_Real eps,
+                         QOP_D3_ColorVector *in_pt); /* lib/asqtad_force_p.c:55-63 */
+void QOP_F3_asqtad_force(QOP_info_t *info, QOP_F3_GaugeField *gauge, QOP_F3_Force *force,
+                         QOP_asqtad_coeffs_t *coeffs, QOP_F_Real eps, QOP_F3_ColorVector *in_pt);
+void QOP_D3_asqtad_force_multi(QOP_info_t *info, QOP_D3_GaugeField *gauge, QOP_D3_Force *force,
+                               QOP_asqtad_coeffs_t *coef, QOP_D_Real eps[],
+                               QOP_D3_ColorVector *in_pt[], int nsrc); /* lib/asqtad_force_p.c:41-53 */
+void QOP_F3_asqtad_force_multi(QOP_info_t *info, QOP_F3_GaugeField *gauge, QOP_F3_Force *force,
+                               QOP_asqtad_coeffs_t *coef, QOP_F_Real eps[],
+                               QOP_F3_ColorVector *in_pt[], int nsrc);
+
 /* ---- request server: continuous batching over resident configurations ----
  * A server keeps fat/long links of any number of gauge configurations, of
  * any lattice sizes, resident on the device (QOP_init is not needed and its
@@ -443,6 +477,12 @@ const char *QOPX_last_error(void);
 #  define QOP_hisq_destroy_L QOP_D3_hisq_destroy_L
 #  define QOP_get_asqtad_links_from_hisq QOP_D3_get_asqtad_links_from_hisq
 #  define QOP_get_asqtad_deps_links_from_hisq QOP_D3_get_asqtad_deps_links_from_hisq
+#  define QOP_Force QOP_D3_Force
+#  define QOP_create_F_from_raw QOP_D3_create_F_from_raw
+#  define QOP_extract_F_to_raw QOP_D3_extract_F_to_raw
+#  define QOP_destroy_F QOP_D3_destroy_F
+#  define QOP_asqtad_force QOP_D3_asqtad_force
+#  define QOP_asqtad_force_multi QOP_D3_asqtad_force_multi
 #elif QOP_Precision == 'F'
 #  define QOP_Real QOP_F_Real
 #  define QOP_ColorVector QOP_F3_ColorVector
@@ -470,6 +510,12 @@ const char *QOPX_last_error(void);
 #  define QOP_hisq_destroy_L QOP_F3_hisq_destroy_L
 #  define QOP_get_asqtad_links_from_hisq QOP_F3_get_asqtad_links_from_hisq
 #  define QOP_get_asqtad_deps_links_from_hisq QOP_F3_get_asqtad_deps_links_from_hisq
+#  define QOP_Force QOP_F3_Force
+#  define QOP_create_F_from_raw QOP_F3_create_F_from_raw
+#  define QOP_extract_F_to_raw QOP_F3_extract_F_to_raw
+#  define QOP_destroy_F QOP_F3_destroy_F
+#  define QOP_asqtad_force QOP_F3_asqtad_force
+#  define QOP_asqtad_force_multi QOP_F3_asqtad_force_multi
 #else
 #  error "bad QOP_Precision"
 #endif

@@ -147,6 +147,19 @@ SIGNATURES = [
     ("QOP_F3_destroy_G", None, [_VP]),
     ("QOP_D3_rephase_G", None, [_VP, C.POINTER(C.c_int), C.POINTER(QOPBc), C.POINTER(QOPSign)]),
     ("QOP_F3_rephase_G", None, [_VP, C.POINTER(C.c_int), C.POINTER(QOPBc), C.POINTER(QOPSign)]),
+    ("QOP_D3_create_F_from_raw", _VP, [_PP, C.c_int]),
+    ("QOP_F3_create_F_from_raw", _VP, [_PP, C.c_int]),
+    ("QOP_D3_extract_F_to_raw", None, [_PP, _VP, C.c_int]),
+    ("QOP_F3_extract_F_to_raw", None, [_PP, _VP, C.c_int]),
+    ("QOP_D3_destroy_F", None, [_VP]),
+    ("QOP_F3_destroy_F", None, [_VP]),
+    ("QOP_asqtad_force_set_opts", C.c_int, [C.POINTER(QOPOpt), C.c_int]),
+    ("QOP_D3_asqtad_force", None, [C.POINTER(QOPInfo), _VP, _VP, C.POINTER(QOPAsqtadCoeffs), C.c_double, _VP]),
+    ("QOP_F3_asqtad_force", None, [C.POINTER(QOPInfo), _VP, _VP, C.POINTER(QOPAsqtadCoeffs), C.c_float, _VP]),
+    ("QOP_D3_asqtad_force_multi", None, [C.POINTER(QOPInfo), _VP, _VP, C.POINTER(QOPAsqtadCoeffs),
+                                         C.POINTER(C.c_double), _PP, C.c_int]),
+    ("QOP_F3_asqtad_force_multi", None, [C.POINTER(QOPInfo), _VP, _VP, C.POINTER(QOPAsqtadCoeffs),
+                                         C.POINTER(C.c_float), _PP, C.c_int]),
     ("QOP_D3_asqtad_create_L_from_raw", _VP, [_PP, _PP, C.c_int]),
     ("QOP_F3_asqtad_create_L_from_raw", _VP, [_PP, _PP, C.c_int]),
     ("QOP_D3_asqtad_create_L_from_G", _VP, [C.POINTER(QOPInfo), C.POINTER(QOPAsqtadCoeffs), _VP]),
@@ -328,9 +341,35 @@ class Session:
         f = self.lib.QOP_D3_asqtad_create_L_from_raw if prec == "D" else self.lib.QOP_F3_asqtad_create_L_from_raw
         return f(ptr_array(fat), ptr_array(lng) if lng is not None else None, QOP_EVENODD)
 
-    def gauge(self, U):
-        U = [np.ascontiguousarray(u, dtype=np.float64) for u in U]
-        return self.lib.QOP_D3_create_G_from_raw(ptr_array(U), QOP_EVENODD)
+    def gauge(self, U, prec="D"):
+        dt = np.float64 if prec == "D" else np.float32
+        U = [np.ascontiguousarray(u, dtype=dt) for u in U]
+        f = self.lib.QOP_D3_create_G_from_raw if prec == "D" else self.lib.QOP_F3_create_G_from_raw
+        return f(ptr_array(U), QOP_EVENODD)
+
+    def force(self, F, prec="D"):
+        """A QOP_Force from four raw (V, 3, 3, 2) arrays."""
+        dt = np.float64 if prec == "D" else np.float32
+        F = [np.ascontiguousarray(f, dtype=dt) for f in F]
+        f = self.lib.QOP_D3_create_F_from_raw if prec == "D" else self.lib.QOP_F3_create_F_from_raw
+        return f(ptr_array(F), QOP_EVENODD)
+
+    def get_force(self, f, prec="D"):
+        dt = np.float64 if prec == "D" else np.float32
+        out = [np.zeros((self.V, 3, 3, 2), dtype=dt) for _ in range(4)]
+        g = self.lib.QOP_D3_extract_F_to_raw if prec == "D" else self.lib.QOP_F3_extract_F_to_raw
+        g(ptr_array(out), f, QOP_EVENODD)
+        return out
+
+    def asqtad_force(self, g, f, eps, xs, coeffs=None, prec="D"):
+        """QOP_asqtad_force_multi(g, f, coeffs, eps, xs); returns the QOP_info_t."""
+        c = QOPAsqtadCoeffs(**(coeffs or MILC_ASQTAD))
+        info = QOPInfo()
+        n = len(xs)
+        ct = C.c_double if prec == "D" else C.c_float
+        fn = self.lib.QOP_D3_asqtad_force_multi if prec == "D" else self.lib.QOP_F3_asqtad_force_multi
+        fn(C.byref(info), g, f, C.byref(c), (ct * n)(*eps), (C.c_void_p * n)(*xs), n)
+        return info
 
     def links_from_gauge(self, g, coeffs=None):
         c = QOPAsqtadCoeffs(**(coeffs or MILC_ASQTAD))

@@ -415,6 +415,118 @@ void QOP_F3_rephase_G(QOP_F3_GaugeField *g, int *r0, QOP_bc_t *bc, QOP_staggered
   rephase_dev(g->lat, g->U, r0, bc, sign, 1, 1, nullptr, 0);
 }
 
+// ---- force fields (common_p.c:80-96, 123-130, 160-169) ------------------------
+extern "C++" {
+namespace {
+template <class FT, typename T>
+FT *create_force(T *raw[], int eo, const char *fn) {
+  auto *f = new FT;
+  f->lat = G.lat;
+  if (!guard_oom(fn, nullptr, [&] {
+        for (int i = 0; i < 4; i++) f->F[i] = upload_mat(G.lat, raw[i], eo);
+      })) {
+    for (int i = 0; i < 4; i++) if (f->F[i]) QOP_CHECK_HIP(hipFree(f->F[i]));
+    delete f;
+    return nullptr;
+  }
+  return f;
+}
+template <class FT>
+void destroy_force(FT *f) {
+  if (!f) return;
+  sync();
+  for (int i = 0; i < 4; i++) if (f->F[i]) QOP_CHECK_HIP(hipFree(f->F[i]));
+  delete f;
+}
+
+struct ForceOpts {
+  int fnmat_src_min = 2;  // lib/asqtad_force.c:4
+  int veclength = 4;
+} g_force_opts;
+
+// QOP_asqtad_force_multi (lib/asqtad_force_p.c:41-53 -> asqtad_force_fnmat_p.c:514-546)
+template <class GT, class FT, class VT, typename R>
+void force_multi(QOP_info_t *info, GT *gauge, FT *force, QOP_asqtad_coeffs_t *c, const R eps[],
+                 VT *in_pt[], int nsrc) {
+  if (nsrc <= 0) {
+    info->final_sec = 0;
+    info->final_flop = 0;
+    info->status = QOP_SUCCESS;
+    return;
+  }
+  const double coef[6] = {c->one_link, c->three_staple, c->five_staple, c->seven_staple, c->lepage, c->naik};
+  std::vector<double> e(eps, eps + nsrc);
+  std::vector<const DevVec *> xs(nsrc);
+  for (int i = 0; i < nsrc; i++) xs[i] = &in_pt[i]->v;
+  double flop = 0;
+  const double t0 = wall_time();
+  if (!guard_oom("QOP_asqtad_force_multi", info, [&] {
+        asqtad_force_dev(gauge->lat, gauge->U, force->F, coef, e.data(), xs.data(), nsrc, &flop, stream());
+      }))
+    return;
+  info->final_sec = wall_time() - t0;
+  info->final_flop = flop;
+  info->status = QOP_SUCCESS;
+}
+}  // namespace
+}  // extern "C++"
+
+QOP_D3_Force *QOP_D3_create_F_from_raw(QOP_D_Real *force[], QOP_evenodd_t eo) {
+  CHECK_INIT;
+  return create_force<QOP_D3_Force>(force, eo, __func__);
+}
+QOP_F3_Force *QOP_F3_create_F_from_raw(QOP_F_Real *force[], QOP_evenodd_t eo) {
+  CHECK_INIT;
+  return create_force<QOP_F3_Force>(force, eo, __func__);
+}
+void QOP_D3_extract_F_to_raw(QOP_D_Real *dest[], QOP_D3_Force *src, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  sync();
+  for (int i = 0; i < 4; i++) download_mat(dest[i], src->F[i], src->lat, eo);
+}
+void QOP_F3_extract_F_to_raw(QOP_F_Real *dest[], QOP_F3_Force *src, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  sync();
+  for (int i = 0; i < 4; i++) download_mat(dest[i], src->F[i], src->lat, eo);
+}
+void QOP_D3_destroy_F(QOP_D3_Force *f) { destroy_force(f); }
+void QOP_F3_destroy_F(QOP_F3_Force *f) { destroy_force(f); }
+
+// lib/asqtad_force.c:22-43: the tags are validated and kept; every nsrc runs
+// the fnmat form here.
+QOP_status_t QOP_asqtad_force_set_opts(QOP_opt_t opts[], int nopts) {
+  CHECK_INIT;
+  int fsm = g_force_opts.fnmat_src_min, vl = g_force_opts.veclength;
+  for (int i = 0; i < nopts; i++) {
+    if (!strcmp(opts[i].tag, "fnmat_src_min")) fsm = (int)opts[i].value;
+    if (!strcmp(opts[i].tag, "veclength")) vl = (int)opts[i].value;
+  }
+  if (fsm < 0 || vl <= 0) return QOP_FAIL;
+  g_force_opts.fnmat_src_min = fsm;
+  g_force_opts.veclength = vl;
+  return QOP_SUCCESS;
+}
+void QOP_D3_asqtad_force_multi(QOP_info_t *info, QOP_D3_GaugeField *gauge, QOP_D3_Force *force,
+                               QOP_asqtad_coeffs_t *coef, QOP_D_Real eps[], QOP_D3_ColorVector *in_pt[],
+                               int nsrc) {
+  CHECK_INIT;
+  force_multi(info, gauge, force, coef, eps, in_pt, nsrc);
+}
+void QOP_F3_asqtad_force_multi(QOP_info_t *info, QOP_F3_GaugeField *gauge, QOP_F3_Force *force,
+                               QOP_asqtad_coeffs_t *coef, QOP_F_Real eps[], QOP_F3_ColorVector *in_pt[],
+                               int nsrc) {
+  CHECK_INIT;
+  force_multi(info, gauge, force, coef, eps, in_pt, nsrc);
+}
+void QOP_D3_asqtad_force(QOP_info_t *info, QOP_D3_GaugeField *gauge, QOP_D3_Force *force,
+                         QOP_asqtad_coeffs_t *coeffs, QOP_D_Real eps, QOP_D3_ColorVector *in_pt) {
+  QOP_D3_asqtad_force_multi(info, gauge, force, coeffs, &eps, &in_pt, 1);
+}
+void QOP_F3_asqtad_force(QOP_info_t *info, QOP_F3_GaugeField *gauge, QOP_F3_Force *force,
+                         QOP_asqtad_coeffs_t *coeffs, QOP_F_Real eps, QOP_F3_ColorVector *in_pt) {
+  QOP_F3_asqtad_force_multi(info, gauge, force, coeffs, &eps, &in_pt, 1);
+}
+
 // ---- links --------------------------------------------------------------------
 QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_raw(QOP_D_Real *fat[], QOP_D_Real *lng[], QOP_evenodd_t eo) {
   CHECK_INIT;

@@ -177,6 +177,14 @@ void launch_fatten(const Lattice *lat, const double *gf[4], const double *gl[4],
                    const double coef[6], double *fat[4], double *lng[4],
                    hipStream_t st);
 
+// Asqtad fermion force (force.hip): F[mu] (raw FP64, device) += -2 TA(U_mu
+// deriv_mu^+) for sum_t eps[t] x_t x_t^+ (full-lattice FP64 or FP32 vectors);
+// coef = {one_link, three_staple, five_staple, seven_staple, lepage, naik};
+// flops (optional) receives the reference's flop count.
+void asqtad_force_dev(const Lattice *lat, double *const U_raw[4], double *F_raw[4],
+                      const double coef[6], const double *eps, const DevVec *const *xs, int n,
+                      double *flops, hipStream_t st);
+
 // raw <-> SoA (kernels.hip); eo selects the sites converted.
 void launch_raw_to_vec(const Lattice *lat, const double *raw_dev, DevVec &v,
                        int eo, hipStream_t st);

@@ -23,6 +23,15 @@ struct QOP_F3_GaugeField_struct {
   qop::Lattice *lat = nullptr;
   double *U[4] = {nullptr, nullptr, nullptr, nullptr};
 };
+// Force fields: device FP64, raw layout, one array per direction.
+struct QOP_D3_Force_struct {
+  qop::Lattice *lat = nullptr;
+  double *F[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+struct QOP_F3_Force_struct {
+  qop::Lattice *lat = nullptr;
+  double *F[4] = {nullptr, nullptr, nullptr, nullptr};
+};
 struct QOP_D3_FermionLinksAsqtad_struct {
   qop::Links L;
 };
