@@ -22,6 +22,10 @@ FETCH_SIZE / WRITE_SIZE (profiles/traffic.json).
 cpu_baseline: the C oracle's FP64 CG (the reference algorithm) on the host
 cores for a bounded number of iterations, extrapolated to the GPU FP64
 solve's iteration count.
+
+Other modes print their own JSON line: --multi (multi-mass), --hisq (HISQ
+multi-mass at 48^3x96), --serve (request server latency), --force (asqtad
+fermion force, see force_bench).
 """
 import argparse
 import ctypes as C
@@ -77,6 +81,9 @@ def main():
                     help="batched multi-mass solves (16 sources x 8 masses, 24^3x48) instead")
     ap.add_argument("--hisq", action="store_true",
                     help="HISQ links, one source x 12 masses at 48^3x96 instead")
+    ap.add_argument("--force", action="store_true",
+                    help="asqtad fermion force (fnmat form) at --lattice instead of the solver bench")
+    ap.add_argument("--force-terms", type=int, default=8, help="vectors x_i of the force")
     ap.add_argument("--fp64-sources", type=int, default=16,
                     help="--multi: sequential FP64 multi-mass solves timed for the baseline")
     args = ap.parse_args()
@@ -89,6 +96,9 @@ def main():
         return
     if args.multi or args.hisq:
         multi_bench(args)
+        return
+    if args.force:
+        force_bench(args)
         return
 
     import qopqdp_amd as q
@@ -440,6 +450,82 @@ def multi_bench(args):
                             "converged": int(fp_ok)},
         "speedup_vs_fp64": round(rate / fp_rate, 2),
     }), flush=True)
+
+
+def force_bench(args):
+    """Asqtad fermion force, QOP_D3_asqtad_force_multi (fnmat form): one step
+    is one force of --force-terms full-lattice vectors (Gaussian, seeded) on a
+    seeded SU(3) gauge field at --lattice (default 32^3x64) with the MILC
+    coefficients, accumulated into a device force field.  cpu_baseline: the C
+    oracle's force of the same number of terms on a 16^3x32 sample (1/16 of
+    the volume at the default lattice), scaled by the volume ratio (every
+    operation of the force is per site)."""
+    import qopqdp_amd as q
+    L = [int(x) for x in args.lattice.split(",")]
+    V = int(np.prod(L))
+    n = args.force_terms
+    s = q.Session(L)
+    U = q.gen_gauge(L, 2000, 0.3)
+    g = s.gauge(U)
+    del U
+    f = s.force([np.zeros((V, 3, 3, 2)) for _ in range(4)])
+    xs = [s.vec(q.gen_source(L, 2100 + i, q.QOP_EVENODD)) for i in range(n)]
+    eps = [1.0 / (i + 1) for i in range(n)]
+    for _ in range(args.warmup):
+        info = s.asqtad_force(g, f, eps, xs)
+        assert info.status == 0
+    t = time.perf_counter()
+    secs = []
+    for _ in range(args.steps):
+        info = s.asqtad_force(g, f, eps, xs)
+        assert info.status == 0
+        secs.append(info.final_sec)
+    dt = time.perf_counter() - t
+    flop = info.final_flop
+    F = s.get_force(f)
+    a = F[0][::101, :, :, 0] + 1j * F[0][::101, :, :, 1]
+    antiherm = float(np.abs(a + np.conj(np.transpose(a, (0, 2, 1)))).max() / np.abs(a).max())
+    for x in xs:
+        s.free(x)
+    s.lib.QOP_D3_destroy_F(f)
+    s.lib.QOP_D3_destroy_G(g)
+    s.close()
+    out = {
+        "metric": "asqtad fermion force (fnmat form) per second",
+        "value": round(args.steps / dt, 3), "unit": "forces/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True,
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"asqtad force {'x'.join(map(str, L))}, {n} terms, MILC coefficients",
+                   "lattice": L, "terms": n},
+        "gflops_reference_count": round(flop / float(np.median(secs)) / 1e9, 1),
+        "antihermiticity_check": antiherm,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = force_cpu_baseline(L, n, eps)
+    print(json.dumps(out), flush=True)
+
+
+def force_cpu_baseline(L, n, eps):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _oracle import Oracle
+    import qopqdp_amd as q
+    orc = Oracle()
+    Ls = [max(2, x // 2) for x in L]
+    Vs = int(np.prod(Ls))
+    rng = np.random.default_rng(3)
+    U = q.gen_gauge(Ls, 2200, 0.3)
+    xs = [rng.standard_normal((Vs, 3, 2)) for _ in range(n)]
+    coeffs = dict(one_link=5.0 / 8, three_staple=-1.0 / 16, five_staple=1.0 / 64,
+                  seven_staple=-1.0 / 384, lepage=-1.0 / 16, naik=-1.0 / 24)
+    t = time.perf_counter()
+    orc.asqtad_force(Ls, U, coeffs, eps, xs)
+    dt = time.perf_counter() - t
+    scale = int(np.prod(L)) / Vs
+    log(f"cpu force baseline: {dt:.2f} s at {Ls} on {orc.threads()} threads")
+    return {"value": round(1.0 / (dt * scale), 6), "unit": "forces/s", "cores": orc.threads(),
+            "kind": "port",
+            "sample": f"one force of {n} terms at {'x'.join(map(str, Ls))} ({dt:.2f} s on "
+                      f"{orc.threads()} OpenMP threads), scaled by the volume ratio {scale:g}"}
 
 
 def serve_bench(args):

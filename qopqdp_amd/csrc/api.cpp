@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <stdexcept>
 #include <string>
 
 #include "handles.h"
@@ -460,10 +461,16 @@ void force_multi(QOP_info_t *info, GT *gauge, FT *force, QOP_asqtad_coeffs_t *c,
   for (int i = 0; i < nsrc; i++) xs[i] = &in_pt[i]->v;
   double flop = 0;
   const double t0 = wall_time();
-  if (!guard_oom("QOP_asqtad_force_multi", info, [&] {
-        asqtad_force_dev(gauge->lat, gauge->U, force->F, coef, e.data(), xs.data(), nsrc, &flop, stream());
-      }))
+  try {
+    if (!guard_oom("QOP_asqtad_force_multi", info, [&] {
+          asqtad_force_dev(gauge->lat, gauge->U, force->F, coef, e.data(), xs.data(), nsrc, &flop, stream());
+        }))
+      return;
+  } catch (const std::runtime_error &err) {
+    fprintf(stderr, "QOP-MI355X: %s\n", err.what());
+    info->status = QOP_FAIL;
     return;
+  }
   info->final_sec = wall_time() - t0;
   info->final_flop = flop;
   info->status = QOP_SUCCESS;

@@ -20,12 +20,34 @@
 #include <stdexcept>
 
 #include "common.h"
-#include "geom_d.h"
 
 namespace qop {
 namespace {
 
 constexpr int TPB = 256;
+
+// Site order of the gathering kernels (x0-tiled, XCD-aware).  A slab is the
+// S = Vh / L0 sites of one x0 and parity; it is cut into tiles of T sites.
+// Logical blocks run tile by tile, and inside a tile over x0 and the two
+// parities, so the x0 +-1 neighbours a block gathers belong to blocks about
+// 2 T / TPB positions away, still in L2, and the x1..x3 neighbours mostly to
+// the same or the partner block.  The hardware dispatches block b to XCD
+// b % 8; XCD k runs the logical blocks [k per, (k+1) per), a contiguous part
+// of that order.  -1 marks padding threads.
+struct Sched {
+  int S, T, nt, L0, bpt, per;  // slab, tile, tiles per slab, extent 0, blocks per tile, blocks per XCD
+};
+__device__ __forceinline__ int site_index(const Sched &q, int Vh) {
+  const int w = (int)(blockIdx.x & 7) * q.per + (int)(blockIdx.x >> 3);
+  const int sub = w % q.bpt;
+  int r = w / q.bpt;
+  const int par = r & 1;
+  r >>= 1;
+  const int x0 = r % q.L0, tile = r / q.L0;
+  const int hs = tile * q.T + sub * TPB + (int)threadIdx.x;
+  if (tile >= q.nt || hs >= q.S) return -1;
+  return par * Vh + x0 * q.S + hs;
+}
 
 struct M3 {
   double2 e[9];
@@ -74,25 +96,61 @@ __device__ __forceinline__ M3 zero3() {
   return m;
 }
 
-__device__ __forceinline__ M3 ld(const double2 *f, long V, long s) {
+// Element planes f + e V are uniform; the per-lane part of the address is a
+// 32-bit byte offset (the volume is below 2^27 sites), so the loads and
+// stores take a scalar base plus one vector offset.
+__device__ __forceinline__ M3 ld(const double2 *f, long V, unsigned s) {
   M3 m;
+  const unsigned off = s * (unsigned)sizeof(double2);
 #pragma unroll
-  for (int e = 0; e < 9; e++) m.e[e] = f[e * V + s];
+  for (int e = 0; e < 9; e++) m.e[e] = *(const double2 *)((const char *)(f + e * V) + off);
   return m;
 }
-__device__ __forceinline__ void st(double2 *f, long V, long s, const M3 &m) {
+__device__ __forceinline__ void st(double2 *f, long V, unsigned s, const M3 &m) {
+  const unsigned off = s * (unsigned)sizeof(double2);
 #pragma unroll
-  for (int e = 0; e < 9; e++) f[e * V + s] = m.e[e];
+  for (int e = 0; e < 9; e++) *(double2 *)((char *)(f + e * V) + off) = m.e[e];
 }
 
-// x + a mu + b nu (mu != nu or b == 0), periodic
-__device__ __forceinline__ long nbr(const Geom &g, const int x[4], int mu, int a, int nu, int b) {
-  int y[4] = {x[0], x[1], x[2], x[3]};
-  y[mu] += a;
-  y[nu] += b;
-#pragma unroll
-  for (int d = 0; d < 4; d++) y[d] = (y[d] % g.L[d] + g.L[d]) % g.L[d];
-  return raw_index_d(g, y);
+// A site's coordinates and lexicographic index (x3 fastest; 32-bit, the
+// volume is below 2^31), found once per thread; neighbours then cost a few
+// integer adds instead of divisions.
+struct Site {
+  int x[4];
+  int l;
+};
+__device__ __forceinline__ Site site_of(const Geom &g, long s) {
+  Site r;
+  const int p = s >= g.Vh;
+  unsigned t = 2u * (unsigned)(s - (p ? g.Vh : 0));
+  const unsigned l0 = t;
+  r.x[3] = (int)(t % (unsigned)g.L[3]); t /= (unsigned)g.L[3];
+  r.x[2] = (int)(t % (unsigned)g.L[2]); t /= (unsigned)g.L[2];
+  r.x[1] = (int)(t % (unsigned)g.L[1]); t /= (unsigned)g.L[1];
+  r.x[0] = (int)t;
+  r.l = (int)l0;
+  if (((r.x[0] + r.x[1] + r.x[2] + r.x[3]) & 1) != p) {
+    r.x[3] += 1;  // L3 is even: x3 + 1 stays inside
+    r.l += 1;
+  }
+  return r;
+}
+
+// raw index of x + a mu + b nu (mu != nu or b == 0, |a|, |b| <= 3), periodic.
+// The extents are even (at least 2), so two conditional wraps suffice and
+// the parity of the neighbour is that of x + a + b.
+__device__ __forceinline__ int wrap(int y, int L) {
+  y = y >= L ? y - L : y;
+  y = y >= L ? y - L : y;
+  y = y < 0 ? y + L : y;
+  y = y < 0 ? y + L : y;
+  return y;
+}
+__device__ __forceinline__ unsigned nbr(const Geom &g, const Site &x, int mu, int a, int nu, int b) {
+  int l = x.l + (wrap(x.x[mu] + a, g.L[mu]) - x.x[mu]) * g.stride[mu];
+  if (b) l += (wrap(x.x[nu] + b, g.L[nu]) - x.x[nu]) * g.stride[nu];
+  const int p = (x.x[0] + x.x[1] + x.x[2] + x.x[3] + a + b) & 1;
+  return (unsigned)(l >> 1) + (p ? (unsigned)g.Vh : 0u);
 }
 
 // A field given as ca A + B (B may be null): the bottom input of side_force,
@@ -100,21 +158,22 @@ __device__ __forceinline__ long nbr(const Geom &g, const int x[4], int mu, int a
 struct Lin {
   const double2 *a;
   double ca;
-  const double2 *b;
+  const double2 *b;  // used when the kernel is instantiated with HASB
 };
-__device__ __forceinline__ M3 ld(const Lin &f, long V, long s) {
+template <bool HASB>
+__device__ __forceinline__ M3 ld(const Lin &f, long V, unsigned s) {
   M3 m = ld(f.a, V, s);
 #pragma unroll
   for (int e = 0; e < 9; e++) {
     m.e[e].x *= f.ca;
     m.e[e].y *= f.ca;
   }
-  if (f.b) {
+  if constexpr (HASB) {
+    const M3 t = ld(f.b, V, s);
 #pragma unroll
     for (int e = 0; e < 9; e++) {
-      const double2 t = f.b[e * V + s];
-      m.e[e].x += t.x;
-      m.e[e].y += t.y;
+      m.e[e].x += t.e[e].x;
+      m.e[e].y += t.e[e].y;
     }
   }
   return m;
@@ -155,26 +214,26 @@ __global__ __launch_bounds__(TPB) void lincomb_kernel(double2 *o, double co, dou
 // x_t(x) x_t(x + dist_s)^+, dist_s = s-th direction (s < 4) or three times
 // it (s >= 4).  Vectors are full-lattice [parity][colour][site] complex.
 template <typename T2>
-__global__ __launch_bounds__(TPB) void mid_kernel(Geom g, long V, const T2 *const *xs, const double *eps, int n, int nmid,
+__global__ __launch_bounds__(TPB) void mid_kernel(Geom g, Sched q, long V, const T2 *const *xs, const double *eps, int n, int nmid,
                            double2 *mid) {
-  const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (s >= V) return;
-  int x[4];
-  raw_coords_d(g, s, x);
+  const int s0 = site_index(q, g.Vh);
+  if (s0 < 0) return;
+  const unsigned s = (unsigned)s0;
+  const Site x = site_of(g, s);
   const long Vh = g.Vh;
   const double sg = s >= Vh ? 1.0 : -1.0;
   auto vidx = [&](long i, int c) { return i + (i >= Vh ? 2 * Vh : 0) + c * Vh; };
   for (int m = 0; m < nmid; m++) {
-    const long j = nbr(g, x, m & 3, m < 4 ? 1 : 3, 0, 0);
+    const unsigned j = nbr(g, x, m & 3, m < 4 ? 1 : 3, 0, 0);
     M3 acc = zero3();
     for (int t = 0; t < n; t++) {
       const T2 *v = xs[t];
       double2 a[3], b[3];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const T2 p = v[vidx(s, c)], q = v[vidx(j, c)];
-        a[c] = make_double2(p.x, p.y);
-        b[c] = make_double2(q.x, q.y);
+        const T2 va = v[vidx(s, c)], vb = v[vidx(j, c)];
+        a[c] = make_double2(va.x, va.y);
+        b[c] = make_double2(vb.x, vb.y);
       }
       const double w = sg * eps[t];
 #pragma unroll
@@ -195,15 +254,16 @@ __global__ __launch_bounds__(TPB) void mid_kernel(Geom g, long V, const T2 *cons
 // out(x) = [out(x) if acc] + sc [K(x) I(x+nu) K(x+mu)^+ + K(x-nu)^+ I(x-nu) K(x-nu+mu)]
 //          + te E(x)
 // I is the field along mu, K the link along nu.
-__global__ __launch_bounds__(TPB) void staple_kernel(Geom g, long V, double2 *out, int acc, double sc, const double2 *I,
+template <bool ACC, bool HASE>
+__global__ __launch_bounds__(TPB) void staple_kernel(Geom g, Sched q, long V, double2 *out, double sc, const double2 *I,
                               const double2 *K, int mu, int nu, double te, const double2 *E) {
-  const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (s >= V) return;
-  int x[4];
-  raw_coords_d(g, s, x);
-  const long xpnu = nbr(g, x, nu, 1, mu, 0), xpmu = nbr(g, x, mu, 1, nu, 0);
-  const long xmnu = nbr(g, x, nu, -1, mu, 0), xmnupmu = nbr(g, x, nu, -1, mu, 1);
-  M3 r = acc ? ld(out, V, s) : zero3();
+  const int s0 = site_index(q, g.Vh);
+  if (s0 < 0) return;
+  const unsigned s = (unsigned)s0;
+  const Site x = site_of(g, s);
+  const unsigned xpnu = nbr(g, x, nu, 1, mu, 0), xpmu = nbr(g, x, mu, 1, nu, 0);
+  const unsigned xmnu = nbr(g, x, nu, -1, mu, 0), xmnupmu = nbr(g, x, nu, -1, mu, 1);
+  M3 r = ACC ? ld(out, V, s) : zero3();
   {
     const M3 t = mm<false, true>(ld(I, V, xpnu), ld(K, V, xpmu));
     axpy<false>(r, sc, mm<false, false>(ld(K, V, s), t));
@@ -212,7 +272,7 @@ __global__ __launch_bounds__(TPB) void staple_kernel(Geom g, long V, double2 *ou
     const M3 t = mm<true, false>(ld(K, V, xmnu), ld(I, V, xmnu));
     axpy<false>(r, sc, mm<false, false>(t, ld(K, V, xmnupmu)));
   }
-  if (E) axpy<false>(r, te, ld(E, V, s));
+  if constexpr (HASE) axpy<false>(r, te, ld(E, V, s));
   st(out, V, s, r);
 }
 
@@ -221,43 +281,56 @@ __global__ __launch_bounds__(TPB) void staple_kernel(Geom g, long V, double2 *ou
 //           + B(x-mu)^+ S(x-mu) T(x-mu+nu) + T(x-mu)^+ S(x-mu) B(x-mu+nu)
 // stpl(x) = [stpl(x) if acc] + S(x) B(x+nu) S(x+mu)^+ + S(x-nu)^+ B(x-nu) S(x-nu+mu)
 // with B = bot, S = side (the link along nu), T = top.
-__global__ __launch_bounds__(TPB) void side_force_kernel(Geom g, long V, double2 *force, double2 *stpl, int acc, Lin bot,
-                                  const double2 *side, const double2 *top, int mu, int nu) {
-  const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (s >= V) return;
-  int x[4];
-  raw_coords_d(g, s, x);
-  const long xpmu = nbr(g, x, mu, 1, nu, 0), xpnu = nbr(g, x, nu, 1, mu, 0);
-  const long xmmu = nbr(g, x, mu, -1, nu, 0), xmmupnu = nbr(g, x, mu, -1, nu, 1);
-  const long xmnu = nbr(g, x, nu, -1, mu, 0), xmnupmu = nbr(g, x, nu, -1, mu, 1);
-  // The phases are kept apart (sched_barrier) so that the scheduler does not
-  // hoist all 14 matrix loads at once: that would need ~500 VGPRs.
+template <bool ACC, bool HASB>
+__global__ __launch_bounds__(TPB) void side_force_kernel(Geom g, Sched q, long V, double2 *force, double2 *stpl,
+                                                         Lin bot, const double2 *side,
+                                                         const double2 *top, int mu, int nu) {
+  const int s0 = site_index(q, g.Vh);
+  if (s0 < 0) return;
+  const unsigned s = (unsigned)s0;
+  const Site x = site_of(g, s);
+  const unsigned xpmu = nbr(g, x, mu, 1, nu, 0), xpnu = nbr(g, x, nu, 1, mu, 0);
+  // SB() keeps the scheduler from hoisting every matrix load to the top:
+  // with all 14 in flight a thread needs ~500 registers (one wave per SIMD).
+#define SB() __builtin_amdgcn_sched_barrier(0)
   {
-    M3 sp = mm<false, false>(ld(side, V, s), mm<false, true>(ld(bot, V, xpnu), ld(side, V, xpmu)));
-    const M3 t = mm<true, false>(ld(side, V, xmnu), ld(bot, V, xmnu));
+    const unsigned xmnu = nbr(g, x, nu, -1, mu, 0), xmnupmu = nbr(g, x, nu, -1, mu, 1);
+    const M3 bs = mm<false, true>(ld<HASB>(bot, V, xpnu), ld(side, V, xpmu));  // B(x+nu) S(x+mu)^+
+    SB();
+    M3 sp = mm<false, false>(ld(side, V, s), bs);
+    SB();
+    const M3 t = mm<true, false>(ld(side, V, xmnu), ld<HASB>(bot, V, xmnu));
+    SB();
     axpy<false>(sp, 1.0, mm<false, false>(t, ld(side, V, xmnupmu)));
-    if (acc) axpy<false>(sp, 1.0, ld(stpl, V, s));
+    if constexpr (ACC) axpy<false>(sp, 1.0, ld(stpl, V, s));
     st(stpl, V, s, sp);
   }
-  __builtin_amdgcn_sched_barrier(0);
+  SB();
+  const unsigned xmmu = nbr(g, x, mu, -1, nu, 0), xmmupnu = nbr(g, x, mu, -1, nu, 1);
   M3 f;
   {
     const M3 Smu = ld(side, V, xpmu);
-    f = mm<false, true>(ld(top, V, s), mm<false, true>(ld(bot, V, xpnu), Smu));  // T S(x+mu) B(x+nu)^+
-    __builtin_amdgcn_sched_barrier(0);
-    const M3 b2 = mm<false, false>(ld(bot, V, s), Smu);  // B(x) S(x+mu)
+    const M3 bs = mm<false, true>(ld<HASB>(bot, V, xpnu), Smu);
+    SB();
+    f = mm<false, true>(ld(top, V, s), bs);  // T S(x+mu) B(x+nu)^+
+    SB();
+    const M3 b2 = mm<false, false>(ld<HASB>(bot, V, s), Smu);  // B(x) S(x+mu)
+    SB();
     axpy<false>(f, 1.0, mm<false, true>(b2, ld(top, V, xpnu)));
   }
-  __builtin_amdgcn_sched_barrier(0);
+  SB();
   {
     const M3 Sm = ld(side, V, xmmu);
-    const M3 t1 = mm<true, false>(ld(bot, V, xmmu), Sm);  // B(x-mu)^+ S(x-mu)
+    const M3 t1 = mm<true, false>(ld<HASB>(bot, V, xmmu), Sm);  // B(x-mu)^+ S(x-mu)
+    SB();
     axpy<false>(f, 1.0, mm<false, false>(t1, ld(top, V, xmmupnu)));
-    __builtin_amdgcn_sched_barrier(0);
+    SB();
     const M3 t2 = mm<true, false>(ld(top, V, xmmu), Sm);  // T(x-mu)^+ S(x-mu)
-    axpy<false>(f, 1.0, mm<false, false>(t2, ld(bot, V, xmmupnu)));
+    SB();
+    axpy<false>(f, 1.0, mm<false, false>(t2, ld<HASB>(bot, V, xmmupnu)));
   }
-  __builtin_amdgcn_sched_barrier(0);
+  SB();
+#undef SB
   axpy<false>(f, 1.0, ld(force, V, s));
   st(force, V, s, f);
 }
@@ -281,14 +354,14 @@ __global__ __launch_bounds__(TPB) void mix_kernel(double2 *mid5, double cl, doub
 // Naik part of QOP_asqtad_deriv (lib/asqtad_force_fnmat_p.c:395-442):
 // D(x) += cn [m(x) U(x+2mu)^+ U(x+mu)^+ + U(x-mu)^+ m(x-mu) U(x+mu)^+
 //             + U(x-mu)^+ U(x-2mu)^+ m(x-2mu)]
-__global__ __launch_bounds__(TPB) void naik_deriv_kernel(Geom g, long V, double2 *D, const double2 *U, const double2 *m,
+__global__ __launch_bounds__(TPB) void naik_deriv_kernel(Geom g, Sched q, long V, double2 *D, const double2 *U, const double2 *m,
                                   double cn, int mu) {
-  const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (s >= V) return;
-  int x[4];
-  raw_coords_d(g, s, x);
-  const long p1 = nbr(g, x, mu, 1, 0, 0), p2 = nbr(g, x, mu, 2, 0, 0);
-  const long m1 = nbr(g, x, mu, -1, 0, 0), m2 = nbr(g, x, mu, -2, 0, 0);
+  const int s0 = site_index(q, g.Vh);
+  if (s0 < 0) return;
+  const unsigned s = (unsigned)s0;
+  const Site x = site_of(g, s);
+  const unsigned p1 = nbr(g, x, mu, 1, 0, 0), p2 = nbr(g, x, mu, 2, 0, 0);
+  const unsigned m1 = nbr(g, x, mu, -1, 0, 0), m2 = nbr(g, x, mu, -2, 0, 0);
   const M3 Up1 = ld(U, V, p1), Um1 = ld(U, V, m1);
   M3 f = mm<false, true>(ld(m, V, s), mm<false, false>(Up1, ld(U, V, p2)));  // m [U(x+mu) U(x+2mu)]^+
   const M3 a = mm<true, false>(Um1, ld(m, V, m1));
@@ -302,16 +375,16 @@ __global__ __launch_bounds__(TPB) void naik_deriv_kernel(Geom g, long V, double2
 
 // Lepage fix-up (lib/asqtad_force_fnmat_p.c:317-339), first half:
 // acc(x) = sum_{nu != mu} [q_nu(x) + q_nu(x)^+], q_nu(x) = m_nu(x) U_nu(x)^+ - m_nu(x-nu)^+ U_nu(x-nu)
-__global__ __launch_bounds__(TPB) void lepage_acc_kernel(Geom g, long V, double2 *acc, const double2 *const *U,
+__global__ __launch_bounds__(TPB) void lepage_acc_kernel(Geom g, Sched q, long V, double2 *acc, const double2 *const *U,
                                   const double2 *const *m, int mu) {
-  const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (s >= V) return;
-  int x[4];
-  raw_coords_d(g, s, x);
+  const int s0 = site_index(q, g.Vh);
+  if (s0 < 0) return;
+  const unsigned s = (unsigned)s0;
+  const Site x = site_of(g, s);
   M3 r = zero3();
   for (int nu = 0; nu < 4; nu++) {
     if (nu == mu) continue;
-    const long b = nbr(g, x, nu, -1, 0, 0);
+    const unsigned b = nbr(g, x, nu, -1, 0, 0);
     M3 q = mm<false, true>(ld(m[nu], V, s), ld(U[nu], V, s));
     axpy<false>(q, -1.0, mm<true, false>(ld(m[nu], V, b), ld(U[nu], V, b)));
     axpy<false>(r, 1.0, q);
@@ -321,12 +394,12 @@ __global__ __launch_bounds__(TPB) void lepage_acc_kernel(Geom g, long V, double2
 }
 
 // second half: D(x) += cf [acc(x) U(x) - U(x) acc(x+mu)]
-__global__ __launch_bounds__(TPB) void lepage_apply_kernel(Geom g, long V, double2 *D, const double2 *U,
+__global__ __launch_bounds__(TPB) void lepage_apply_kernel(Geom g, Sched q, long V, double2 *D, const double2 *U,
                                     const double2 *acc, double cf, int mu) {
-  const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (s >= V) return;
-  int x[4];
-  raw_coords_d(g, s, x);
+  const int s0 = site_index(q, g.Vh);
+  if (s0 < 0) return;
+  const unsigned s = (unsigned)s0;
+  const Site x = site_of(g, s);
   const M3 u = ld(U, V, s);
   M3 t = mm<false, false>(ld(acc, V, s), u);
   axpy<false>(t, -1.0, mm<false, false>(u, ld(acc, V, nbr(g, x, mu, 1, 0, 0))));
@@ -363,6 +436,19 @@ __global__ __launch_bounds__(TPB) void force_kernel(long V, double2 *F, const do
 }
 
 inline dim3 grid(long n) { return dim3((unsigned)((n + TPB - 1) / TPB)); }
+// The schedule for a lattice: tiles of up to 4096 sites (whole blocks).
+inline Sched make_sched(const Lattice *lat) {
+  Sched q;
+  q.L0 = lat->L[0];
+  q.S = (int)(lat->Vh / q.L0);
+  const int Sb = (q.S + TPB - 1) / TPB * TPB;
+  q.T = Sb < 4096 ? Sb : 4096;
+  q.nt = (q.S + q.T - 1) / q.T;
+  q.bpt = q.T / TPB;
+  const long W = (long)q.nt * q.L0 * 2 * q.bpt;
+  q.per = (int)((W + 7) / 8);
+  return q;
+}
 
 }  // namespace
 
@@ -381,6 +467,8 @@ void asqtad_force_dev(const Lattice *lat, double *const U_raw[4], double *F_raw[
   coef1 -= 6 * coefL;
   const bool have5 = coef5 != 0 || coef7 != 0 || coefL != 0;
   const bool have3 = coef3 != 0 || have5;
+  if (V >= (1L << 27))
+    throw std::runtime_error("asqtad force: lattice volume of 2^27 sites or more (32-bit offsets)");
   const int prec = xs[0]->prec;
   for (int t = 0; t < n; t++)
     if (xs[t]->prec != prec || xs[t]->lat != lat || (prec != P64 && prec != P32))
@@ -423,27 +511,42 @@ void asqtad_force_dev(const Lattice *lat, double *const U_raw[4], double *F_raw[
   const double2 *const *d_mid = (const double2 *const *)(tab.p + 2 * n + 4);
 
   double nfl = 0;  // per site
-  const dim3 b(TPB), gs = grid(V), ge = grid(n9);
+  const Sched sq = make_sched(lat);
+  const dim3 b(TPB), gs = grid(V), ge = grid(n9), gx(8 * sq.per);
   for (int mu = 0; mu < 4; mu++) {
     hipLaunchKernelGGL(to_soa_kernel, gs, b, 0, st, (const double2 *)U_raw[mu], U[mu], V);
     hipLaunchKernelGGL(lincomb_kernel, ge, b, 0, st, D[mu], 0.0, 0.0, (const double2 *)nullptr, 0.0,
                        (const double2 *)nullptr, n9);
   }
   if (prec == P64)
-    hipLaunchKernelGGL(mid_kernel<double2>, gs, b, 0, st, g, V, (const double2 *const *)tab.p, d_eps, n,
+    hipLaunchKernelGGL(mid_kernel<double2>, gx, b, 0, st, g, sq, V, (const double2 *const *)tab.p, d_eps, n,
                        nmid, mid[0]);
   else
-    hipLaunchKernelGGL(mid_kernel<float2>, gs, b, 0, st, g, V, (const float2 *const *)tab.p, d_eps, n,
+    hipLaunchKernelGGL(mid_kernel<float2>, gx, b, 0, st, g, sq, V, (const float2 *const *)tab.p, d_eps, n,
                        nmid, mid[0]);
   nfl += (54 + 36) * 4.0 * n * (naik ? 2 : 1);
 
   auto staple = [&](double2 *out, int acc, double sc, const double2 *I, const double2 *K, int mu,
                     int nu, double te, const double2 *E) {
-    hipLaunchKernelGGL(staple_kernel, gs, b, 0, st, g, V, out, acc, sc, I, K, mu, nu, te, E);
+    if (acc && E)
+      hipLaunchKernelGGL((staple_kernel<true, true>), gx, b, 0, st, g, sq, V, out, sc, I, K, mu, nu, te, E);
+    else if (acc)
+      hipLaunchKernelGGL((staple_kernel<true, false>), gx, b, 0, st, g, sq, V, out, sc, I, K, mu, nu, te, E);
+    else if (E)
+      hipLaunchKernelGGL((staple_kernel<false, true>), gx, b, 0, st, g, sq, V, out, sc, I, K, mu, nu, te, E);
+    else
+      hipLaunchKernelGGL((staple_kernel<false, false>), gx, b, 0, st, g, sq, V, out, sc, I, K, mu, nu, te, E);
   };
   auto side_force = [&](double2 *force, Lin bot, const double2 *side, const double2 *top, int mu,
                         int nu, double2 *stpl, int acc) {
-    hipLaunchKernelGGL(side_force_kernel, gs, b, 0, st, g, V, force, stpl, acc, bot, side, top, mu, nu);
+    if (acc && bot.b)
+      hipLaunchKernelGGL((side_force_kernel<true, true>), gx, b, 0, st, g, sq, V, force, stpl, bot, side, top, mu, nu);
+    else if (acc)
+      hipLaunchKernelGGL((side_force_kernel<true, false>), gx, b, 0, st, g, sq, V, force, stpl, bot, side, top, mu, nu);
+    else if (bot.b)
+      hipLaunchKernelGGL((side_force_kernel<false, true>), gx, b, 0, st, g, sq, V, force, stpl, bot, side, top, mu, nu);
+    else
+      hipLaunchKernelGGL((side_force_kernel<false, false>), gx, b, 0, st, g, sq, V, force, stpl, bot, side, top, mu, nu);
   };
   auto lincomb = [&](double2 *o, double co, double ca, const double2 *A, double cb, const double2 *B) {
     hipLaunchKernelGGL(lincomb_kernel, ge, b, 0, st, o, co, ca, A, cb, B, n9);
@@ -516,8 +619,8 @@ void asqtad_force_dev(const Lattice *lat, double *const U_raw[4], double *F_raw[
   }
   if (coefL != 0) {
     for (int mu = 0; mu < 4; mu++) {
-      hipLaunchKernelGGL(lepage_acc_kernel, gs, b, 0, st, g, V, mid3, d_U, d_mid, mu);
-      hipLaunchKernelGGL(lepage_apply_kernel, gs, b, 0, st, g, V, D[mu], (const double2 *)U[mu],
+      hipLaunchKernelGGL(lepage_acc_kernel, gx, b, 0, st, g, sq, V, mid3, d_U, d_mid, mu);
+      hipLaunchKernelGGL(lepage_apply_kernel, gx, b, 0, st, g, sq, V, D[mu], (const double2 *)U[mu],
                          (const double2 *)mid3, -coefL, mu);
     }
     nfl += 4 * (3 * (2 * 198 + 3 * 18) + 198 + 216 + 36);
@@ -525,7 +628,7 @@ void asqtad_force_dev(const Lattice *lat, double *const U_raw[4], double *F_raw[
   // Naik part (lib/asqtad_force_fnmat_p.c:395-445)
   if (naik) {
     for (int mu = 0; mu < 4; mu++)
-      hipLaunchKernelGGL(naik_deriv_kernel, gs, b, 0, st, g, V, D[mu], (const double2 *)U[mu],
+      hipLaunchKernelGGL(naik_deriv_kernel, gx, b, 0, st, g, sq, V, D[mu], (const double2 *)U[mu],
                          (const double2 *)mid[4 + mu], coef[5], mu);
     nfl += 4 * (4 * 198 + 216 + 18 + 36);
   }
