@@ -369,3 +369,106 @@ void or_asqtad_force_multi(const int L[4], double *const gauge[4], double *force
   for (int i = 0; i < 12; i++) free(w.t[i]);
   geom_free(&g);
 }
+
+/* HISQ fermion force, fnmat form (lib/hisq_force_fnmat_p.c:7-231).
+ * Level 2: for each epsilon-Naik group k the mids (factor +1, odd sites
+ * negated, group 0 over all terms) are differentiated through the asqtad
+ * smearing of W (group 0: level-2 coefficients; k > 0: the difference set)
+ * and summed with weight 1 (k = 0) or eps_naik[k] into F2 (:56-118).  The
+ * U(3) projection's derivative turns F2 into the mid of level 1
+ * (or_u3_deriv_contract, then the adjoint: :141-152); level 1 is the fat7
+ * derivative on U; deriv = 2 x level 1 and force += antiherm(U deriv^+)
+ * (:171-181, 211-219). */
+int or_hisq_force_multi(const int L[4], double *const U[4], double *force[4],
+                        const or_asqtad_coeffs_t *fat7, const or_asqtad_coeffs_t *lev2,
+                        double diff_one_link, double diff_naik, int n_naiks,
+                        const double eps_naik[], const double eps[], double *const x[],
+                        const int n_orders_naik[], double force_filter) {
+  geom_t g;
+  geom_init(&g, L);
+  ws_t w;
+  w.g = &g;
+  for (int i = 0; i < 12; i++) w.t[i] = fnew(&g);
+  const long V = g.V, Vh = V / 2;
+  double *Vl[4], *Wl[4], *F1[4], *F2[4], *mid[8];
+  for (int i = 0; i < 4; i++) {
+    Vl[i] = fnew(&g);
+    Wl[i] = fnew(&g);
+    F1[i] = fnew(&g);
+    F2[i] = fnew(&g);
+  }
+  for (int i = 0; i < 8; i++) mid[i] = fnew(&g);
+  /* V = fat7(U) (user normalisation: 2 x internal link), W = U(3) projection */
+  or_asqtad_coeffs_t c7 = *fat7;
+  c7.naik = 0;
+  or_make_imp_links(L, &c7, U, U, Vl, NULL);
+  or_reunit_opts_t ro = {1, 0, 1e-8, 1e-8};
+#pragma omp parallel for
+  for (long i = 0; i < 4 * V; i++) {
+    const int mu = (int)(i / V);
+    const long s = i % V;
+    double fl = 0;
+    or_u3_un_analytic(Vl[mu] + 18 * s, Wl[mu] + 18 * s, &ro, &fl);
+  }
+  int nterms = 0;
+  for (int k = 0; k < n_naiks; k++) nterms += n_orders_naik[k];
+  int shift = 0;
+  for (int k = 0; k < n_naiks; k++) {
+    const int cnt = k == 0 ? nterms : n_orders_naik[k];
+    get_mid(&g, mid, 1, eps + shift, 1.0, x + shift, cnt);
+    get_mid(&g, mid + 4, 3, eps + shift, 1.0, x + shift, cnt);
+    for (int d = 0; d < 8; d++)
+      for (long i = Vh; i < V; i++)
+        for (int e = 0; e < 18; e++) mid[d][18 * i + e] = -mid[d][18 * i + e];
+    or_asqtad_coeffs_t c2 = *lev2;
+    if (k > 0) {
+      c2.one_link = diff_one_link;
+      c2.three_staple = c2.five_staple = c2.seven_staple = c2.lepage = 0;
+      c2.naik = diff_naik;
+    }
+    for (int mu = 0; mu < 4; mu++) fzero(&g, F1[mu]);
+    fat_deriv(&w, Wl, F1, &c2, mid);
+    if (c2.naik != 0) naik_deriv(&w, Wl, F1, c2.naik, mid + 4);
+    const double cm = k == 0 ? 1.0 : eps_naik[k];
+    for (int mu = 0; mu < 4; mu++) faxpy(&g, F2[mu], cm, F1[mu], 0);
+    shift += n_orders_naik[k];
+  }
+  /* projection derivative; the level-1 mid is its adjoint */
+  int nfilt = 0;
+#pragma omp parallel for reduction(+ : nfilt)
+  for (long i = 0; i < 4 * V; i++) {
+    const int mu = (int)(i / V);
+    const long s = i % V;
+    double y[18];
+    nfilt += or_u3_deriv_contract(Vl[mu] + 18 * s, F2[mu] + 18 * s, force_filter, y);
+    double *m = mid[mu] + 18 * s;
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) {
+        RE(m, r, c) = RE(y, c, r);
+        IM(m, r, c) = -IM(y, c, r);
+      }
+  }
+  for (int mu = 0; mu < 4; mu++) fzero(&g, F1[mu]);
+  c7.naik = 0;
+  fat_deriv(&w, U, F1, &c7, mid);
+  /* force += antiherm(U (2 F1)^+) */
+  for (int mu = 0; mu < 4; mu++) {
+#pragma omp parallel for
+    for (long i = 0; i < V; i++) {
+      double m[18], a[18];
+      smul(m, U[mu] + 18 * i, 0, F1[mu] + 18 * i, 1, 2.0, 0);
+      antiherm(a, m);
+      for (int e = 0; e < 18; e++) force[mu][18 * i + e] += a[e];
+    }
+  }
+  for (int i = 0; i < 4; i++) {
+    free(Vl[i]);
+    free(Wl[i]);
+    free(F1[i]);
+    free(F2[i]);
+  }
+  for (int i = 0; i < 8; i++) free(mid[i]);
+  for (int i = 0; i < 12; i++) free(w.t[i]);
+  geom_free(&g);
+  return nfilt;
+}

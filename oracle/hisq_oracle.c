@@ -16,6 +16,13 @@
  *   or_hisq_W           lib/hisq_links_p.c:62-152 (fat7 stage, x2 of the
  *                       internal 1/2, U(3) projection)            free field: fat7 weight 1 so W = U;
  *                                                              gauge covariance of W
+ *   or_u3_deriv_contract lib/hisq_reunit1_p.c:366-433 (+ the derivative of
+ *                       hisq_reunit_utilities_p.c:301-640, restated from the
+ *                       mathematics in the eigenbasis of V^dagger V)
+ *                                                              central differences of
+ *                                                              tr(W F^+) + tr(W^+ F) on
+ *                                                              generic and near-unitary V,
+ *                                                              with and without the filter
  * The level-2 asqtad smearing of W is or_make_imp_links (qop_oracle.c), as
  * lib/hisq_links_p.c:233 calls QOP_asqtad_create_L_from_G on W.
  */
@@ -81,10 +88,11 @@ static double det_abs2(const double *v) {
  * A = V until they are mutually orthogonal, A = V R with R = prod G unitary;
  * then A = U diag(sigma), V = U diag(sigma) R^dagger and W = U R^dagger.  A
  * zero column of A (singular V) takes the unit vector completing U. */
-void or_u3_polar_jacobi(const double *V, double *W) {
-  double a[18], r[18];
-  memcpy(a, V, sizeof a);
-  memset(r, 0, sizeof r);
+/* One-sided Jacobi: a = V r with orthogonal columns, r unitary (the right
+ * singular vectors of V, i.e. the eigenvectors of V^dagger V). */
+static void jacobi_right(const double *V, double *a, double *r) {
+  memcpy(a, V, 18 * sizeof(double));
+  memset(r, 0, 18 * sizeof(double));
   for (int i = 0; i < 3; i++) RE(r, i, i) = 1;
   for (int sweep = 0; sweep < 60; sweep++) {
     double worst = 0;
@@ -120,6 +128,11 @@ void or_u3_polar_jacobi(const double *V, double *W) {
       }
     if (worst < 1e-16) break;
   }
+}
+
+void or_u3_polar_jacobi(const double *V, double *W) {
+  double a[18], r[18];
+  jacobi_right(V, a, r);
   /* U: normalised columns of a */
   double u[18], sig[3], smax = 0;
   for (int j = 0; j < 3; j++) {
@@ -251,4 +264,82 @@ int or_hisq_W(const int L[4], const or_asqtad_coeffs_t *fat7, double *const U[4]
   }
   if (flops) *flops += fl;
   return nsvd;
+}
+
+/* Derivative of the U(3) projection contracted with the force of the level
+ * above (lib/hisq_reunit1_p.c:366-433 with hisq_reunit_utilities_p.c:301-640;
+ * restated from the mathematics, not from that routine's closed form).
+ *
+ * With W = V Q^-1/2, Q = V^dagger V (+ f 1 when the force filter acts),
+ * F the accumulated derivative with respect to W and
+ * Phi(V) = tr(W F^dagger) + tr(W^dagger F), the reference returns
+ * Y(n,m) = dPhi/dV_mn (V^* held fixed).  In the eigenbasis Q = E diag(l) E^+,
+ * s = sqrt(l):  dQ^-1/2 = E [G o (E^+ dQ E)] E^+, G_ij = -1 / (s_i s_j (s_i + s_j)),
+ * which gives
+ *   Y = Q^-1/2 F^+ + (N + N^+) V^+,   N = E [G o (E^+ F^+ V E)] E^+.
+ * The filter (hisq_reunit_utilities_p.c:437-456): when the smallest
+ * eigenvalue is below f > 0, f is added to all of them.  Returns 1 when the
+ * filter acted. */
+static void mmul_t(double *c, const double *a, int ta, const double *b, int tb) {
+  double t[18];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double re = 0, im = 0;
+      for (int k = 0; k < 3; k++) {
+        double ar = ta ? RE(a, k, i) : RE(a, i, k), ai = ta ? -IM(a, k, i) : IM(a, i, k);
+        double br = tb ? RE(b, j, k) : RE(b, k, j), bi = tb ? -IM(b, j, k) : IM(b, k, j);
+        re += ar * br - ai * bi;
+        im += ar * bi + ai * br;
+      }
+      RE(t, i, j) = re;
+      IM(t, i, j) = im;
+    }
+  memcpy(c, t, sizeof t);
+}
+
+int or_u3_deriv_contract(const double *V, const double *F, double filter, double *Y) {
+  double a[18], E[18], l[3], s[3];
+  jacobi_right(V, a, E);
+  for (int j = 0; j < 3; j++) {
+    l[j] = 0;
+    for (int k = 0; k < 3; k++) l[j] += RE(a, k, j) * RE(a, k, j) + IM(a, k, j) * IM(a, k, j);
+  }
+  int filtered = 0;
+  if (filter > 0 && (l[0] < filter || l[1] < filter || l[2] < filter)) {
+    for (int j = 0; j < 3; j++) l[j] += filter;
+    filtered = 1;
+  }
+  for (int j = 0; j < 3; j++) s[j] = sqrt(l[j]);
+  /* Q^-1/2 = E diag(1/s) E^+ */
+  double D[18], Qm[18], t[18];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      RE(t, i, j) = RE(E, i, j) / s[j];
+      IM(t, i, j) = IM(E, i, j) / s[j];
+    }
+  mmul_t(Qm, t, 0, E, 1);
+  /* M' = E^+ F^+ V E, N' = G o M', N = E N' E^+ */
+  double M[18], Mp[18], N[18];
+  mmul_t(M, F, 1, V, 0);
+  mmul_t(t, E, 1, M, 0);
+  mmul_t(Mp, t, 0, E, 0);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double g = -1.0 / (s[i] * s[j] * (s[i] + s[j]));
+      RE(Mp, i, j) *= g;
+      IM(Mp, i, j) *= g;
+    }
+  mmul_t(t, E, 0, Mp, 0);
+  mmul_t(N, t, 0, E, 1);
+  /* Y = Q^-1/2 F^+ + (N + N^+) V^+ */
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      RE(D, i, j) = RE(N, i, j) + RE(N, j, i);
+      IM(D, i, j) = IM(N, i, j) - IM(N, j, i);
+    }
+  double y1[18], y2[18];
+  mmul_t(y1, Qm, 0, F, 1);
+  mmul_t(y2, D, 0, V, 1);
+  for (int e = 0; e < 18; e++) Y[e] = y1[e] + y2[e];
+  return filtered;
 }

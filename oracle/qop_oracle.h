@@ -101,6 +101,21 @@ void or_u3_polar_jacobi(const double *V, double *W);
 int or_hisq_W(const int L[4], const or_asqtad_coeffs_t *fat7, double *const U[4], double *W[4],
               const or_reunit_opts_t *o, double *flops);
 
+/* Derivative of the U(3) projection W = V (V^+V [+ f])^-1/2 contracted with
+ * F: Y(n,m) = d[tr(W F^+) + tr(W^+ F)]/dV_mn (lib/hisq_reunit1_p.c:366-433);
+ * returns 1 when the force filter f acted. */
+int or_u3_deriv_contract(const double *V, const double *F, double filter, double *Y);
+/* HISQ fermion force, fnmat form (lib/hisq_force_fnmat_p.c:7-231):
+ * force[mu] += antiherm(U_mu deriv_mu^+).  fat7: level-1 coefficients (naik
+ * ignored, lepage as the caller wants it), lev2: level-2 asqtad
+ * coefficients, diff_*: the epsilon-Naik difference set; the x are grouped
+ * by n_orders_naik[0..n_naiks-1].  Returns the number of filtered links. */
+int or_hisq_force_multi(const int L[4], double *const U[4], double *force[4],
+                        const or_asqtad_coeffs_t *fat7, const or_asqtad_coeffs_t *lev2,
+                        double diff_one_link, double diff_naik, int n_naiks,
+                        const double eps_naik[], const double eps[], double *const x[],
+                        const int n_orders_naik[], double force_filter);
+
 /* Number of iterations done by the last inner solver calls (diagnostics). */
 /* Asqtad fermion force, fnmat form (lib/asqtad_force_fnmat_p.c:474-546):
  * force[mu] += -2 antiherm(U_mu deriv_mu^+), deriv the derivative of the fat

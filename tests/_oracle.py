@@ -82,6 +82,13 @@ class Oracle:
             L.or_u3_un_analytic.restype = C.c_int
             L.or_u3_un_analytic.argtypes = [_D, _D, C.POINTER(ReunitOpts), C.POINTER(C.c_double)]
             L.or_u3_polar_jacobi.argtypes = [_D, _D]
+            L.or_u3_deriv_contract.restype = C.c_int
+            L.or_u3_deriv_contract.argtypes = [_D, _D, C.c_double, _D]
+            L.or_hisq_force_multi.restype = C.c_int
+            L.or_hisq_force_multi.argtypes = [_IV, _PP, _PP, C.POINTER(Coeffs), C.POINTER(Coeffs),
+                                              C.c_double, C.c_double, C.c_int,
+                                              C.POINTER(C.c_double), C.POINTER(C.c_double), _PP,
+                                              C.POINTER(C.c_int), C.c_double]
             L.or_hisq_W.restype = C.c_int
             L.or_hisq_W.argtypes = [_IV, C.POINTER(Coeffs), _PP, _PP, C.POINTER(ReunitOpts),
                                     C.POINTER(C.c_double)]
@@ -226,6 +233,30 @@ class Oracle:
         self.lib.or_u3_polar_jacobi(v, w)
         w = w.reshape(3, 3, 2)
         return w[..., 0] + 1j * w[..., 1]
+
+    def u3_deriv_contract(self, V, F, filt=0.0):
+        """Y(n,m) = d[tr(W F^+) + tr(W^+ F)]/dV_mn for (3,3) complex V, F; (Y, filtered)"""
+        v = np.ascontiguousarray(np.stack([V.real, V.imag], -1), dtype=np.float64).reshape(-1)
+        f = np.ascontiguousarray(np.stack([F.real, F.imag], -1), dtype=np.float64).reshape(-1)
+        y = np.zeros(18)
+        k = self.lib.or_u3_deriv_contract(v, f, filt, y)
+        y = y.reshape(3, 3, 2)
+        return y[..., 0] + 1j * y[..., 1], k
+
+    def hisq_force(self, L, U, fat7, lev2, diff_one_link, diff_naik, eps_naik, eps, xs, n_orders,
+                   force_filter=0.0, force=None):
+        """force[mu] += antiherm(U deriv^+) of the HISQ fnmat force
+        (lib/hisq_force_fnmat_p.c:7-231); returns (force, filtered links)."""
+        V = int(np.prod(L))
+        if force is None:
+            force = [np.zeros((V, 3, 3, 2)) for _ in range(4)]
+        xs = [np.ascontiguousarray(x, dtype=np.float64) for x in xs]
+        n = len(n_orders)
+        k = self.lib.or_hisq_force_multi(
+            _iv(L), _pa([np.ascontiguousarray(u) for u in U]), _pa(force), C.byref(Coeffs(**fat7)),
+            C.byref(Coeffs(**lev2)), diff_one_link, diff_naik, n, (C.c_double * n)(*eps_naik),
+            (C.c_double * len(eps))(*eps), _pa(xs), (C.c_int * n)(*n_orders), force_filter)
+        return force, k
 
     def hisq_W(self, L, U, fat7, opts=None):
         """fat7 smearing + U(3) projection: list of 4 (V,3,3,2) arrays, #fallbacks, flops"""
