@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--force", action="store_true",
                     help="asqtad fermion force (fnmat form) at --lattice instead of the solver bench")
     ap.add_argument("--force-terms", type=int, default=8, help="vectors x_i of the force")
+    ap.add_argument("--force-action", default="asqtad", choices=["asqtad", "hisq"],
+                    help="--force: asqtad (MILC coefficients) or HISQ (MILC HISQ, one Naik group)")
     ap.add_argument("--fp64-sources", type=int, default=16,
                     help="--multi: sequential FP64 multi-mass solves timed for the baseline")
     args = ap.parse_args()
@@ -453,10 +455,12 @@ def multi_bench(args):
 
 
 def force_bench(args):
-    """Asqtad fermion force, QOP_D3_asqtad_force_multi (fnmat form): one step
-    is one force of --force-terms full-lattice vectors (Gaussian, seeded) on a
-    seeded SU(3) gauge field at --lattice (default 32^3x64) with the MILC
-    coefficients, accumulated into a device force field.  cpu_baseline: the C
+    """Fermion force (fnmat form): one step is one force of --force-terms
+    full-lattice vectors (Gaussian, seeded) on a seeded SU(3) gauge field at
+    --lattice (default 32^3x64), accumulated into a device force field.
+    --force-action asqtad: QOP_D3_asqtad_force_multi, MILC coefficients;
+    hisq: QOP_D3_hisq_force_multi on HISQ links built from the rephased field
+    (MILC HISQ coefficients, one epsilon-Naik group).  cpu_baseline: the C
     oracle's force of the same number of terms on a 16^3x32 sample (1/16 of
     the volume at the default lattice), scaled by the volume ratio (every
     operation of the force is per site)."""
@@ -464,20 +468,38 @@ def force_bench(args):
     L = [int(x) for x in args.lattice.split(",")]
     V = int(np.prod(L))
     n = args.force_terms
+    hisq = args.force_action == "hisq"
     s = q.Session(L)
     U = q.gen_gauge(L, 2000, 0.3)
     g = s.gauge(U)
     del U
+    flh = None
+    if hisq:
+        r0, bc, sign, keep = q.milc_rephase_args(L)
+        s.lib.QOP_D3_rephase_G(g, r0, C.byref(bc), C.byref(sign))
+        hc = q.hisq_coeffs()
+        hinfo = q.QOPInfo()
+        flh = s.lib.QOP_D3_hisq_create_L_from_G(C.byref(hinfo), C.byref(hc), g)
+        assert hinfo.status == 0
     f = s.force([np.zeros((V, 3, 3, 2)) for _ in range(4)])
     xs = [s.vec(q.gen_source(L, 2100 + i, q.QOP_EVENODD)) for i in range(n)]
     eps = [1.0 / (i + 1) for i in range(n)]
+
+    def one():
+        if not hisq:
+            return s.asqtad_force(g, f, eps, xs)
+        info = q.QOPInfo()
+        s.lib.QOP_D3_hisq_force_multi(C.byref(info), flh, f, C.byref(hc), (C.c_double * n)(*eps),
+                                      (C.c_void_p * n)(*xs), (C.c_int * 1)(n))
+        return info
+
     for _ in range(args.warmup):
-        info = s.asqtad_force(g, f, eps, xs)
+        info = one()
         assert info.status == 0
     t = time.perf_counter()
     secs = []
     for _ in range(args.steps):
-        info = s.asqtad_force(g, f, eps, xs)
+        info = one()
         assert info.status == 0
         secs.append(info.final_sec)
     dt = time.perf_counter() - t
@@ -488,24 +510,27 @@ def force_bench(args):
     for x in xs:
         s.free(x)
     s.lib.QOP_D3_destroy_F(f)
+    if flh:
+        s.lib.QOP_D3_hisq_destroy_L(flh)
     s.lib.QOP_D3_destroy_G(g)
     s.close()
     out = {
-        "metric": "asqtad fermion force (fnmat form) per second",
+        "metric": f"{args.force_action} fermion force (fnmat form) per second",
         "value": round(args.steps / dt, 3), "unit": "forces/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True,
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"asqtad force {'x'.join(map(str, L))}, {n} terms, MILC coefficients",
+        "config": {"workload": f"{args.force_action} force {'x'.join(map(str, L))}, {n} terms, MILC "
+                               f"{'HISQ ' if hisq else ''}coefficients",
                    "lattice": L, "terms": n},
         "gflops_reference_count": round(flop / float(np.median(secs)) / 1e9, 1),
         "antihermiticity_check": antiherm,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = force_cpu_baseline(L, n, eps)
+        out["cpu_baseline"] = force_cpu_baseline(L, n, eps, hisq)
     print(json.dumps(out), flush=True)
 
 
-def force_cpu_baseline(L, n, eps):
+def force_cpu_baseline(L, n, eps, hisq=False):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from _oracle import Oracle
     import qopqdp_amd as q
@@ -517,8 +542,16 @@ def force_cpu_baseline(L, n, eps):
     xs = [rng.standard_normal((Vs, 3, 2)) for _ in range(n)]
     coeffs = dict(one_link=5.0 / 8, three_staple=-1.0 / 16, five_staple=1.0 / 64,
                   seven_staple=-1.0 / 384, lepage=-1.0 / 16, naik=-1.0 / 24)
+    if hisq:
+        orc.rephase_milc(Ls, U)
+        fat7 = dict(one_link=1 / 8, three_staple=-1 / 16, five_staple=1 / 64, seven_staple=-1 / 384,
+                    lepage=0.0, naik=0.0)
+        lev2 = dict(coeffs, one_link=1.0, lepage=-1 / 8)
     t = time.perf_counter()
-    orc.asqtad_force(Ls, U, coeffs, eps, xs)
+    if hisq:
+        orc.hisq_force(Ls, U, fat7, lev2, 0.0, -1 / 24, [0.0], eps, xs, [n], force_filter=5e-5)
+    else:
+        orc.asqtad_force(Ls, U, coeffs, eps, xs)
     dt = time.perf_counter() - t
     scale = int(np.prod(L)) / Vs
     log(f"cpu force baseline: {dt:.2f} s at {Ls} on {orc.threads()} threads")

@@ -367,6 +367,27 @@ void QOP_F3_asqtad_force_multi(QOP_info_t *info, QOP_F3_GaugeField *gauge, QOP_F
                                QOP_asqtad_coeffs_t *coef, QOP_F_Real eps[],
                                QOP_F3_ColorVector *in_pt[], int nsrc);
 
+/* ---- HISQ fermion force (include/qop_pc.h:485-510; lib/hisq_force_p.c) ----
+ * force[mu] += TA(U_mu deriv_mu^+), deriv the derivative of
+ * sum_i eps[i] x_i x_i^+ through the level-2 smearing of W (per
+ * epsilon-Naik group: in_pt grouped by n_orders_naik[0..n_naiks-1]), the
+ * U(3) projection W = V (V^+V)^-1/2 and the fat7 level on U
+ * (lib/hisq_force_fnmat_p.c:7-231).  Needs the auxiliary links kept by
+ * QOP_hisq_create_L_from_G (option want_aux, default 1) and
+ * umethod = QOP_UNITARIZE_RATIONAL; QOP_FAIL otherwise.  The number of links
+ * whose eigenvalues the force filter raised is added to
+ * QOP_info_hisq_force_filter_counter(info).  Options (lib/hisq_force.c:29-52):
+ * fnmat_src_min, veclength (kept; every nsrc takes the fnmat form),
+ * force_filter (default 5e-5). */
+#define QOP_info_hisq_force_filter_counter(info) ((info)->count2)
+QOP_status_t QOP_hisq_force_set_opts(QOP_opt_t opts[], int nopts); /* lib/hisq_force.c:29-52 */
+void QOP_D3_hisq_force_multi(QOP_info_t *info, QOP_D3_FermionLinksHisq *flh, QOP_D3_Force *force,
+                             QOP_hisq_coeffs_t *coef, QOP_D_Real eps[], QOP_D3_ColorVector *in_pt[],
+                             int *n_orders_naik); /* lib/hisq_force_p.c:63-83 */
+void QOP_F3_hisq_force_multi(QOP_info_t *info, QOP_F3_FermionLinksHisq *flh, QOP_F3_Force *force,
+                             QOP_hisq_coeffs_t *coef, QOP_F_Real eps[], QOP_F3_ColorVector *in_pt[],
+                             int *n_orders_naik);
+
 /* ---- request server: continuous batching over resident configurations ----
  * A server keeps fat/long links of any number of gauge configurations, of
  * any lattice sizes, resident on the device (QOP_init is not needed and its
@@ -483,6 +504,7 @@ const char *QOPX_last_error(void);
 #  define QOP_destroy_F QOP_D3_destroy_F
 #  define QOP_asqtad_force QOP_D3_asqtad_force
 #  define QOP_asqtad_force_multi QOP_D3_asqtad_force_multi
+#  define QOP_hisq_force_multi QOP_D3_hisq_force_multi
 #elif QOP_Precision == 'F'
 #  define QOP_Real QOP_F_Real
 #  define QOP_ColorVector QOP_F3_ColorVector
@@ -516,6 +538,7 @@ const char *QOPX_last_error(void);
 #  define QOP_destroy_F QOP_F3_destroy_F
 #  define QOP_asqtad_force QOP_F3_asqtad_force
 #  define QOP_asqtad_force_multi QOP_F3_asqtad_force_multi
+#  define QOP_hisq_force_multi QOP_F3_hisq_force_multi
 #else
 #  error "bad QOP_Precision"
 #endif

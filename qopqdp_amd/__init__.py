@@ -175,6 +175,11 @@ SIGNATURES = [
     ("QOP_hisq_links_set_opts", C.c_int, [C.POINTER(QOPOpt), C.c_int]),
     ("QOP_D3_hisq_create_L_from_G", _VP, [C.POINTER(QOPInfo), C.POINTER(QOPHisqCoeffs), _VP]),
     ("QOP_F3_hisq_create_L_from_G", _VP, [C.POINTER(QOPInfo), C.POINTER(QOPHisqCoeffs), _VP]),
+    ("QOP_hisq_force_set_opts", C.c_int, [C.POINTER(QOPOpt), C.c_int]),
+    ("QOP_D3_hisq_force_multi", None, [C.POINTER(QOPInfo), _VP, _VP, C.POINTER(QOPHisqCoeffs),
+                                       C.POINTER(C.c_double), _PP, C.POINTER(C.c_int)]),
+    ("QOP_F3_hisq_force_multi", None, [C.POINTER(QOPInfo), _VP, _VP, C.POINTER(QOPHisqCoeffs),
+                                       C.POINTER(C.c_float), _PP, C.POINTER(C.c_int)]),
     ("QOP_D3_hisq_destroy_L", None, [_VP]),
     ("QOP_F3_hisq_destroy_L", None, [_VP]),
     ("QOP_D3_get_asqtad_links_from_hisq", C.POINTER(_VP), [_VP]),

@@ -185,6 +185,17 @@ void asqtad_force_dev(const Lattice *lat, double *const U_raw[4], double *F_raw[
                       const double coef[6], const double *eps, const DevVec *const *xs, int n,
                       double *flops, hipStream_t st);
 
+// HISQ fermion force (force.hip): F[mu] += TA(U_mu deriv_mu^+) with deriv
+// through level 2 (W, lev2 or the difference set diff per epsilon-Naik group),
+// the U(3) projection of V and the fat7 level on U (raw FP64 device links);
+// the vectors are grouped by n_orders[0..n_naiks-1]; filter: force filter;
+// nfilt receives the number of filtered links.
+void hisq_force_dev(const Lattice *lat, double *const U_raw[4], double *const V_raw[4],
+                    double *const W_raw[4], double *F_raw[4], const double fat7[6],
+                    const double lev2[6], const double diff[6], int n_naiks, const double *eps_naik,
+                    const double *eps, const DevVec *const *xs, const int *n_orders, double filter,
+                    double *flops, int *nfilt, hipStream_t st);
+
 // raw <-> SoA (kernels.hip); eo selects the sites converted.
 void launch_raw_to_vec(const Lattice *lat, const double *raw_dev, DevVec &v,
                        int eo, hipStream_t st);

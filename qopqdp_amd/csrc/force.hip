@@ -179,6 +179,11 @@ __device__ __forceinline__ M3 ld(const Lin &f, long V, unsigned s) {
   return m;
 }
 
+// Four field pointers passed by value (one per direction).
+struct Ptr4 {
+  const double2 *p[4];
+};
+
 // raw [V][18] -> site-innermost [9][V]
 __global__ __launch_bounds__(TPB) void to_soa_kernel(const double2 *raw, double2 *f, long V) {
   const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -209,19 +214,22 @@ __global__ __launch_bounds__(TPB) void lincomb_kernel(double2 *o, double co, dou
   o[k] = r;
 }
 
-// QOP_get_mid (lib/asqtad_force_fnmat_p.c:4-44) with the factor -1 and the
-// negation on odd sites (:491-503): mid_s(x) = -(-1)^p(x) sum_t eps_t
-// x_t(x) x_t(x + dist_s)^+, dist_s = s-th direction (s < 4) or three times
-// it (s >= 4).  Vectors are full-lattice [parity][colour][site] complex.
+// QOP_get_mid (lib/asqtad_force_fnmat_p.c:4-44) with its factor (scale: -1
+// in the asqtad force, :491-503; +1 in the HISQ force,
+// lib/hisq_force_fnmat_p.c:63-76) and the negation on odd sites:
+// mid_s(x) = scale (-1)^p(x) sum_t eps_t x_t(x) x_t(x + dist_s)^+,
+// dist_s = s-th direction (s < 4) or three times it (s >= 4).  Vectors are
+// full-lattice [parity][colour][site] complex.
 template <typename T2>
-__global__ __launch_bounds__(TPB) void mid_kernel(Geom g, Sched q, long V, const T2 *const *xs, const double *eps, int n, int nmid,
-                           double2 *mid) {
+__global__ __launch_bounds__(TPB) void mid_kernel(Geom g, Sched q, long V, const T2 *const *xs,
+                                                  const double *eps, int n, int nmid, double scale,
+                                                  double2 *mid) {
   const int s0 = site_index(q, g.Vh);
   if (s0 < 0) return;
   const unsigned s = (unsigned)s0;
   const Site x = site_of(g, s);
   const long Vh = g.Vh;
-  const double sg = s >= Vh ? 1.0 : -1.0;
+  const double sg = s >= Vh ? -scale : scale;
   auto vidx = [&](long i, int c) { return i + (i >= Vh ? 2 * Vh : 0) + c * Vh; };
   for (int m = 0; m < nmid; m++) {
     const unsigned j = nbr(g, x, m & 3, m < 4 ? 1 : 3, 0, 0);
@@ -375,8 +383,8 @@ __global__ __launch_bounds__(TPB) void naik_deriv_kernel(Geom g, Sched q, long V
 
 // Lepage fix-up (lib/asqtad_force_fnmat_p.c:317-339), first half:
 // acc(x) = sum_{nu != mu} [q_nu(x) + q_nu(x)^+], q_nu(x) = m_nu(x) U_nu(x)^+ - m_nu(x-nu)^+ U_nu(x-nu)
-__global__ __launch_bounds__(TPB) void lepage_acc_kernel(Geom g, Sched q, long V, double2 *acc, const double2 *const *U,
-                                  const double2 *const *m, int mu) {
+__global__ __launch_bounds__(TPB) void lepage_acc_kernel(Geom g, Sched q, long V, double2 *acc, Ptr4 U,
+                                                         Ptr4 m, int mu) {
   const int s0 = site_index(q, g.Vh);
   if (s0 < 0) return;
   const unsigned s = (unsigned)s0;
@@ -385,10 +393,10 @@ __global__ __launch_bounds__(TPB) void lepage_acc_kernel(Geom g, Sched q, long V
   for (int nu = 0; nu < 4; nu++) {
     if (nu == mu) continue;
     const unsigned b = nbr(g, x, nu, -1, 0, 0);
-    M3 q = mm<false, true>(ld(m[nu], V, s), ld(U[nu], V, s));
-    axpy<false>(q, -1.0, mm<true, false>(ld(m[nu], V, b), ld(U[nu], V, b)));
-    axpy<false>(r, 1.0, q);
-    axpy<true>(r, 1.0, q);
+    M3 t = mm<false, true>(ld(m.p[nu], V, s), ld(U.p[nu], V, s));
+    axpy<false>(t, -1.0, mm<true, false>(ld(m.p[nu], V, b), ld(U.p[nu], V, b)));
+    axpy<false>(r, 1.0, t);
+    axpy<true>(r, 1.0, t);
   }
   st(acc, V, s, r);
 }
@@ -408,9 +416,11 @@ __global__ __launch_bounds__(TPB) void lepage_apply_kernel(Geom g, Sched q, long
   st(D, V, s, d);
 }
 
-// force (raw layout) += -2 TA(U D^+), TA the traceless antihermitian part
-// (lib/asqtad_force_fnmat_p.c:531-539)
-__global__ __launch_bounds__(TPB) void force_kernel(long V, double2 *F, const double2 *U, const double2 *D) {
+// force (raw layout) += fs TA(U D^+), TA the traceless antihermitian part:
+// fs = -2 in the asqtad force (lib/asqtad_force_fnmat_p.c:531-539), +2 in the
+// HISQ force (factor 2 of lib/hisq_force_fnmat_p.c:171-181, then :211-219)
+__global__ __launch_bounds__(TPB) void force_kernel(long V, double2 *F, const double2 *U, const double2 *D,
+                                                    double fs) {
   const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (s >= V) return;
   const M3 a = mm<false, true>(ld(U, V, s), ld(D, V, s));
@@ -429,10 +439,123 @@ __global__ __launch_bounds__(TPB) void force_kernel(long V, double2 *F, const do
 #pragma unroll
   for (int e = 0; e < 9; e++) {
     double2 f = F[9 * s + e];
-    f.x = fma(-2.0, t.e[e].x, f.x);
-    f.y = fma(-2.0, t.e[e].y, f.y);
+    f.x = fma(fs, t.e[e].x, f.x);
+    f.y = fma(fs, t.e[e].y, f.y);
     F[9 * s + e] = f;
   }
+}
+
+// Derivative of the U(3) projection W = V (V^+V)^-1/2 contracted with F2,
+// the derivative accumulated with respect to W (lib/hisq_reunit1_p.c:366-433
+// and the derivative of lib/hisq_reunit_utilities_p.c:301-640, computed here
+// in the eigenbasis of Q = V^+V instead of the reference's closed form):
+// with Q = E diag(l) E^+ from a one-sided Jacobi rotation of V (a = V E has
+// orthogonal columns, l_j = |a_j|^2), s = sqrt(l),
+//   Y = Q^-1/2 F2^+ + (N + N^+) V^+,
+//   N = E [G o (E^+ F2^+ V E)] E^+,  G_ij = -1 / (s_i s_j (s_i + s_j)),
+// i.e. Y(n,m) = d[tr(W F2^+) + tr(W^+ F2)]/dV_mn; G has no division by
+// eigenvalue differences, so equal eigenvalues need no special case.  The
+// force filter (hisq_reunit_utilities_p.c:437-456) adds f to every eigenvalue
+// when the smallest is below f.  Writes the level-1 mid Y^+ (site-innermost)
+// and counts the filtered links.
+template <int P, int Q>
+__device__ __forceinline__ void jacobi_rot(M3 &a, M3 &E, double &worst) {
+  double al = 0, be = 0, gr = 0, gi = 0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double2 x = a.e[3 * k + P], y = a.e[3 * k + Q];
+    al = fma(x.x, x.x, fma(x.y, x.y, al));
+    be = fma(y.x, y.x, fma(y.y, y.y, be));
+    gr = fma(x.x, y.x, fma(x.y, y.y, gr));
+    gi = fma(x.x, y.y, fma(-x.y, y.x, gi));
+  }
+  const double g = sqrt(gr * gr + gi * gi);
+  const double ab = sqrt(al * be);
+  if (g == 0 || g <= 1e-17 * ab) return;
+  worst = fmax(worst, g / ab);
+  const double cph = gr / g, sph = gi / g;
+  const double zeta = (be - al) / (2 * g);
+  const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+  const double c = 1 / sqrt(1 + t * t), sn = c * t;
+  // columns P, Q:  p' = c p - s e^{-i phi} q,  q' = s e^{i phi} p + c q
+#pragma unroll
+  for (int m = 0; m < 2; m++) {
+    M3 &X = m ? E : a;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const double2 pv = X.e[3 * k + P], qv = X.e[3 * k + Q];
+      X.e[3 * k + P] = make_double2(c * pv.x - sn * (cph * qv.x + sph * qv.y),
+                                    c * pv.y - sn * (cph * qv.y - sph * qv.x));
+      X.e[3 * k + Q] = make_double2(sn * (cph * pv.x - sph * pv.y) + c * qv.x,
+                                    sn * (cph * pv.y + sph * pv.x) + c * qv.y);
+    }
+  }
+}
+
+__global__ __launch_bounds__(TPB) void u3_deriv_kernel(long V, const double2 *Vraw, const double2 *F2,
+                                                       double2 *mid, double filter, int *nfilt) {
+  const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (s >= V) return;
+  M3 v;
+#pragma unroll
+  for (int e = 0; e < 9; e++) v.e[e] = Vraw[9 * s + e];
+  M3 a = v, E = zero3();
+  E.e[0].x = E.e[4].x = E.e[8].x = 1.0;
+  for (int sweep = 0; sweep < 40; sweep++) {
+    double worst = 0;
+    jacobi_rot<0, 1>(a, E, worst);
+    jacobi_rot<0, 2>(a, E, worst);
+    jacobi_rot<1, 2>(a, E, worst);
+    if (worst < 1e-16) break;
+  }
+  double l[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    l[j] = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) l[j] = fma(a.e[3 * k + j].x, a.e[3 * k + j].x, fma(a.e[3 * k + j].y, a.e[3 * k + j].y, l[j]));
+  }
+  if (filter > 0 && (l[0] < filter || l[1] < filter || l[2] < filter)) {
+    l[0] += filter;
+    l[1] += filter;
+    l[2] += filter;
+    atomicAdd(nfilt, 1);
+  }
+  double sq[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) sq[j] = sqrt(l[j]);
+  const M3 F = ld(F2, V, (unsigned)s);
+  // Q^-1/2 = E diag(1/s) E^+
+  M3 t = E;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      t.e[3 * i + j].x /= sq[j];
+      t.e[3 * i + j].y /= sq[j];
+    }
+  const M3 Qm = mm<false, true>(t, E);
+  M3 Mp = mm<false, false>(mm<true, false>(E, mm<true, false>(F, v)), E);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const double gm = -1.0 / (sq[i] * sq[j] * (sq[i] + sq[j]));
+      Mp.e[3 * i + j].x *= gm;
+      Mp.e[3 * i + j].y *= gm;
+    }
+  const M3 N = mm<false, true>(mm<false, false>(E, Mp), E);
+  M3 D = N;
+  axpy<true>(D, 1.0, N);
+  M3 Y = mm<false, true>(Qm, F);
+  axpy<false>(Y, 1.0, mm<false, true>(D, v));
+  // mid = Y^+
+  M3 o;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) o.e[3 * i + j] = cj(Y.e[3 * j + i]);
+  st(mid, V, (unsigned)s, o);
 }
 
 inline dim3 grid(long n) { return dim3((unsigned)((n + TPB - 1) / TPB)); }
@@ -456,102 +579,131 @@ inline Sched make_sched(const Lattice *lat) {
 constexpr double STAPLE_FLOPS = 3 * 198 + 216 + 18;
 constexpr double SIDE_FORCE_FLOPS = 7 * 198 + 3 * 216 + 3 * 18;
 
-void asqtad_force_dev(const Lattice *lat, double *const U_raw[4], double *F_raw[4],
-                      const double coef[6], const double *eps, const DevVec *const *xs, int n,
-                      double *flops, hipStream_t st) {
-  const long V = lat->V;
-  const Geom &g = lat->g;
-  const long n9 = 9 * V;
-  const bool naik = coef[5] != 0;
+namespace {
+
+// One force evaluation: launch geometry, site-innermost workspace and the
+// device table of vector pointers and eps.  nfields: fields besides the
+// derivative scratch (stpl3[4], mid5[4], stpl5, mid3).
+class ForceRun {
+ public:
+  ForceRun(const Lattice *lat, int nfields, const double *eps, const DevVec *const *xs, int n,
+           hipStream_t st)
+      : lat_(lat), g_(lat->g), V_(lat->V), n9_(9 * lat->V), st_(st), sq_(make_sched(lat)),
+        b_(TPB), gs_(grid(lat->V)), ge_(grid(9 * lat->V)), gx_(8 * sq_.per) {
+    if (V_ >= (1L << 27))
+      throw std::runtime_error("fermion force: lattice volume of 2^27 sites or more (32-bit offsets)");
+    prec_ = n > 0 ? xs[0]->prec : P64;
+    for (int t = 0; t < n; t++)
+      if (xs[t]->prec != prec_ || xs[t]->lat != lat || (prec_ != P64 && prec_ != P32))
+        throw std::runtime_error("fermion force: vectors must share the lattice and be FP64 or FP32");
+    dev_alloc(&buf_.p, (size_t)(nfields + 10) * n9_ * sizeof(double2));
+    next_ = buf_.p;
+    for (auto &q : stpl3_) q = take();
+    for (auto &q : mid5_) q = take();
+    stpl5_ = take();
+    mid3_ = take();
+    const size_t ntab = (size_t)2 * n + 1;
+    dev_alloc(&tab_.p, ntab * sizeof(double));
+    std::vector<double> h(ntab);
+    for (int t = 0; t < n; t++) {
+      const void *d = xs[t]->data;
+      memcpy(&h[t], &d, sizeof d);
+      h[n + t] = eps[t];
+    }
+    QOP_CHECK_HIP(hipMemcpy(tab_.p, h.data(), ntab * sizeof(double), hipMemcpyHostToDevice));
+    n_ = n;
+  }
+  double2 *take() {
+    double2 *r = next_;
+    next_ += n9_;
+    return r;
+  }
+  double flops_per_site = 0;
+
+  void to_soa(const double *raw, double2 *f) {
+    hipLaunchKernelGGL(to_soa_kernel, gs_, b_, 0, st_, (const double2 *)raw, f, V_);
+  }
+  // o = co o + ca A + cb B
+  void lincomb(double2 *o, double co, double ca, const double2 *A, double cb, const double2 *B) {
+    hipLaunchKernelGGL(lincomb_kernel, ge_, b_, 0, st_, o, co, ca, A, cb, B, n9_);
+  }
+  // mid fields of terms [t0, t0 + cnt): nmid = 4 (one-hop) or 8 (and three-hop)
+  void mids(int t0, int cnt, int nmid, double scale, double2 *mid) {
+    const double *tab = tab_.p;
+    if (prec_ == P64)
+      hipLaunchKernelGGL(mid_kernel<double2>, gx_, b_, 0, st_, g_, sq_, V_,
+                         (const double2 *const *)(tab + t0), tab + n_ + t0, cnt, nmid, scale, mid);
+    else
+      hipLaunchKernelGGL(mid_kernel<float2>, gx_, b_, 0, st_, g_, sq_, V_,
+                         (const float2 *const *)(tab + t0), tab + n_ + t0, cnt, nmid, scale, mid);
+    flops_per_site += (54 + 36) * 4.0 * cnt * (nmid / 4);
+  }
+  // QOP_asqtad_deriv (lib/asqtad_force_fnmat_p.c:201-464): D += the
+  // derivative of the smearing of the links Ul with coefficients coef for
+  // the mids (midN: three-hop mids, used when coef[5] != 0).
+  void deriv(double2 *const Ul[4], double2 *D[4], const double coef[6], double2 *const mid[4],
+             double2 *const midN[4]);
+  // force (raw) += fs TA(Ul D^+)
+  void final(double *F_raw[4], double2 *const Ul[4], double2 *const D[4], double fs) {
+    for (int mu = 0; mu < 4; mu++)
+      hipLaunchKernelGGL(force_kernel, gs_, b_, 0, st_, V_, (double2 *)F_raw[mu], (const double2 *)Ul[mu],
+                         (const double2 *)D[mu], fs);
+    flops_per_site += 4 * (198 + 24 + 18);
+  }
+  void u3_deriv(const double *Vraw, const double2 *F2, double2 *mid, double filter, int *nfilt) {
+    hipLaunchKernelGGL(u3_deriv_kernel, gs_, b_, 0, st_, V_, (const double2 *)Vraw, F2, mid, filter, nfilt);
+  }
+  void finish() {
+    QOP_CHECK_HIP(hipGetLastError());
+    QOP_CHECK_HIP(hipStreamSynchronize(st_));
+  }
+  long V() const { return V_; }
+
+ private:
+  void staple(double2 *out, int acc, double sc, const double2 *I, const double2 *K, int mu, int nu,
+              double te, const double2 *E) {
+    if (acc && E)
+      hipLaunchKernelGGL((staple_kernel<true, true>), gx_, b_, 0, st_, g_, sq_, V_, out, sc, I, K, mu, nu, te, E);
+    else if (acc)
+      hipLaunchKernelGGL((staple_kernel<true, false>), gx_, b_, 0, st_, g_, sq_, V_, out, sc, I, K, mu, nu, te, E);
+    else if (E)
+      hipLaunchKernelGGL((staple_kernel<false, true>), gx_, b_, 0, st_, g_, sq_, V_, out, sc, I, K, mu, nu, te, E);
+    else
+      hipLaunchKernelGGL((staple_kernel<false, false>), gx_, b_, 0, st_, g_, sq_, V_, out, sc, I, K, mu, nu, te, E);
+  }
+  void side_force(double2 *force, Lin bot, const double2 *side, const double2 *top, int mu, int nu,
+                  double2 *stpl, int acc) {
+    if (acc && bot.b)
+      hipLaunchKernelGGL((side_force_kernel<true, true>), gx_, b_, 0, st_, g_, sq_, V_, force, stpl, bot, side, top, mu, nu);
+    else if (acc)
+      hipLaunchKernelGGL((side_force_kernel<true, false>), gx_, b_, 0, st_, g_, sq_, V_, force, stpl, bot, side, top, mu, nu);
+    else if (bot.b)
+      hipLaunchKernelGGL((side_force_kernel<false, true>), gx_, b_, 0, st_, g_, sq_, V_, force, stpl, bot, side, top, mu, nu);
+    else
+      hipLaunchKernelGGL((side_force_kernel<false, false>), gx_, b_, 0, st_, g_, sq_, V_, force, stpl, bot, side, top, mu, nu);
+  }
+
+  const Lattice *lat_;
+  Geom g_;
+  long V_, n9_;
+  hipStream_t st_;
+  Sched sq_;
+  dim3 b_, gs_, ge_, gx_;
+  int prec_ = P64, n_ = 0;
+  DevBuf<double2> buf_;
+  DevBuf<double> tab_;
+  double2 *next_ = nullptr;
+  double2 *stpl3_[4] = {}, *mid5_[4] = {}, *stpl5_ = nullptr, *mid3_ = nullptr;
+};
+
+void ForceRun::deriv(double2 *const U[4], double2 *D[4], const double coef[6], double2 *const mid[4],
+                     double2 *const midN[4]) {
   double coef1 = coef[0], coef3 = coef[1], coef5 = coef[2], coef7 = coef[3], coefL = coef[4];
   coef1 -= 6 * coefL;
   const bool have5 = coef5 != 0 || coef7 != 0 || coefL != 0;
   const bool have3 = coef3 != 0 || have5;
-  if (V >= (1L << 27))
-    throw std::runtime_error("asqtad force: lattice volume of 2^27 sites or more (32-bit offsets)");
-  const int prec = xs[0]->prec;
-  for (int t = 0; t < n; t++)
-    if (xs[t]->prec != prec || xs[t]->lat != lat || (prec != P64 && prec != P32))
-      throw std::runtime_error("asqtad force: vectors must share the lattice and be FP64 or FP32");
-
-  // site-innermost fields: U[4], mid[4 or 8], deriv[4], stpl3[4], mid5[4], stpl5, mid3
-  const int nmid = naik ? 8 : 4;
-  const int nf = 4 + nmid + 4 + (have5 ? 8 : 0) + 2;
-  DevBuf<double2> buf;
-  dev_alloc(&buf.p, (size_t)nf * n9 * sizeof(double2));
-  double2 *p = buf.p;
-  auto take = [&] { double2 *r = p; p += n9; return r; };
-  double2 *U[4], *mid[8] = {}, *D[4], *stpl3[4] = {}, *mid5[4] = {};
-  for (auto &u : U) u = take();
-  for (int i = 0; i < nmid; i++) mid[i] = take();
-  for (auto &d : D) d = take();
-  if (have5) {
-    for (auto &q : stpl3) q = take();
-    for (auto &q : mid5) q = take();
-  }
-  double2 *stpl5 = take(), *mid3 = take();
-
-  // vector pointers, eps and the U/mid pointer tables of the Lepage kernel
-  const size_t ntab = (size_t)n * 2 + 8;
-  DevBuf<double> tab;
-  dev_alloc(&tab.p, ntab * sizeof(double));
-  std::vector<double> htab(ntab);
-  for (int t = 0; t < n; t++) {
-    const void *d = xs[t]->data;
-    memcpy(&htab[t], &d, sizeof d);
-    htab[n + t] = eps[t];
-  }
-  for (int i = 0; i < 4; i++) {
-    memcpy(&htab[2 * n + i], &U[i], sizeof U[i]);
-    memcpy(&htab[2 * n + 4 + i], &mid[i], sizeof mid[i]);
-  }
-  QOP_CHECK_HIP(hipMemcpyAsync(tab.p, htab.data(), ntab * sizeof(double), hipMemcpyHostToDevice, st));
-  const double *d_eps = tab.p + n;
-  const double2 *const *d_U = (const double2 *const *)(tab.p + 2 * n);
-  const double2 *const *d_mid = (const double2 *const *)(tab.p + 2 * n + 4);
-
-  double nfl = 0;  // per site
-  const Sched sq = make_sched(lat);
-  const dim3 b(TPB), gs = grid(V), ge = grid(n9), gx(8 * sq.per);
-  for (int mu = 0; mu < 4; mu++) {
-    hipLaunchKernelGGL(to_soa_kernel, gs, b, 0, st, (const double2 *)U_raw[mu], U[mu], V);
-    hipLaunchKernelGGL(lincomb_kernel, ge, b, 0, st, D[mu], 0.0, 0.0, (const double2 *)nullptr, 0.0,
-                       (const double2 *)nullptr, n9);
-  }
-  if (prec == P64)
-    hipLaunchKernelGGL(mid_kernel<double2>, gx, b, 0, st, g, sq, V, (const double2 *const *)tab.p, d_eps, n,
-                       nmid, mid[0]);
-  else
-    hipLaunchKernelGGL(mid_kernel<float2>, gx, b, 0, st, g, sq, V, (const float2 *const *)tab.p, d_eps, n,
-                       nmid, mid[0]);
-  nfl += (54 + 36) * 4.0 * n * (naik ? 2 : 1);
-
-  auto staple = [&](double2 *out, int acc, double sc, const double2 *I, const double2 *K, int mu,
-                    int nu, double te, const double2 *E) {
-    if (acc && E)
-      hipLaunchKernelGGL((staple_kernel<true, true>), gx, b, 0, st, g, sq, V, out, sc, I, K, mu, nu, te, E);
-    else if (acc)
-      hipLaunchKernelGGL((staple_kernel<true, false>), gx, b, 0, st, g, sq, V, out, sc, I, K, mu, nu, te, E);
-    else if (E)
-      hipLaunchKernelGGL((staple_kernel<false, true>), gx, b, 0, st, g, sq, V, out, sc, I, K, mu, nu, te, E);
-    else
-      hipLaunchKernelGGL((staple_kernel<false, false>), gx, b, 0, st, g, sq, V, out, sc, I, K, mu, nu, te, E);
-  };
-  auto side_force = [&](double2 *force, Lin bot, const double2 *side, const double2 *top, int mu,
-                        int nu, double2 *stpl, int acc) {
-    if (acc && bot.b)
-      hipLaunchKernelGGL((side_force_kernel<true, true>), gx, b, 0, st, g, sq, V, force, stpl, bot, side, top, mu, nu);
-    else if (acc)
-      hipLaunchKernelGGL((side_force_kernel<true, false>), gx, b, 0, st, g, sq, V, force, stpl, bot, side, top, mu, nu);
-    else if (bot.b)
-      hipLaunchKernelGGL((side_force_kernel<false, true>), gx, b, 0, st, g, sq, V, force, stpl, bot, side, top, mu, nu);
-    else
-      hipLaunchKernelGGL((side_force_kernel<false, false>), gx, b, 0, st, g, sq, V, force, stpl, bot, side, top, mu, nu);
-  };
-  auto lincomb = [&](double2 *o, double co, double ca, const double2 *A, double cb, const double2 *B) {
-    hipLaunchKernelGGL(lincomb_kernel, ge, b, 0, st, o, co, ca, A, cb, B, n9);
-  };
-
+  double2 **stpl3 = stpl3_, **mid5 = mid5_, *stpl5 = stpl5_, *mid3 = mid3_;
+  double &nfl = flops_per_site;
   // QOP_fat_deriv (lib/asqtad_force_fnmat_p.c:236-316)
   for (int sig = 0; sig < 4; sig++) {
     if (have5) {
@@ -585,8 +737,8 @@ void asqtad_force_dev(const Lattice *lat, double *const U_raw[4], double *F_raw[
           side_force(D[rho], Lin{mid[sig], 1.0, nullptr}, U[rho], stpl5, sig, rho, mid3, 0);
           nfl += SIDE_FORCE_FLOPS;
           if (coefL != 0) nfl += 36;
-          hipLaunchKernelGGL(mix_kernel, ge, b, 0, st, coefL != 0 ? mid5[rho] : nullptr, coefL, mid3,
-                             coef7, coef5, (const double2 *)mid[sig], n9);
+          hipLaunchKernelGGL(mix_kernel, ge_, b_, 0, st_, coefL != 0 ? mid5[rho] : nullptr, coefL, mid3,
+                             coef7, coef5, (const double2 *)mid[sig], n9_);
         } else {
           lincomb(mid3, 0.0, coef5, mid[sig], 0.0, nullptr);
         }
@@ -618,27 +770,97 @@ void asqtad_force_dev(const Lattice *lat, double *const U_raw[4], double *F_raw[
       lincomb(D[sig], 1.0, have3 ? 1.0 : 0.0, have3 ? mid3 : nullptr, coef1, coef1 != 0 ? mid[sig] : nullptr);
   }
   if (coefL != 0) {
+    // Lepage fix-up (:317-339)
+    const Ptr4 Up = {{U[0], U[1], U[2], U[3]}}, Mp = {{mid[0], mid[1], mid[2], mid[3]}};
     for (int mu = 0; mu < 4; mu++) {
-      hipLaunchKernelGGL(lepage_acc_kernel, gx, b, 0, st, g, sq, V, mid3, d_U, d_mid, mu);
-      hipLaunchKernelGGL(lepage_apply_kernel, gx, b, 0, st, g, sq, V, D[mu], (const double2 *)U[mu],
+      hipLaunchKernelGGL(lepage_acc_kernel, gx_, b_, 0, st_, g_, sq_, V_, mid3, Up, Mp, mu);
+      hipLaunchKernelGGL(lepage_apply_kernel, gx_, b_, 0, st_, g_, sq_, V_, D[mu], (const double2 *)U[mu],
                          (const double2 *)mid3, -coefL, mu);
     }
     nfl += 4 * (3 * (2 * 198 + 3 * 18) + 198 + 216 + 36);
   }
   // Naik part (lib/asqtad_force_fnmat_p.c:395-445)
-  if (naik) {
+  if (coef[5] != 0 && midN) {
     for (int mu = 0; mu < 4; mu++)
-      hipLaunchKernelGGL(naik_deriv_kernel, gx, b, 0, st, g, sq, V, D[mu], (const double2 *)U[mu],
-                         (const double2 *)mid[4 + mu], coef[5], mu);
+      hipLaunchKernelGGL(naik_deriv_kernel, gx_, b_, 0, st_, g_, sq_, V_, D[mu], (const double2 *)U[mu],
+                         (const double2 *)midN[mu], coef[5], mu);
     nfl += 4 * (4 * 198 + 216 + 18 + 36);
   }
-  for (int mu = 0; mu < 4; mu++)
-    hipLaunchKernelGGL(force_kernel, gs, b, 0, st, V, (double2 *)F_raw[mu], (const double2 *)U[mu],
-                       (const double2 *)D[mu]);
-  nfl += 4 * (198 + 24 + 18);
-  QOP_CHECK_HIP(hipGetLastError());
-  QOP_CHECK_HIP(hipStreamSynchronize(st));
-  if (flops) *flops = nfl * (double)V;
+}
+
+}  // namespace
+
+void asqtad_force_dev(const Lattice *lat, double *const U_raw[4], double *F_raw[4],
+                      const double coef[6], const double *eps, const DevVec *const *xs, int n,
+                      double *flops, hipStream_t st) {
+  const bool naik = coef[5] != 0;
+  const int nmid = naik ? 8 : 4;
+  ForceRun r(lat, 4 + nmid + 4, eps, xs, n, st);
+  double2 *U[4], *mid[8] = {}, *D[4];
+  for (auto &u : U) u = r.take();
+  for (int i = 0; i < nmid; i++) mid[i] = r.take();
+  for (auto &d : D) d = r.take();
+  for (int mu = 0; mu < 4; mu++) {
+    r.to_soa(U_raw[mu], U[mu]);
+    r.lincomb(D[mu], 0.0, 0.0, nullptr, 0.0, nullptr);
+  }
+  r.mids(0, n, nmid, -1.0, mid[0]);
+  r.deriv(U, D, coef, mid, naik ? mid + 4 : nullptr);
+  r.final(F_raw, U, D, -2.0);
+  r.finish();
+  if (flops) *flops = r.flops_per_site * (double)r.V();
+}
+
+// QOP_hisq_force_multi (lib/hisq_force_p.c:63-83 -> hisq_force_fnmat_p.c:7-231)
+void hisq_force_dev(const Lattice *lat, double *const U_raw[4], double *const V_raw[4],
+                    double *const W_raw[4], double *F_raw[4], const double fat7[6],
+                    const double lev2[6], const double diff[6], int n_naiks, const double *eps_naik,
+                    const double *eps, const DevVec *const *xs, const int *n_orders, double filter,
+                    double *flops, int *nfilt, hipStream_t st) {
+  int nterms = 0;
+  for (int k = 0; k < n_naiks; k++) nterms += n_orders[k];
+  ForceRun r(lat, 4 + 4 + 8 + 4 + 4, eps, xs, nterms, st);
+  double2 *U[4], *W[4], *mid[8], *F1[4], *F2[4];
+  for (auto &u : U) u = r.take();
+  for (auto &w : W) w = r.take();
+  for (auto &m : mid) m = r.take();
+  for (auto &f : F1) f = r.take();
+  for (auto &f : F2) f = r.take();
+  DevBuf<int> cnt;
+  dev_alloc(&cnt.p, sizeof(int));
+  QOP_CHECK_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int), st));
+  for (int mu = 0; mu < 4; mu++) {
+    r.to_soa(U_raw[mu], U[mu]);
+    r.to_soa(W_raw[mu], W[mu]);
+    r.lincomb(F2[mu], 0.0, 0.0, nullptr, 0.0, nullptr);
+  }
+  // level 2 per epsilon-Naik group (hisq_force_fnmat_p.c:56-118); group 0
+  // takes the mids of all terms
+  int shift = 0;
+  for (int k = 0; k < n_naiks; k++) {
+    const int c = k == 0 ? nterms : n_orders[k];
+    r.mids(shift, c, 8, 1.0, mid[0]);
+    for (int mu = 0; mu < 4; mu++) r.lincomb(F1[mu], 0.0, 0.0, nullptr, 0.0, nullptr);
+    r.deriv(W, F1, k == 0 ? lev2 : diff, mid, mid + 4);
+    const double cm = k == 0 ? 1.0 : eps_naik[k];
+    for (int mu = 0; mu < 4; mu++) r.lincomb(F2[mu], 1.0, cm, F1[mu], 0.0, nullptr);
+    r.flops_per_site += 4 * 36;
+    shift += n_orders[k];
+  }
+  // U(3) projection derivative -> level-1 mids (:141-152)
+  for (int mu = 0; mu < 4; mu++) r.u3_deriv(V_raw[mu], F2[mu], mid[mu], filter, cnt.p);
+  r.flops_per_site += 4 * (198 + 81 * 16);
+  // level 1: fat7 on U (:144-152), deriv = 2 x level 1, force += TA(U deriv^+)
+  for (int mu = 0; mu < 4; mu++) r.lincomb(F1[mu], 0.0, 0.0, nullptr, 0.0, nullptr);
+  double c7[6] = {fat7[0], fat7[1], fat7[2], fat7[3], fat7[4], 0.0};
+  r.deriv(U, F1, c7, mid, nullptr);
+  r.final(F_raw, U, F1, 2.0);
+  r.flops_per_site += 4 * 36;
+  int h = 0;
+  QOP_CHECK_HIP(hipMemcpyAsync(&h, cnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  r.finish();
+  if (nfilt) *nfilt = h;
+  if (flops) *flops = r.flops_per_site * (double)r.V();
 }
 
 }  // namespace qop

@@ -5,11 +5,15 @@
 //   asqtad smearing of W with the level-2 coefficients (fat + Naik), and for
 //   n_naiks > 1 the epsilon-Naik sets fn[i] = L2 + eps_i L3 with L3 the
 //   "difference" coefficients.
-// Options as lib/hisq_links.c:4-77 (QOP_hisq_links_set_opts).  The auxiliary
-// links (U, V, Y, W; want_aux) serve the HISQ force, which this library does
-// not provide, so they are not kept.
+// Options as lib/hisq_links.c:4-77 (QOP_hisq_links_set_opts).  With want_aux
+// (the default) the auxiliary links the HISQ force needs stay on the device:
+// U (the rephased gauge field), V (fat7) and W (its U(3) projection; Y = W
+// for U(3) projection), raw FP64.
+// QOP_{D3,F3}_hisq_force_multi (lib/hisq_force_p.c:63-83) and
+// QOP_hisq_force_set_opts (lib/hisq_force.c:4-52) are here too.
 #include <chrono>
 #include <cstring>
+#include <stdexcept>
 #include <vector>
 
 #include "handles.h"
@@ -22,11 +26,15 @@ struct QOP_D3_FermionLinksHisq_struct {
   int n_naiks = 1;
   std::vector<QOP_D3_FermionLinksAsqtad *> fn;
   QOP_D3_FermionLinksAsqtad *fn_deps = nullptr;
+  qop::Lattice *lat = nullptr;
+  double *aux[3][4] = {};  // U, V, W (want_aux)
 };
 struct QOP_F3_FermionLinksHisq_struct {
   int n_naiks = 1;
   std::vector<QOP_F3_FermionLinksAsqtad *> fn;
   QOP_F3_FermionLinksAsqtad *fn_deps = nullptr;
+  qop::Lattice *lat = nullptr;
+  double *aux[3][4] = {};  // U, V, W (want_aux)
 };
 
 namespace {
@@ -115,14 +123,23 @@ HisqT *hisq_create(QOP_info_t *info, QOP_hisq_coeffs_t *c, GaugeT *gauge) {
   QOP_asqtad_coeffs_t a3 = {c->difference_one_link, 0, 0, 0, 0, c->difference_naik};
 
   // fat7 stage (hisq_links_p.c:118-128); the raw smeared link is the
-  // reference's 2 x internal link
-  Raw4 W;
+  // reference's 2 x internal link.  Without want_aux it is projected in place.
+  const bool aux = g_hisq.want_aux != 0;
+  Raw4 W, Vf, Uc;
   W.alloc(lat);
+  if (aux) {
+    Vf.alloc(lat);
+    Uc.alloc(lat);
+    for (int i = 0; i < 4; i++)
+      QOP_CHECK_HIP(hipMemcpyAsync(Uc.m[i], gauge->U[i], (size_t)lat->V * 18 * sizeof(double),
+                                   hipMemcpyDeviceToDevice, st));
+  }
+  double **V7 = aux ? Vf.m : W.m;
   {
     double k[6];
     coef6(a1, k);
     const double *g[4] = {gauge->U[0], gauge->U[1], gauge->U[2], gauge->U[3]};
-    launch_fatten(lat, g, g, k, W.m, nullptr, st);
+    launch_fatten(lat, g, g, k, V7, nullptr, st);
     flops += smear_flops(a1, lat) + 198.0 * lat->V;
   }
   // U(3) projection in place (hisq_links_p.c:139-150)
@@ -131,7 +148,7 @@ HisqT *hisq_create(QOP_info_t *info, QOP_hisq_coeffs_t *c, GaugeT *gauge) {
     U3Args u;
     u.V = lat->V;
     for (int i = 0; i < 4; i++) {
-      u.in[i] = W.m[i];
+      u.in[i] = V7[i];
       u.out[i] = W.m[i];
     }
     u.allow_svd = g_hisq.reunit_allow_svd;
@@ -164,9 +181,13 @@ HisqT *hisq_create(QOP_info_t *info, QOP_hisq_coeffs_t *c, GaugeT *gauge) {
       for (auto *f : h->fn)
         if (f) { f->L.free_all(); delete f; }
       if (h->fn_deps) { h->fn_deps->L.free_all(); delete h->fn_deps; }
+      for (auto &a : h->aux)
+        for (auto *p : a)
+          if (p) QOP_CHECK_HIP(hipFree(p));
       delete h;
     }
   } owner{h};
+  h->lat = lat;
   h->n_naiks = n_naiks;
   h->fn.assign(n_naiks, nullptr);
   const double *w[4] = {W.m[0], W.m[1], W.m[2], W.m[3]};
@@ -214,6 +235,14 @@ HisqT *hisq_create(QOP_info_t *info, QOP_hisq_coeffs_t *c, GaugeT *gauge) {
   h->fn[0] = links_from_raw_dev<LinksT>(lat, fat2, naik2 ? &lng2 : nullptr);
   fat2.free();
   lng2.free();
+  if (aux) {
+    for (int i = 0; i < 4; i++) {
+      h->aux[0][i] = Uc.m[i];
+      h->aux[1][i] = Vf.m[i];
+      h->aux[2][i] = W.m[i];
+      Uc.m[i] = Vf.m[i] = W.m[i] = nullptr;
+    }
+  }
   W.free();
   sync();
   if (info) {
@@ -232,7 +261,69 @@ void hisq_destroy(HisqT *h, DestroyL destroy) {
   for (auto *f : h->fn)
     if (f) destroy(f);
   if (h->fn_deps) destroy(h->fn_deps);
+  sync();
+  for (auto &a : h->aux)
+    for (auto *p : a)
+      if (p) QOP_CHECK_HIP(hipFree(p));
   delete h;
+}
+
+struct HisqForceOpts {
+  int fnmat_src_min = 1, veclength = 4;  // lib/hisq_force.c:4-9
+  double force_filter = 5e-5;
+} g_hforce;
+
+// QOP_hisq_force_multi (lib/hisq_force_p.c:63-83, hisq_force_fnmat_p.c:193-231)
+template <class HisqT, class FT, class VT, typename R>
+void hisq_force_multi(QOP_info_t *info, HisqT *flh, FT *force, QOP_hisq_coeffs_t *c, const R eps[],
+                      VT *in_pt[], int *n_orders_naik) {
+  const int n_naiks = c->n_naiks < 1 ? 1 : c->n_naiks;
+  int nterms = 0;
+  for (int k = 0; k < n_naiks; k++) nterms += n_orders_naik[k];
+  if (!flh->aux[0][0]) {
+    fprintf(stderr, "QOP-MI355X: hisq_force_multi: the HISQ links were built without want_aux\n");
+    info->status = QOP_FAIL;
+    return;
+  }
+  if (c->umethod != QOP_UNITARIZE_RATIONAL) {
+    // lib/hisq_force_fnmat_p.c:130-169: NONE differentiates the fat7 links
+    // directly; only the rational (U(3) projection) form is built here
+    fprintf(stderr, "QOP-MI355X: hisq_force_multi: unitarize method %d not supported\n", (int)c->umethod);
+    info->status = QOP_FAIL;
+    return;
+  }
+  if (nterms <= 0) {
+    info->final_sec = info->final_flop = 0;
+    info->status = QOP_SUCCESS;
+    return;
+  }
+  const double fat7[6] = {c->fat7_one_link, c->fat7_three_staple, c->fat7_five_staple,
+                          c->fat7_seven_staple, g_hisq.use_fat7_lepage ? c->fat7_lepage : 0.0, 0.0};
+  const double lev2[6] = {c->asqtad_one_link, c->asqtad_three_staple, c->asqtad_five_staple,
+                          c->asqtad_seven_staple, c->asqtad_lepage, c->asqtad_naik};
+  const double diff[6] = {c->difference_one_link, 0, 0, 0, 0, c->difference_naik};
+  std::vector<double> e(eps, eps + nterms);
+  std::vector<const DevVec *> xs(nterms);
+  for (int i = 0; i < nterms; i++) xs[i] = &in_pt[i]->v;
+  double flop = 0;
+  int nfilt = 0;
+  const double t0 = now();
+  try {
+    if (!guard_oom("QOP_hisq_force_multi", info, [&] {
+          hisq_force_dev(flh->lat, flh->aux[0], flh->aux[1], flh->aux[2], force->F, fat7, lev2, diff,
+                         n_naiks, c->eps_naik, e.data(), xs.data(), n_orders_naik,
+                         g_hforce.force_filter, &flop, &nfilt, stream());
+        }))
+      return;
+  } catch (const std::runtime_error &err) {
+    fprintf(stderr, "QOP-MI355X: %s\n", err.what());
+    info->status = QOP_FAIL;
+    return;
+  }
+  info->final_sec = now() - t0;
+  info->final_flop = flop;
+  info->count2 += nfilt;  // QOP_info_hisq_force_filter_counter
+  info->status = QOP_SUCCESS;
 }
 
 }  // namespace
@@ -285,6 +376,32 @@ QOP_D3_FermionLinksAsqtad *QOP_D3_get_asqtad_deps_links_from_hisq(QOP_D3_Fermion
 }
 QOP_F3_FermionLinksAsqtad *QOP_F3_get_asqtad_deps_links_from_hisq(QOP_F3_FermionLinksHisq *h) {
   return h ? h->fn_deps : nullptr;
+}
+
+// lib/hisq_force.c:29-52: every nsrc takes the fnmat form here
+QOP_status_t QOP_hisq_force_set_opts(QOP_opt_t opts[], int nopts) {
+  int fsm = g_hforce.fnmat_src_min, vl = g_hforce.veclength;
+  double ff = g_hforce.force_filter;
+  for (int i = 0; i < nopts; i++) {
+    if (!strcmp(opts[i].tag, "fnmat_src_min")) fsm = (int)opts[i].value;
+    if (!strcmp(opts[i].tag, "veclength")) vl = (int)opts[i].value;
+    if (!strcmp(opts[i].tag, "force_filter")) ff = opts[i].value;
+  }
+  if (fsm < 0 || vl <= 0 || !(ff >= 0)) return QOP_FAIL;
+  g_hforce.fnmat_src_min = fsm;
+  g_hforce.veclength = vl;
+  g_hforce.force_filter = ff;
+  return QOP_SUCCESS;
+}
+void QOP_D3_hisq_force_multi(QOP_info_t *info, QOP_D3_FermionLinksHisq *flh, QOP_D3_Force *force,
+                             QOP_hisq_coeffs_t *coef, QOP_D_Real eps[], QOP_D3_ColorVector *in_pt[],
+                             int *n_orders_naik) {
+  hisq_force_multi(info, flh, force, coef, eps, in_pt, n_orders_naik);
+}
+void QOP_F3_hisq_force_multi(QOP_info_t *info, QOP_F3_FermionLinksHisq *flh, QOP_F3_Force *force,
+                             QOP_hisq_coeffs_t *coef, QOP_F_Real eps[], QOP_F3_ColorVector *in_pt[],
+                             int *n_orders_naik) {
+  hisq_force_multi(info, flh, force, coef, eps, in_pt, n_orders_naik);
 }
 
 }  // extern "C"
