@@ -24,6 +24,7 @@
  *   or_plaquette          examples/test_common.c:171-228   unit gauge -> 1; gauge invariance
  */
 #include "qop_oracle.h"
+#include <complex.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -937,6 +938,12 @@ static void solve_multi(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t **ra,
 /* ------------------------------------------------------------------ */
 /* QOP_asqtad_invert (asqtad_invert_p.c:649-851)                      */
 /* ------------------------------------------------------------------ */
+static void invert_eigcg(linop_ctx *lc, or_invert_arg_t *ia, or_resid_arg_t *ra, double *out,
+                         const double *in, double *p, or_eigcg_t *eg);
+
+static void invert_eo(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t *ra, double mass,
+                      double *out, const double *in, double *final_flop, or_eigcg_t *eg);
+
 void or_invert(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t *ra,
                double mass, double *out, const double *in, double *final_flop) {
   if (ia->evenodd != OR_EVENODD) {
@@ -945,6 +952,19 @@ void or_invert(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t *ra,
     solve_multi(l, ia, rap, &mass, outp, inp, 1, final_flop);
     return;
   }
+  invert_eo(l, ia, ra, mass, out, in, final_flop, NULL);
+}
+
+/* The EVENODD invert with QOP_asqtad.cgtype = 1: eigCG for the inner solves
+ * (asqtad_invert_p.c:683-687, 767-769), state kept in eg across calls as
+ * the reference keeps it in the links object. */
+void or_invert_eigcg(or_links_t *l, or_eigcg_t *eg, or_invert_arg_t *ia, or_resid_arg_t *ra,
+                     double mass, double *out, const double *in, double *final_flop) {
+  invert_eo(l, ia, ra, mass, out, in, final_flop, eg);
+}
+
+static void invert_eo(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t *ra, double mass,
+                      double *out, const double *in, double *final_flop, or_eigcg_t *eg) {
   double nflop = 0.5 * (20 * 3 + 2 * 8 * 9 * l->nlinks);
   int ineo = ia->evenodd, iter = 0, nrestart = -1;
   int max_iter_old = ia->max_iter, max_restarts_old = ia->max_restarts;
@@ -978,7 +998,8 @@ void or_invert(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t *ra,
   ra->relmin = 0.99 * relminold;
   ia->max_restarts = 0;
   do {
-    invert_cg(&lc, ia, ra, qout, qin, cgp);
+    if (eg) invert_eigcg(&lc, ia, ra, qout, qin, cgp, eg);
+    else invert_cg(&lc, ia, ra, qout, qin, cgp);
     iter += ra->final_iter;
     /* reconstruct: qout_o = (qin_o - D qout_e)/mass on opp(cgeo) (:91-100) */
     int oe = oppsub(cgeo);
@@ -1101,4 +1122,480 @@ void or_invert_multi(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t *ra,
     }
   if (final_flop) *final_flop = flop;
   free(cgp);
+}
+
+/* ------------------------------------------------------------------ */
+/* eigCG (lib/invert_eigcg_p.c:1-1070, Stathopoulos and Orginos,       */
+/* arXiv:0707.0131).  The small dense eigenproblems the reference     */
+/* hands to LAPACK (dsyev, zheev: all eigenpairs, ascending) are       */
+/* solved here by cyclic Jacobi; eigenvectors are defined up to sign   */
+/* or phase, which the algorithm does not depend on.                   */
+/* ------------------------------------------------------------------ */
+struct or_eigcg_s {
+  int numax, m, nev, nu, nv, nn, addvecs, cap, eo;
+  double **u;
+  double *l;
+};
+
+or_eigcg_t *or_eigcg_create(void) {
+  or_eigcg_t *eg = calloc(1, sizeof(or_eigcg_t));
+  eg->eo = -1;
+  return eg;
+}
+void or_eigcg_destroy(or_eigcg_t *eg) {
+  if (!eg) return;
+  if (eg->u) {
+    for (int i = 0; i < eg->cap; i++) free(eg->u[i]);
+    free(eg->u);
+    free(eg->l);
+  }
+  free(eg);
+}
+void or_eigcg_set(or_eigcg_t *eg, int numax, int m, int nev) {
+  eg->numax = numax;
+  eg->m = m;
+  eg->nev = nev;
+}
+int or_eigcg_nu(const or_eigcg_t *eg, double *l, int maxn) {
+  for (int i = 0; i < eg->nu && i < maxn; i++) l[i] = eg->l[i];
+  return eg->nu;
+}
+
+/* real symmetric a (n x n row-major, destroyed): w ascending, z columns */
+static void jacobi_sym(int n, double *a, double *w, double *z) {
+  memset(z, 0, (size_t)n * n * sizeof(double));
+  for (int i = 0; i < n; i++) z[i * n + i] = 1;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0, tot = 0;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        tot += a[i * n + j] * a[i * n + j];
+        if (i != j) off += a[i * n + j] * a[i * n + j];
+      }
+    if (off <= 1e-30 * tot || off == 0) break;
+    for (int p = 0; p < n - 1; p++)
+      for (int q = p + 1; q < n; q++) {
+        double apq = a[p * n + q];
+        if (apq == 0) continue;
+        double theta = (a[q * n + q] - a[p * n + p]) / (2 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+        double c = 1 / sqrt(t * t + 1), sn = t * c;
+        for (int k = 0; k < n; k++) {
+          double akp = a[k * n + p], akq = a[k * n + q];
+          a[k * n + p] = c * akp - sn * akq;
+          a[k * n + q] = sn * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          double apk = a[p * n + k], aqk = a[q * n + k];
+          a[p * n + k] = c * apk - sn * aqk;
+          a[q * n + k] = sn * apk + c * aqk;
+        }
+        for (int k = 0; k < n; k++) {
+          double zkp = z[k * n + p], zkq = z[k * n + q];
+          z[k * n + p] = c * zkp - sn * zkq;
+          z[k * n + q] = sn * zkp + c * zkq;
+        }
+      }
+  }
+  for (int i = 0; i < n; i++) w[i] = a[i * n + i];
+  /* sort ascending (selection, columns follow) */
+  for (int i = 0; i < n; i++) {
+    int k = i;
+    for (int j = i + 1; j < n; j++)
+      if (w[j] < w[k]) k = j;
+    if (k != i) {
+      double t = w[i]; w[i] = w[k]; w[k] = t;
+      for (int r = 0; r < n; r++) {
+        double x = z[r * n + i]; z[r * n + i] = z[r * n + k]; z[r * n + k] = x;
+      }
+    }
+  }
+}
+
+/* complex Hermitian a (n x n row-major, re/im interleaved, destroyed) */
+static void jacobi_herm(int n, double complex *a, double *w, double complex *z) {
+  memset(z, 0, (size_t)n * n * sizeof(double complex));
+  for (int i = 0; i < n; i++) z[i * n + i] = 1;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0, tot = 0;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        double v = creal(a[i * n + j] * conj(a[i * n + j]));
+        tot += v;
+        if (i != j) off += v;
+      }
+    if (off <= 1e-30 * tot || off == 0) break;
+    for (int p = 0; p < n - 1; p++)
+      for (int q = p + 1; q < n; q++) {
+        double complex apq = a[p * n + q];
+        double g = cabs(apq);
+        if (g == 0) continue;
+        double complex ph = apq / g; /* a_pq = g e^{i phi} */
+        double theta = (creal(a[q * n + q]) - creal(a[p * n + p])) / (2 * g);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1));
+        double c = 1 / sqrt(t * t + 1), sn = t * c;
+        /* J: columns p' = c e_p - sn conj(ph) e_q, q' = sn ph e_p + c e_q,
+         * A <- J^+ A J zeroes a_pq */
+        for (int k = 0; k < n; k++) {
+          double complex akp = a[k * n + p], akq = a[k * n + q];
+          a[k * n + p] = c * akp - sn * conj(ph) * akq;
+          a[k * n + q] = sn * ph * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) {
+          double complex apk = a[p * n + k], aqk = a[q * n + k];
+          a[p * n + k] = c * apk - sn * ph * aqk;
+          a[q * n + k] = sn * conj(ph) * apk + c * aqk;
+        }
+        for (int k = 0; k < n; k++) {
+          double complex zkp = z[k * n + p], zkq = z[k * n + q];
+          z[k * n + p] = c * zkp - sn * conj(ph) * zkq;
+          z[k * n + q] = sn * ph * zkp + c * zkq;
+        }
+      }
+  }
+  for (int i = 0; i < n; i++) w[i] = creal(a[i * n + i]);
+  for (int i = 0; i < n; i++) {
+    int k = i;
+    for (int j = i + 1; j < n; j++)
+      if (w[j] < w[k]) k = j;
+    if (k != i) {
+      double t = w[i]; w[i] = w[k]; w[k] = t;
+      for (int r = 0; r < n; r++) {
+        double complex x = z[r * n + i]; z[r * n + i] = z[r * n + k]; z[r * n + k] = x;
+      }
+    }
+  }
+}
+
+typedef struct {
+  int m, nev;
+  double *t, *ta, *tb, *td, *y, *h; /* m x m, m x m, m x m, m, m x 2nev, 2nev x 2nev (row-major) */
+} eg_temps;
+
+/* v'_j = sum_i R(i, j) v_i for j < n2 (R n1 x n2, row stride ld, complex or real) */
+static void rotate_vecs(linop_ctx *lc, double **v, int n1, int n2, const double *Rr,
+                        const double complex *Rz, int ld) {
+  or_links_t *l = lc->l;
+  double **tv = malloc(n2 * sizeof(double *));
+  for (int j = 0; j < n2; j++) {
+    tv[j] = vnew(l);
+    for (int i = 0; i < n1; i++) {
+      double cr = Rr ? Rr[i * ld + j] : creal(Rz[i * ld + j]);
+      double ci = Rr ? 0.0 : cimag(Rz[i * ld + j]);
+      vcaxpy(l, tv[j], cr, ci, v[i], lc->eo);
+    }
+  }
+  for (int j = 0; j < n2; j++) {
+    vcopy(l, v[j], tv[j], lc->eo);
+    free(tv[j]);
+  }
+  free(tv);
+}
+
+static void normalize1(linop_ctx *lc, double *v) {
+  const double r = 1.0 / sqrt(norm2(lc->l, v, lc->eo));
+  long b, e;
+  sub_range(lc->l, lc->eo, &b, &e);
+  for (long i = 6 * b; i < 6 * e; i++) v[i] *= r;
+}
+
+/* invert_eigcg_p.c:201-213 */
+static void orthonormalize_vecs(linop_ctx *lc, double **v, int i0, int n) {
+  for (int i = i0; i < i0 + n; i++) {
+    for (int j = 0; j < i; j++) {
+      double zr, zi;
+      cdot(lc->l, v[j], v[i], lc->eo, &zr, &zi);
+      vcaxpy(lc->l, v[i], -zr, -zi, v[j], lc->eo);
+    }
+    normalize1(lc, v[i]);
+  }
+}
+
+/* invert_eigcg_p.c:215-326 (the evrsq = NULL branch) */
+static void rayleigh_ritz(linop_ctx *lc, double **u, double *lv, int nu, int nv, double *Mp) {
+  int n = nu + nv;
+  if (n <= 0) return;
+  orthonormalize_vecs(lc, u, nu, nv);
+  double complex *mm = calloc((size_t)n * n, sizeof(double complex));
+  double complex *tt = malloc((size_t)n * n * sizeof(double complex));
+  double *vv = malloc(n * sizeof(double));
+  for (int i = 0; i < nu; i++) mm[i * n + i] = lv[i];
+  for (int j = nu; j < n; j++) {
+    lin_plain(lc, Mp, u[j]);
+    for (int i = 0; i < j; i++) {
+      double zr, zi;
+      cdot(lc->l, u[i], Mp, lc->eo, &zr, &zi);
+      mm[i * n + j] = zr + I * zi;
+      mm[j * n + i] = zr - I * zi;
+    }
+    double zr, zi;
+    cdot(lc->l, u[j], Mp, lc->eo, &zr, &zi);
+    mm[j * n + j] = zr + I * zi;
+  }
+  jacobi_herm(n, mm, vv, tt);
+  rotate_vecs(lc, u, n, n, NULL, tt, n);
+  for (int i = 0; i < n; i++) lv[i] = vv[i];
+  if (n == 1) lv[1] = 0;
+  free(mm); free(tt); free(vv);
+}
+
+/* invert_eigcg_p.c:328-341 */
+static void initcg(linop_ctx *lc, double *x, double **u, const double *lv, int n, const double *b) {
+  if (n <= 0) return;
+  vzero(lc->l, x, lc->eo);
+  for (int i = 0; i < n; i++) {
+    double zr, zi;
+    cdot(lc->l, u[i], b, lc->eo, &zr, &zi);
+    vcaxpy(lc->l, x, zr / lv[i], zi / lv[i], u[i], lc->eo);
+  }
+}
+
+/* t(i, j) = Re <v_i, A v_j> (invert_eigcg_p.c:363-394) */
+static void gett(linop_ctx *lc, double *t, int ldt, double **v, int n, double *Mp) {
+  for (int j = 0; j < n; j++) {
+    lin_plain(lc, Mp, v[j]);
+    for (int i = 0; i < n; i++) t[i * ldt + j] = redot(lc->l, v[i], Mp, lc->eo);
+  }
+}
+
+/* td[0:nv], tb[0:nv,0:nv] = eigenpairs of t[0:nv,0:nv] (:488-501) */
+static void diag_t(eg_temps *et, int nv) {
+  int m = et->m;
+  double *a = malloc((size_t)nv * nv * sizeof(double)), *z = malloc((size_t)nv * nv * sizeof(double));
+  for (int i = 0; i < nv; i++)
+    for (int j = 0; j < nv; j++) a[i * nv + j] = et->t[i * m + j];
+  jacobi_sym(nv, a, et->td, z);
+  for (int i = 0; i < nv; i++)
+    for (int j = 0; j < nv; j++) et->tb[i * m + j] = z[i * nv + j];
+  free(a); free(z);
+}
+
+/* invert_eigcg_p.c:503-550 */
+static void set_y(eg_temps *et, int noff, int nev, int nv, int n) {
+  int m = et->m, ly = 2 * et->nev;
+  double *y = et->y;
+  for (int i = 0; i < nv; i++)
+    for (int j = 0; j < nev; j++) y[i * ly + noff + j] = et->tb[i * m + j];
+  for (int i = nv; i < n; i++)
+    for (int j = 0; j < nev; j++) y[i * ly + noff + j] = 0;
+  if (!noff) return;
+  /* dorthocols: columns noff.. against columns 0..noff-1 and each other */
+  for (int c = noff; c < noff + nev; c++) {
+    for (int k = 0; k < c; k++) {
+      double d = 0;
+      for (int i = 0; i < n; i++) d += y[i * ly + k] * y[i * ly + c];
+      for (int i = 0; i < n; i++) y[i * ly + c] -= d * y[i * ly + k];
+    }
+    double s = 0;
+    for (int i = 0; i < n; i++) s += y[i * ly + c] * y[i * ly + c];
+    s = 1 / sqrt(s);
+    for (int i = 0; i < n; i++) y[i * ly + c] *= s;
+  }
+  int k = noff + nev;
+  /* tb1 = t y4 (n x k); h0 = y4^T tb1; eig -> td, ta; tb1 = y4 ta */
+  double *tb1 = calloc((size_t)n * k, sizeof(double));
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < k; j++) {
+      double s2 = 0;
+      for (int q = 0; q < n; q++) s2 += et->t[i * m + q] * y[q * ly + j];
+      tb1[i * k + j] = s2;
+    }
+  double *h0 = calloc((size_t)k * k, sizeof(double)), *ta = malloc((size_t)k * k * sizeof(double));
+  for (int i = 0; i < k; i++)
+    for (int j = 0; j < k; j++) {
+      double s2 = 0;
+      for (int q = 0; q < n; q++) s2 += y[q * ly + i] * tb1[q * k + j];
+      h0[i * k + j] = s2;
+    }
+  jacobi_sym(k, h0, et->td, ta);
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < k; j++) {
+      double s2 = 0;
+      for (int q = 0; q < k; q++) s2 += y[i * ly + q] * ta[q * k + j];
+      et->tb[i * m + j] = s2;
+    }
+  free(tb1); free(h0); free(ta);
+}
+
+/* invert_eigcg_p.c:552-559: v[0:n] = v[0:nv] tb[0:nv, 0:n] */
+static void rot_t(linop_ctx *lc, eg_temps *et, int n, double **v, int nv) {
+  if (n > nv) n = nv;
+  rotate_vecs(lc, v, nv, n, et->tb, NULL, et->m);
+}
+
+static void setv(linop_ctx *lc, double **v, int *nv, const double *r, double rsq) {
+  vzero(lc->l, v[*nv], lc->eo);
+  vaxpy(lc->l, v[*nv], 1 / sqrt(rsq), r, v[*nv], lc->eo);
+  (*nv)++;
+}
+
+/* invert_eigcg_p.c:561-616 */
+static void reset_t(linop_ctx *lc, eg_temps *et, int nev, double **v, int *nvp, double a0, double a,
+                    double b, const double *r, double rsq, double *Mp) {
+  int m = et->m, nv = 2 * nev;
+  setv(lc, v, &nv, r, rsq);
+  memset(et->t, 0, (size_t)m * m * sizeof(double));
+  lin_plain(lc, Mp, v[nv - 1]);
+  for (int i = 0; i < nv - 1; i++) {
+    double rr = redot(lc->l, v[i], Mp, lc->eo);
+    et->t[i * m + nv - 1] = rr;
+    et->t[(nv - 1) * m + i] = rr;
+    et->t[i * m + i] = et->td[i];
+  }
+  et->t[(nv - 1) * m + nv - 1] = 1. / a + b / a0;
+  *nvp = nv;
+}
+
+/* QOP_invert_eigcg (invert_eigcg_p.c:618-1068) */
+static void invert_eigcg(linop_ctx *lc, or_invert_arg_t *ia, or_resid_arg_t *ra, double *out,
+                         const double *in, double *p, or_eigcg_t *eg) {
+  or_links_t *l = lc->l;
+  const int eo = lc->eo;
+  double a = 0, b = 0, a0 = 0, rsq, relnorm2, oldrsq, pkp, insq, rsqstop;
+  int iteration = 0, total = 0, nrestart = -1;
+  int max_iterations = ia->max_iter, max_restarts = ia->max_restarts;
+  if (max_restarts < 0) max_restarts = 5;
+  const int nev = eg->nev, m = eg->m, numax = eg->numax;
+  if (!eg->u) {
+    eg->cap = numax;
+    eg->u = malloc(numax * sizeof(double *));
+    eg->l = calloc(numax + 1, sizeof(double));
+    for (int i = 0; i < numax; i++) eg->u[i] = vnew(l);
+    eg->nu = eg->nv = eg->nn = 0;
+    eg->addvecs = 1;
+  }
+  if (eg->eo != eo) {
+    /* the vectors live on the parity of the solves that made them; a solve
+     * on the other parity starts a new space (the reference would
+     * orthonormalise vectors that are zero there) */
+    eg->nu = eg->nv = eg->nn = 0;
+    eg->addvecs = 1;
+    eg->eo = eo;
+  }
+  double **u = eg->u, *lv = eg->l;
+  int nu = eg->nu, nv = eg->nv, nn = eg->nn, addvecs = eg->addvecs;
+  /* the reference writes the Lanczos vectors to u[nu + nn ..] without a
+   * bound; stop collecting when they would not fit */
+  if (addvecs && nu + nn + m > eg->cap) addvecs = 0;
+  eg_temps et = {m, nev, calloc((size_t)m * m, 8), calloc((size_t)m * m, 8), calloc((size_t)m * m, 8),
+                 calloc(m + 1, 8), calloc((size_t)m * 2 * nev, 8), calloc((size_t)4 * nev * nev, 8)};
+  double *r = vnew(l), *Mp = vnew(l), *Mq = vnew(l);
+  double **v = NULL;
+  insq = norm2(l, in, eo);
+  rsqstop = ra->rsqmin * insq;
+  rsq = 0;
+  relnorm2 = 1.;
+  ra->final_rsq = 0; ra->final_rel = 0; ra->final_iter = 0; ra->final_restart = 0;
+  if (insq == 0.) {
+    vzero(l, out, eo);
+    goto done;
+  }
+  do {
+    vcopy(l, p, out, eo);
+    lin_plain(lc, Mp, p);
+    iteration = 1;
+    total++;
+    vaxpy(l, r, -1.0, Mp, in, eo);
+    rsq = norm2(l, r, eo);
+    if (((rsqstop <= 0 || rsq < rsqstop) && (ra->relmin <= 0 || relnorm2 < ra->relmin)) ||
+        total >= max_iterations)
+      break;
+    if (addvecs && nn >= m) {
+      int nr = numax - 2 * m - 1;
+      if (nr >= nu) nr = nu - 2;
+      if (nr < 0) nr = 0;
+      double oldl = nu == 0 ? 9999 : lv[nr];
+      rayleigh_ritz(lc, u, lv, nu, nn, Mq);
+      nu += nn;
+      nn = 0;
+      double delta = fabs(2 * (oldl - lv[nr]) / (oldl + lv[nr]));
+      if (delta < 1e-4) addvecs = 0;
+      else if (nu > numax - 2 * m) nu = numax - 2 * m;
+    }
+    if (nu > 0) {
+      initcg(lc, p, u, lv, nu, r);
+      vaxpy(l, out, 1.0, p, out, eo);
+      vcopy(l, p, out, eo);
+      lin_plain(lc, Mp, p);
+      iteration = 1;
+      total++;
+      vaxpy(l, r, -1.0, Mp, in, eo);
+      rsq = norm2(l, r, eo);
+      if (ra->relmin > 0) relnorm2 = or_relnorm2(l, r, out, eo);
+      if (((rsqstop <= 0 || rsq < rsqstop) && (ra->relmin <= 0 || relnorm2 < ra->relmin)) ||
+          total >= max_iterations)
+        break;
+    }
+    if (addvecs) {
+      v = u + nu + nn;
+      memset(et.t, 0, (size_t)m * m * sizeof(double));
+    }
+    vcopy(l, p, r, eo);
+    while (1) {
+      lin_plain(lc, Mp, p);
+      iteration++;
+      total++;
+      pkp = redot(l, p, Mp, eo);
+      a0 = a;
+      a = rsq / pkp;
+      if (addvecs) {
+        if (nv == 0) {
+          et.t[0] = 1. / a;
+          setv(lc, v, &nv, r, rsq);
+        } else if (nv < m) {
+          double td = 1. / a + b / a0, ts = -sqrt(b) / a0;
+          et.t[nv * m + nv] = td;
+          et.t[nv * m + nv - 1] = ts;
+          et.t[(nv - 1) * m + nv] = ts;
+          setv(lc, v, &nv, r, rsq);
+        } else {
+          diag_t(&et, nv);
+          set_y(&et, 0, nev, nv, nv);
+          diag_t(&et, nv - 1);
+          set_y(&et, nev, nev, nv - 1, nv);
+          rot_t(lc, &et, 2 * nev, v, nv);
+          reset_t(lc, &et, nev, v, &nv, a0, a, b, r, rsq, Mq);
+        }
+      }
+      vaxpy(l, out, a, p, out, eo);
+      vaxpy(l, r, -a, Mp, r, eo);
+      oldrsq = rsq;
+      rsq = norm2(l, r, eo);
+      if (ra->relmin > 0) relnorm2 = or_relnorm2(l, r, out, eo);
+      if (((rsqstop <= 0 || rsq < rsqstop) && (ra->relmin <= 0 || relnorm2 < ra->relmin)) ||
+          total >= max_iterations)
+        break;
+      b = rsq / oldrsq;
+      vaxpy(l, p, b, p, r, eo);
+    }
+    if (addvecs) {
+      if (nv > nev) {
+        orthonormalize_vecs(lc, v, 0, nv);
+        gett(lc, et.t, m, v, nv, Mq);
+        diag_t(&et, nv);
+        set_y(&et, 0, nev, nv, nv);
+        rot_t(lc, &et, nev, v, nv);
+        nn += nev;
+      }
+      nv = 0;
+    }
+    nrestart++;
+  } while (total < max_iterations && nrestart < max_restarts);
+  if (addvecs) {
+    rayleigh_ritz(lc, u, lv, 0, nu + nn + nv, Mq);
+    nu += nn + nv;
+    nn = 0;
+    nv = 0;
+  }
+  eg->nu = nu;
+  eg->nv = nv;
+  eg->nn = nn;
+  eg->addvecs = addvecs;
+  ra->final_rsq = rsq / insq;
+  ra->final_rel = relnorm2;
+  ra->final_iter = total;
+  ra->final_restart = nrestart;
+done:
+  free(r); free(Mp); free(Mq);
+  free(et.t); free(et.ta); free(et.tb); free(et.td); free(et.y); free(et.h);
 }

@@ -83,6 +83,17 @@ void or_invert(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t *ra,
 void or_invert_multi(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t *ra,
                      const double *masses, const int *nmass, double **out,
                      double *const *in, int nsrc, double *final_flop);
+/* eigCG state (the reference keeps it in the links object:
+ * asqtad_invert_p.c:683-687, invert_eigcg_p.c:618-1068). */
+typedef struct or_eigcg_s or_eigcg_t;
+or_eigcg_t *or_eigcg_create(void);
+void or_eigcg_destroy(or_eigcg_t *eg);
+void or_eigcg_set(or_eigcg_t *eg, int numax, int m, int nev);
+/* number of deflation vectors nu; their Ritz values into l[0:min(nu, maxn)] */
+int or_eigcg_nu(const or_eigcg_t *eg, double *l, int maxn);
+/* QOP_asqtad_invert (EVENODD) with cgtype = 1: eigCG inner solves. */
+void or_invert_eigcg(or_links_t *l, or_eigcg_t *eg, or_invert_arg_t *ia, or_resid_arg_t *ra,
+                     double mass, double *out, const double *in, double *final_flop);
 /* Fermilab relative residual (invert_utilities_p.c:7-40). */
 double or_relnorm2(or_links_t *l, const double *r, const double *x, int eo);
 

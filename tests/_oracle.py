@@ -77,6 +77,13 @@ class Oracle:
                                     _D, _D, C.POINTER(C.c_double)]
             L.or_invert_multi.argtypes = [C.c_void_p, C.POINTER(InvArg), C.POINTER(ResArg),
                                           _D, _IV, _PP, _PP, C.c_int, C.POINTER(C.c_double)]
+            L.or_eigcg_create.restype = C.c_void_p
+            L.or_eigcg_destroy.argtypes = [C.c_void_p]
+            L.or_eigcg_set.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+            L.or_eigcg_nu.restype = C.c_int
+            L.or_eigcg_nu.argtypes = [C.c_void_p, _D, C.c_int]
+            L.or_invert_eigcg.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(InvArg), C.POINTER(ResArg),
+                                          C.c_double, _D, _D, C.POINTER(C.c_double)]
             L.or_relnorm2.restype = C.c_double
             L.or_relnorm2.argtypes = [C.c_void_p, _D, _D, C.c_int]
             L.or_u3_un_analytic.restype = C.c_int
@@ -176,6 +183,30 @@ class Oracle:
         self.lib.or_invert(h, C.byref(ia), C.byref(ra), mass, out.reshape(-1),
                            np.ascontiguousarray(vin).reshape(-1), C.byref(flop))
         return out, ra, flop.value
+
+    def eigcg_state(self):
+        return self.lib.or_eigcg_create()
+
+    def eigcg_free(self, eg):
+        self.lib.or_eigcg_destroy(eg)
+
+    def eigcg_ritz(self, eg, maxn=1000):
+        l = np.zeros(maxn)
+        n = self.lib.or_eigcg_nu(eg, l, maxn)
+        return l[:min(n, maxn)]
+
+    def invert_eigcg(self, h, eg, mass, vin, rsqmin, numax, m, nev, out=None, max_iter=2000,
+                     restart=1000, max_restarts=5):
+        """EVENODD QOP_asqtad_invert with cgtype = 1 (eigCG), state eg kept across calls."""
+        self.lib.or_eigcg_set(eg, numax, m, nev)
+        ia = InvArg(1e9, max_iter, restart, max_restarts, EVENODD)
+        ra = ResArg(rsqmin, 0, 0, 0, 0, 0)
+        if out is None:
+            out = np.zeros_like(vin)
+        flop = C.c_double(0)
+        self.lib.or_invert_eigcg(h, eg, C.byref(ia), C.byref(ra), mass, out.reshape(-1),
+                                 np.ascontiguousarray(vin).reshape(-1), C.byref(flop))
+        return out, ra
 
     def invert_multi(self, h, masses, vins, evenodd, rsqmin, max_iter=2000, restart=1000,
                      max_restarts=5):
