@@ -263,6 +263,9 @@ QOP_status_t QOP_asqtad_invert_set_opts(QOP_opt_t opts[], int nopts) {
   double join = o.ru_join;
   get("qmx_ru_join", join);
   if (!(join >= 0 && join < 1)) return QOP_FAIL;
+  // eigCG sizes (not checked by the reference, lib/asqtad_invert.c:94-107):
+  // the restart keeps 2 nev + 1 Lanczos vectors of m, and the space holds m
+  if (cg == 1 && (nev < 1 || m < 2 * nev + 2 || numax < m)) return QOP_FAIL;
   if (msp != 16 && msp != 32) return QOP_FAIL;
   if (solver < 0 || solver > 1) return QOP_FAIL;
   if (prec != 64 && prec != 32 && prec != 16) return QOP_FAIL;

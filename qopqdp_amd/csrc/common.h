@@ -140,8 +140,11 @@ struct LinkSet {
 void linkset_free(LinkSet &l);
 
 // Link object behind QOP_{D,F}3_FermionLinksAsqtad.
+struct EigcgState;  // eigcg.h
+
 struct Links {
   Lattice *lat = nullptr;
+  EigcgState *eig = nullptr;  // eigCG deflation space (option cg = 1), owned
   int nl = 16;
   LinkSet l64, l32, l16;
   // single-stored forward links (styles 0 and 2: backward links are formed

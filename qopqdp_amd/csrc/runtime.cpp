@@ -4,6 +4,7 @@
 #include <mutex>
 
 #include "common.h"
+#include "eigcg.h"
 
 namespace qop {
 
@@ -154,6 +155,8 @@ void Links::drop_derived() {
 void Links::free_all() {
   linkset_free(l64);
   drop_derived();
+  delete eig;
+  eig = nullptr;
 }
 
 hipStream_t stream() {

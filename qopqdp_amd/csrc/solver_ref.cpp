@@ -14,6 +14,7 @@
 #include <cstring>
 
 #include "solver.h"
+#include "eigcg.h"
 
 namespace qop {
 
@@ -563,7 +564,17 @@ void invert(Links &L, InvArg *ia, ResArg *ra, double mass, DevVec &out, const De
   double dtime = 0;
   do {
     double t0 = wall_time();
-    invert_cg(L, P64, mass, cgeo, ia, ra, qout.v, qin.v);
+    if (options().cgtype == 1) {
+      // eigCG inner solves (asqtad_invert_p.c:683-687, 767-769); the sizes
+      // are taken from the options at every call, as the reference does
+      if (!L.eig) L.eig = new EigcgState;
+      L.eig->numax = options().eigcg_numax;
+      L.eig->m = options().eigcg_m;
+      L.eig->nev = options().eigcg_nev;
+      invert_eigcg(L, mass, cgeo, ia, ra, qout.v, qin.v, *L.eig);
+    } else {
+      invert_cg(L, P64, mass, cgeo, ia, ra, qout.v, qin.v);
+    }
     sync();
     dtime += wall_time() - t0;
     iter += ra->final_iter;
