@@ -188,6 +188,11 @@ void asqtad_force_dev(const Lattice *lat, double *const U_raw[4], double *F_raw[
                       const double coef[6], const double *eps, const DevVec *const *xs, int n,
                       double *flops, hipStream_t st);
 
+// Asqtad fattening of raw FP64 device links (force.hip): fat = smeared gf,
+// lng (optional) = naik gl gl gl; coef as for asqtad_force_dev.
+void fatten_dev(const Lattice *lat, const double *const gf[4], const double *const gl[4],
+                const double coef[6], double *fat[4], double *lng[4], hipStream_t st);
+
 // HISQ fermion force (force.hip): F[mu] += TA(U_mu deriv_mu^+) with deriv
 // through level 2 (W, lev2 or the difference set diff per epsilon-Naik group),
 // the U(3) projection of V and the fat7 level on U (raw FP64 device links);

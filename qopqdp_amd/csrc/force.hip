@@ -445,6 +445,58 @@ __global__ __launch_bounds__(TPB) void force_kernel(long V, double2 *F, const do
   }
 }
 
+// ---------------------------------------------------------------------------
+// Asqtad fattening (lib/asqtad_dslash_p.c:391-633, compute_gen_staple /
+// make_imp_links) on the same site-innermost fields and site order: each
+// generalised staple S(x) = K(x) I(x+nu) K(x+mu)^+ + K(x-nu)^+ I(x-nu) K(x-nu+mu)
+// (I the field along mu, K the link along nu) is one kernel that adds c S
+// to the fat link and, when the next smearing level needs it, stores S.
+template <bool STORE>
+__global__ __launch_bounds__(TPB) void fat_staple_kernel(Geom g, Sched q, long V, double2 *out, double2 *fat,
+                                                         double c, const double2 *I, const double2 *K,
+                                                         int mu, int nu) {
+  const int s0 = site_index(q, g.Vh);
+  if (s0 < 0) return;
+  const unsigned s = (unsigned)s0;
+  const Site x = site_of(g, s);
+  const unsigned xpnu = nbr(g, x, nu, 1, mu, 0), xpmu = nbr(g, x, mu, 1, nu, 0);
+  const unsigned xmnu = nbr(g, x, nu, -1, mu, 0), xmnupmu = nbr(g, x, nu, -1, mu, 1);
+  M3 r = mm<false, false>(ld(K, V, s), mm<false, true>(ld(I, V, xpnu), ld(K, V, xpmu)));
+  {
+    const M3 t = mm<true, false>(ld(K, V, xmnu), ld(I, V, xmnu));
+    axpy<false>(r, 1.0, mm<false, false>(t, ld(K, V, xmnupmu)));
+  }
+  if constexpr (STORE) st(out, V, s, r);
+  M3 f = ld(fat, V, s);
+  axpy<false>(f, c, r);
+  st(fat, V, s, f);
+}
+
+// Naik link L(x) = c U(x) U(x+mu) U(x+2mu) (lib/asqtad_dslash_p.c:600-620)
+__global__ __launch_bounds__(TPB) void naik_link_kernel(Geom g, Sched q, long V, double2 *L, const double2 *U,
+                                                        double c, int mu) {
+  const int s0 = site_index(q, g.Vh);
+  if (s0 < 0) return;
+  const unsigned s = (unsigned)s0;
+  const Site x = site_of(g, s);
+  const M3 t = mm<false, false>(ld(U, V, nbr(g, x, mu, 1, 0, 0)), ld(U, V, nbr(g, x, mu, 2, 0, 0)));
+  M3 o = mm<false, false>(ld(U, V, s), t);
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    o.e[e].x *= c;
+    o.e[e].y *= c;
+  }
+  st(L, V, s, o);
+}
+
+// site-innermost [9][V] -> raw [V][18]
+__global__ __launch_bounds__(TPB) void from_soa_kernel(const double2 *f, double2 *raw, long V) {
+  const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (s >= V) return;
+#pragma unroll
+  for (int e = 0; e < 9; e++) raw[9 * s + e] = f[e * V + s];
+}
+
 // Derivative of the U(3) projection W = V (V^+V)^-1/2 contracted with F2,
 // the derivative accumulated with respect to W (lib/hisq_reunit1_p.c:366-433
 // and the derivative of lib/hisq_reunit_utilities_p.c:301-640, computed here
@@ -861,6 +913,73 @@ void hisq_force_dev(const Lattice *lat, double *const U_raw[4], double *const V_
   r.finish();
   if (nfilt) *nfilt = h;
   if (flops) *flops = r.flops_per_site * (double)r.V();
+}
+
+// Asqtad fattening (lib/asqtad_dslash_p.c:391-633) of raw FP64 device links:
+// fat = coef1' U + sum of staples (coef1' = coef1 - 6 lepage, :531), lng =
+// naik U U U from gl; the order of accumulation is the reference's.
+void fatten_dev(const Lattice *lat, const double *const gf[4], const double *const gl[4],
+                const double c[6], double *fat_raw[4], double *lng_raw[4], hipStream_t st) {
+  const long V = lat->V, n9 = 9 * V;
+  if (V >= (1L << 27))
+    throw std::runtime_error("fattening: lattice volume of 2^27 sites or more (32-bit offsets)");
+  const Geom &g = lat->g;
+  const double coef1 = c[0] - 6 * c[4], coef3 = c[1], coef5 = c[2], coef7 = c[3], coefL = c[4];
+  const bool have5 = coef5 != 0 || coef7 != 0 || coefL != 0;
+  const bool have3 = coef3 != 0 || have5;
+  const bool naik = lng_raw != nullptr;
+  const bool sep_long = naik && (gl[0] != gf[0] || gl[1] != gf[1] || gl[2] != gf[2] || gl[3] != gf[3]);
+  const int nf = 4 + 1 + (have3 ? 2 : 0) + (naik ? 1 : 0) + (sep_long ? 4 : 0);
+  DevBuf<double2> buf;
+  dev_alloc(&buf.p, (size_t)nf * n9 * sizeof(double2));
+  double2 *p = buf.p;
+  auto take = [&] { double2 *r = p; p += n9; return r; };
+  double2 *U[4], *UL[4], *F = take(), *S3 = nullptr, *S5 = nullptr, *Lk = nullptr;
+  for (auto &u : U) u = take();
+  if (have3) {
+    S3 = take();
+    S5 = take();
+  }
+  if (naik) Lk = take();
+  for (int i = 0; i < 4; i++) UL[i] = sep_long ? take() : U[i];
+  const Sched sq = make_sched(lat);
+  const dim3 b(TPB), gs = grid(V), ge = grid(n9), gx(8 * sq.per);
+  for (int mu = 0; mu < 4; mu++) {
+    hipLaunchKernelGGL(to_soa_kernel, gs, b, 0, st, (const double2 *)gf[mu], U[mu], V);
+    if (sep_long) hipLaunchKernelGGL(to_soa_kernel, gs, b, 0, st, (const double2 *)gl[mu], UL[mu], V);
+  }
+  auto add = [&](bool store, double2 *out, double cc, const double2 *I, const double2 *K, int mu, int nu) {
+    if (store)
+      hipLaunchKernelGGL(fat_staple_kernel<true>, gx, b, 0, st, g, sq, V, out, F, cc, I, K, mu, nu);
+    else
+      hipLaunchKernelGGL(fat_staple_kernel<false>, gx, b, 0, st, g, sq, V, out, F, cc, I, K, mu, nu);
+  };
+  for (int dir = 0; dir < 4; dir++) {
+    hipLaunchKernelGGL(lincomb_kernel, ge, b, 0, st, F, 0.0, coef1, (const double2 *)U[dir], 0.0,
+                       (const double2 *)nullptr, n9);
+    if (have3) {
+      for (int nu = 0; nu < 4; nu++) {
+        if (nu == dir) continue;
+        add(true, S3, coef3, U[dir], U[nu], dir, nu);
+        if (coefL != 0) add(false, nullptr, coefL, S3, U[nu], dir, nu);
+        if (coef5 != 0 || coef7 != 0)
+          for (int rho = 0; rho < 4; rho++) {
+            if (rho == dir || rho == nu) continue;
+            add(coef7 != 0, S5, coef5, S3, U[rho], dir, rho);
+            if (coef7 != 0)
+              for (int sig = 0; sig < 4; sig++)
+                if (sig != dir && sig != nu && sig != rho) add(false, nullptr, coef7, S5, U[sig], dir, sig);
+          }
+      }
+    }
+    hipLaunchKernelGGL(from_soa_kernel, gs, b, 0, st, (const double2 *)F, (double2 *)fat_raw[dir], V);
+    if (naik) {
+      hipLaunchKernelGGL(naik_link_kernel, gx, b, 0, st, g, sq, V, Lk, (const double2 *)UL[dir], c[5], dir);
+      hipLaunchKernelGGL(from_soa_kernel, gs, b, 0, st, (const double2 *)Lk, (double2 *)lng_raw[dir], V);
+    }
+  }
+  QOP_CHECK_HIP(hipGetLastError());
+  QOP_CHECK_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace qop

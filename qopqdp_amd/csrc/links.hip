@@ -5,7 +5,8 @@
 //  * extraction back to raw (x2, lib/asqtad_dslash_p.c:915-938);
 //  * staggered phases and boundary phases (lib/common_p.c:464-526,
 //    lib/asqtad_dslash_p.c:1385-1482);
-//  * asqtad fattening (lib/asqtad_dslash_p.c:391-633) on raw-layout matrices.
+//  * asqtad fattening (lib/asqtad_dslash_p.c:391-633) of raw-layout links,
+//    computed by fatten_dev (force.hip) on site-innermost copies.
 #include "common.h"
 #include "geom_d.h"
 #include "kernels.h"
@@ -212,136 +213,11 @@ void launch_rephase(const Lattice *lat, double *U_dev, int mu, const int r0[4],
 }
 
 // ---------------------------------------------------------------------------
-// fattening (compute_gen_staple / make_imp_links)
-__device__ __forceinline__ void mmul(double *c, const double *a, const double *b) {
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) {
-      double re = 0, im = 0;
-      for (int k = 0; k < 3; k++) {
-        re += a[6 * i + 2 * k] * b[6 * k + 2 * j] - a[6 * i + 2 * k + 1] * b[6 * k + 2 * j + 1];
-        im += a[6 * i + 2 * k] * b[6 * k + 2 * j + 1] + a[6 * i + 2 * k + 1] * b[6 * k + 2 * j];
-      }
-      c[6 * i + 2 * j] = re; c[6 * i + 2 * j + 1] = im;
-    }
-}
-__device__ __forceinline__ void mmul_na(double *c, const double *a, const double *b) {
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) {
-      double re = 0, im = 0;
-      for (int k = 0; k < 3; k++) {
-        re += a[6 * i + 2 * k] * b[6 * j + 2 * k] + a[6 * i + 2 * k + 1] * b[6 * j + 2 * k + 1];
-        im += -a[6 * i + 2 * k] * b[6 * j + 2 * k + 1] + a[6 * i + 2 * k + 1] * b[6 * j + 2 * k];
-      }
-      c[6 * i + 2 * j] = re; c[6 * i + 2 * j + 1] = im;
-    }
-}
-__device__ __forceinline__ void mmul_an(double *c, const double *a, const double *b) {
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) {
-      double re = 0, im = 0;
-      for (int k = 0; k < 3; k++) {
-        re += a[6 * k + 2 * i] * b[6 * k + 2 * j] + a[6 * k + 2 * i + 1] * b[6 * k + 2 * j + 1];
-        im += a[6 * k + 2 * i] * b[6 * k + 2 * j + 1] - a[6 * k + 2 * i + 1] * b[6 * k + 2 * j];
-      }
-      c[6 * i + 2 * j] = re; c[6 * i + 2 * j + 1] = im;
-    }
-}
-
-__global__ void scale_kernel(double *o, const double *a, double s, long n) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i < n) o[i] = s * a[i];
-}
-
-// upper(x) = U_nu(x) link(x+nu) U_nu(x+mu)^+, lower(x) = U_nu(x-nu)^+ link(x-nu) U_nu(x-nu+mu)
-__global__ void staple_kernel(Geom g, double *staple, int mu, int nu, const double *link,
-                              double coef, const double *gnu, double *fl) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const long V = 2L * g.Vh;
-  if (i >= V) return;
-  int x[4], y[4];
-  raw_coords_d(g, i, x);
-  long xpnu = shift_d(g, x, nu, 1), xpmu = shift_d(g, x, mu, 1), xmnu = shift_d(g, x, nu, -1);
-  raw_coords_d(g, xmnu, y);
-  long xmnupmu = shift_d(g, y, mu, 1);
-  double t1[18], up[18], lo[18], a[18], b[18];
-  for (int e = 0; e < 18; e++) { a[e] = link[18 * xpnu + e]; b[e] = gnu[18 * xpmu + e]; }
-  mmul_na(t1, a, b);
-  for (int e = 0; e < 18; e++) a[e] = gnu[18 * i + e];
-  mmul(up, a, t1);
-  for (int e = 0; e < 18; e++) { a[e] = gnu[18 * xmnu + e]; b[e] = link[18 * xmnu + e]; }
-  mmul_an(t1, a, b);
-  for (int e = 0; e < 18; e++) a[e] = gnu[18 * xmnupmu + e];
-  mmul(lo, t1, a);
-  double *f = fl + 18 * i;
-  if (staple) {
-    double *s = staple + 18 * i;
-    for (int e = 0; e < 18; e++) {
-      double v = up[e] + lo[e];
-      s[e] = v;
-      f[e] += coef * v;
-    }
-  } else {
-    for (int e = 0; e < 18; e++) f[e] += coef * up[e];
-    for (int e = 0; e < 18; e++) f[e] += coef * lo[e];
-  }
-}
-
-// L(x) = naik * U(x) [U(x+mu) U(x+2mu)]
-__global__ void naik_kernel(Geom g, int mu, const double *U, double naik, double *out) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const long V = 2L * g.Vh;
-  if (i >= V) return;
-  int x[4];
-  raw_coords_d(g, i, x);
-  long x1 = shift_d(g, x, mu, 1), x2 = shift_d(g, x, mu, 2);
-  double a[18], b[18], t[18], o[18];
-  for (int e = 0; e < 18; e++) { a[e] = U[18 * x1 + e]; b[e] = U[18 * x2 + e]; }
-  mmul(t, a, b);
-  for (int e = 0; e < 18; e++) a[e] = U[18 * i + e];
-  mmul(o, a, t);
-  for (int e = 0; e < 18; e++) out[18 * i + e] = naik * o[e];
-}
-
+// fattening (compute_gen_staple / make_imp_links): site-innermost kernels
+// shared with the force (force.hip, fatten_dev)
 void launch_fatten(const Lattice *lat, const double *gf[4], const double *gl[4],
                    const double c[6], double *fat[4], double *lng[4], hipStream_t st) {
-  const long V = lat->V;
-  dim3 g((V + 255) / 256), b(256);
-  double coef1 = c[0], coef3 = c[1], coef5 = c[2], coef7 = c[3], coefL = c[4];
-  coef1 -= 6 * coefL;  // lib/asqtad_dslash_p.c:531
-  bool have5 = coef5 != 0 || coef7 != 0 || coefL != 0;
-  bool have3 = coef3 != 0 || have5;
-  DevBuf<double> sbuf, tbuf;
-  if (have3) {
-    dev_alloc(&sbuf.p, V * 18 * sizeof(double));
-    dev_alloc(&tbuf.p, V * 18 * sizeof(double));
-  }
-  double *staple = sbuf.p, *temp = tbuf.p;
-  for (int dir = 0; dir < 4; dir++) {
-    dim3 gs((18 * V + 255) / 256);
-    hipLaunchKernelGGL(scale_kernel, gs, b, 0, st, fat[dir], gf[dir], coef1, 18 * V);
-    if (!have3) continue;
-    for (int nu = 0; nu < 4; nu++) {
-      if (nu == dir) continue;
-      hipLaunchKernelGGL(staple_kernel, g, b, 0, st, lat->g, staple, dir, nu, gf[dir], coef3, gf[nu], fat[dir]);
-      if (coefL != 0)
-        hipLaunchKernelGGL(staple_kernel, g, b, 0, st, lat->g, (double *)nullptr, dir, nu, (const double *)staple, coefL, gf[nu], fat[dir]);
-      if (coef5 != 0 || coef7 != 0) {
-        for (int rho = 0; rho < 4; rho++) {
-          if (rho == dir || rho == nu) continue;
-          hipLaunchKernelGGL(staple_kernel, g, b, 0, st, lat->g, temp, dir, rho, (const double *)staple, coef5, gf[rho], fat[dir]);
-          if (coef7 != 0)
-            for (int sig = 0; sig < 4; sig++)
-              if (sig != dir && sig != nu && sig != rho)
-                hipLaunchKernelGGL(staple_kernel, g, b, 0, st, lat->g, (double *)nullptr, dir, sig, (const double *)temp, coef7, gf[sig], fat[dir]);
-        }
-      }
-    }
-  }
-  if (lng)
-    for (int dir = 0; dir < 4; dir++)
-      hipLaunchKernelGGL(naik_kernel, g, b, 0, st, lat->g, dir, gl[dir], c[5], lng[dir]);
-  QOP_CHECK_HIP(hipGetLastError());
-  QOP_CHECK_HIP(hipStreamSynchronize(st));
+  fatten_dev(lat, gf, gl, c, fat, lng, st);
 }
 
 }  // namespace qop
