@@ -7,6 +7,7 @@
 // sign or phase, which the algorithm does not depend on.
 #include "eigcg.h"
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -173,15 +174,53 @@ class Eig {
     sync();
   }
 
-  // invert_eigcg_p.c:201-213
-  void orthonormalize(const int *s, int i0, int n) {
-    for (int i = i0; i < i0 + n; i++) {
-      DevVec vi = view(s[i]);
-      for (int j = 0; j < i; j++) {
-        const DevVec vj = view(s[j]);
-        const Cplx z = cdot(vj, vi, eo_);
-        caxpy(vi, Cplx{-z.re, -z.im}, vj, eo_);
+  double2 *slot(int i) const { return eg_.store + (size_t)i * 3 * Vh_; }
+
+  // G(i, j) = <slot x0+i, slot y0+j> (n1 x n2, host row-major), in tiles
+  // the Gram kernel takes
+  std::vector<zd> gram(int x0, int n1, int y0, int n2) {
+    constexpr int NBLK = 256;
+    if (!partial_.p) dev_alloc(&partial_.p, (size_t)NBLK * gram_max_pairs() * sizeof(double2));
+    if (!gdev_.p) dev_alloc(&gdev_.p, (size_t)gram_max_pairs() * sizeof(double2));
+    std::vector<zd> G((size_t)n1 * n2);
+    const int cj0 = std::min(n2, gram_max_rows() / 2);
+    for (int j0 = 0; j0 < n2; j0 += cj0) {
+      const int cj = std::min(cj0, n2 - j0);
+      const int ci0 = std::max(1, std::min(gram_max_rows() - cj, gram_max_pairs() / cj));
+      for (int i0 = 0; i0 < n1; i0 += ci0) {
+        const int ci = std::min(ci0, n1 - i0);
+        launch_gram(slot(x0 + i0), slot(y0 + j0), 3 * Vh_, ci, cj, 3 * Vh_, partial_.p, NBLK, gdev_.p, stream());
+        std::vector<double2> h((size_t)ci * cj);
+        QOP_CHECK_HIP(hipMemcpyAsync(h.data(), gdev_.p, h.size() * sizeof(double2), hipMemcpyDeviceToHost,
+                                     stream()));
+        sync();
+        for (int i = 0; i < ci; i++)
+          for (int j = 0; j < cj; j++)
+            G[(size_t)(i0 + i) * n2 + j0 + j] = zd(h[(size_t)i * cj + j].x, h[(size_t)i * cj + j].y);
       }
+    }
+    return G;
+  }
+
+  // slots s0+i0 .. s0+i0+n-1 orthonormalised against all slots before them
+  // (invert_eigcg_p.c:201-213; classical Gram-Schmidt applied twice, one
+  // Gram product and one rotation per pass, where the reference subtracts
+  // vector by vector)
+  void orthonormalize(int s0, int i0, int n) {
+    for (int i = i0; i < i0 + n; i++) {
+      for (int pass = 0; pass < 2 && i > 0; pass++) {
+        const std::vector<zd> c = gram(s0, i, s0 + i, 1);
+        std::vector<int> sl(i + 1);
+        std::vector<zd> R(i + 1);
+        sl[0] = s0 + i;
+        R[0] = 1;
+        for (int k = 0; k < i; k++) {
+          sl[k + 1] = s0 + k;
+          R[k + 1] = -c[k];
+        }
+        rotate(sl.data(), i + 1, 1, R);
+      }
+      DevVec vi = view(s0 + i);
       const double r = 1.0 / std::sqrt(norm2(vi, eo_));
       scal(vi, Cplx{r, 0}, vi, eo_);
     }
@@ -191,21 +230,25 @@ class Eig {
   void rayleigh_ritz(int nu, int nv, DevVec &Mp) {
     const int n = nu + nv;
     if (n <= 0) return;
+    (void)Mp;
     std::vector<int> s(n);
     for (int i = 0; i < n; i++) s[i] = i;
-    orthonormalize(s.data(), nu, nv);
+    orthonormalize(0, nu, nv);
     std::vector<zd> mm((size_t)n * n, zd(0)), tt;
     std::vector<double> vv;
     for (int i = 0; i < nu; i++) mm[(size_t)i * n + i] = eg_.l[i];
+    // A u_j of the new vectors into the scratch slots, then one Gram matrix
     for (int j = nu; j < n; j++) {
-      DevVec uj = view(j);
-      linop(Mp, uj);
-      for (int i = 0; i <= j; i++) {
-        const Cplx z = cdot(view(i), Mp, eo_);
-        mm[(size_t)i * n + j] = zd(z.re, z.im);
-        if (i < j) mm[(size_t)j * n + i] = zd(z.re, -z.im);
-      }
+      DevVec uj = view(j), w = view(eg_.cap + j - nu);
+      linop(w, uj);
     }
+    const std::vector<zd> G = gram(0, n, eg_.cap, nv);
+    for (int j = nu; j < n; j++)
+      for (int i = 0; i <= j; i++) {
+        const zd z = G[(size_t)i * nv + (j - nu)];
+        mm[(size_t)i * n + j] = z;
+        if (i < j) mm[(size_t)j * n + i] = std::conj(z);
+      }
     jacobi_herm(n, mm, vv, tt);
     rotate(s.data(), n, n, tt);
     for (int i = 0; i < n; i++) eg_.l[i] = vv[i];
@@ -293,13 +336,17 @@ class Eig {
     scal(d, Cplx{1 / std::sqrt(rsq), 0}, r, eo_);
     nv++;
   }
-  // t(i, j) = Re <v_i, A v_j> (:363-394)
+  // t(i, j) = Re <v_i, A v_j> (:363-394): A v_j into the scratch slots, then
+  // one Gram matrix
   void gett(int v0, int n, DevVec &Mp) {
+    (void)Mp;
     for (int j = 0; j < n; j++) {
-      DevVec vj = view(v0 + j);
-      linop(Mp, vj);
-      for (int i = 0; i < n; i++) t[(size_t)i * m + j] = redot(view(v0 + i), Mp, eo_);
+      DevVec vj = view(v0 + j), w = view(eg_.cap + j);
+      linop(w, vj);
     }
+    const std::vector<zd> G = gram(v0, n, eg_.cap, n);
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) t[(size_t)i * m + j] = G[(size_t)i * n + j].real();
   }
   // invert_eigcg_p.c:561-616
   void reset_t(int v0, int &nv, double a0, double a, double b, const DevVec &r, double rsq, DevVec &Mp) {
@@ -324,6 +371,7 @@ class Eig {
   EigcgState &eg_;
   long Vh_;
   TmpVec tmp_;
+  DevBuf<double2> partial_, gdev_;
 };
 
 }  // namespace
@@ -453,9 +501,7 @@ void invert_eigcg(Links &L, double mass, int eo, InvArg *ia, ResArg *ra, DevVec 
     }
     if (addvecs) {
       if (nv > nev) {
-        std::vector<int> s(nv);
-        for (int i = 0; i < nv; i++) s[i] = v0 + i;
-        E.orthonormalize(s.data(), 0, nv);
+        E.orthonormalize(v0, 0, nv);
         E.gett(v0, nv, Mq.v);
         E.diag_t(nv);
         E.set_y(0, nev, nv, nv);

@@ -29,4 +29,12 @@ void invert_eigcg(Links &L, double mass, int eo, InvArg *ia, ResArg *ra, DevVec 
 void launch_rotate(const double2 *const *x, double2 *const *y, const double2 *R, int n1, int n2,
                    int ld, long len, hipStream_t st);
 
+// G(i, j) = sum_k conj(X_i[k]) Y_j[k] over K complex elements, vectors at
+// stride ld (eigcg.hip); partial: nblk * n1 * n2 complex scratch; G: device
+// n1 x n2 row-major.  Needs n1 n2 <= gram_max_pairs(), n1 + n2 <= gram_max_rows().
+void launch_gram(const double2 *X, const double2 *Y, long ld, int n1, int n2, long K, double2 *partial,
+                 int nblk, double2 *G, hipStream_t st);
+int gram_max_pairs();
+int gram_max_rows();
+
 }  // namespace qop
