@@ -147,7 +147,9 @@ class Eig {
     DevVec v;
     v.lat = L_.lat;
     v.prec = P64;
-    v.data = (char *)(eg_.store + (size_t)i * 3 * Vh_) - (eo_ ? (size_t)3 * Vh_ * sizeof(double2) : 0);
+    // the store leads with one spare half, so the view's other-parity half
+    // (never touched: every operation here is on subset eo) stays inside it
+    v.data = (char *)slot(i) - (eo_ ? (size_t)3 * Vh_ * sizeof(double2) : 0);
     return v;
   }
   double linop(DevVec &o, const DevVec &i) { return ddagd_norm2(L_, P64, mass_, o, i, eo_, tmp_.v); }
@@ -156,8 +158,8 @@ class Eig {
   void rotate(const int *s, int n1, int n2, const std::vector<zd> &R) {
     std::vector<const double2 *> xp(n1);
     std::vector<double2 *> yp(n2);
-    for (int i = 0; i < n1; i++) xp[i] = eg_.store + (size_t)s[i] * 3 * Vh_;
-    for (int j = 0; j < n2; j++) yp[j] = eg_.store + (size_t)(eg_.cap + j) * 3 * Vh_;
+    for (int i = 0; i < n1; i++) xp[i] = slot(s[i]);
+    for (int j = 0; j < n2; j++) yp[j] = slot(eg_.cap + j);
     DevBuf<char> buf;
     const size_t nb = (n1 + n2) * sizeof(void *) + (size_t)n1 * n2 * sizeof(double2);
     dev_alloc(&buf.p, nb);
@@ -169,12 +171,12 @@ class Eig {
     launch_rotate((const double2 *const *)buf.p, (double2 *const *)(buf.p + n1 * sizeof(void *)),
                   (const double2 *)(buf.p + (n1 + n2) * sizeof(void *)), n1, n2, n2, 3 * Vh_, stream());
     for (int j = 0; j < n2; j++)
-      QOP_CHECK_HIP(hipMemcpyAsync(eg_.store + (size_t)s[j] * 3 * Vh_, yp[j], 3 * Vh_ * sizeof(double2),
+      QOP_CHECK_HIP(hipMemcpyAsync(slot(s[j]), yp[j], 3 * Vh_ * sizeof(double2),
                                    hipMemcpyDeviceToDevice, stream()));
     sync();
   }
 
-  double2 *slot(int i) const { return eg_.store + (size_t)i * 3 * Vh_; }
+  double2 *slot(int i) const { return eg_.store + (size_t)(i + 1) * 3 * Vh_; }
 
   // G(i, j) = <slot x0+i, slot y0+j> (n1 x n2, host row-major), in tiles
   // the Gram kernel takes
@@ -390,7 +392,7 @@ void invert_eigcg(Links &L, double mass, int eo, InvArg *ia, ResArg *ra, DevVec 
   if (!eg.store) {
     eg.cap = numax;
     eg.lat = lat;
-    dev_alloc(&eg.store, (size_t)2 * numax * 3 * lat->Vh * sizeof(double2));
+    dev_alloc(&eg.store, (size_t)(2 * numax + 1) * 3 * lat->Vh * sizeof(double2));
     eg.l.assign(numax + 1, 0.0);
     eg.nu = eg.nv = eg.nn = 0;
     eg.addvecs = 1;
