@@ -106,6 +106,28 @@ __device__ __forceinline__ M3 ld(const double2 *f, long V, unsigned s) {
   for (int e = 0; e < 9; e++) m.e[e] = *(const double2 *)((const char *)(f + e * V) + off);
   return m;
 }
+// Non-temporal forms for a site's own matrices that a kernel reads or writes
+// once (accumulators, outputs): they then take less of the L2 the
+// neighbour gathers reuse.
+__device__ __forceinline__ M3 ld_nt(const double2 *f, long V, unsigned s) {
+  M3 m;
+  const unsigned off = s * (unsigned)sizeof(double2);
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    const double *p = (const double *)((const char *)(f + e * V) + off);
+    m.e[e] = make_double2(__builtin_nontemporal_load(p), __builtin_nontemporal_load(p + 1));
+  }
+  return m;
+}
+__device__ __forceinline__ void st_nt(double2 *f, long V, unsigned s, const M3 &m) {
+  const unsigned off = s * (unsigned)sizeof(double2);
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    double *p = (double *)((char *)(f + e * V) + off);
+    __builtin_nontemporal_store(m.e[e].x, p);
+    __builtin_nontemporal_store(m.e[e].y, p + 1);
+  }
+}
 __device__ __forceinline__ void st(double2 *f, long V, unsigned s, const M3 &m) {
   const unsigned off = s * (unsigned)sizeof(double2);
 #pragma unroll
@@ -271,7 +293,7 @@ __global__ __launch_bounds__(TPB) void staple_kernel(Geom g, Sched q, long V, do
   const Site x = site_of(g, s);
   const unsigned xpnu = nbr(g, x, nu, 1, mu, 0), xpmu = nbr(g, x, mu, 1, nu, 0);
   const unsigned xmnu = nbr(g, x, nu, -1, mu, 0), xmnupmu = nbr(g, x, nu, -1, mu, 1);
-  M3 r = ACC ? ld(out, V, s) : zero3();
+  M3 r = ACC ? ld_nt(out, V, s) : zero3();
   {
     const M3 t = mm<false, true>(ld(I, V, xpnu), ld(K, V, xpmu));
     axpy<false>(r, sc, mm<false, false>(ld(K, V, s), t));
@@ -281,7 +303,7 @@ __global__ __launch_bounds__(TPB) void staple_kernel(Geom g, Sched q, long V, do
     axpy<false>(r, sc, mm<false, false>(t, ld(K, V, xmnupmu)));
   }
   if constexpr (HASE) axpy<false>(r, te, ld(E, V, s));
-  st(out, V, s, r);
+  st_nt(out, V, s, r);
 }
 
 // side_force (lib/asqtad_force_fnmat_p.c:129-199) at one site:
@@ -310,8 +332,8 @@ __global__ __launch_bounds__(TPB) void side_force_kernel(Geom g, Sched q, long V
     const M3 t = mm<true, false>(ld(side, V, xmnu), ld<HASB>(bot, V, xmnu));
     SB();
     axpy<false>(sp, 1.0, mm<false, false>(t, ld(side, V, xmnupmu)));
-    if constexpr (ACC) axpy<false>(sp, 1.0, ld(stpl, V, s));
-    st(stpl, V, s, sp);
+    if constexpr (ACC) axpy<false>(sp, 1.0, ld_nt(stpl, V, s));
+    st_nt(stpl, V, s, sp);
   }
   SB();
   const unsigned xmmu = nbr(g, x, mu, -1, nu, 0), xmmupnu = nbr(g, x, mu, -1, nu, 1);
@@ -339,8 +361,8 @@ __global__ __launch_bounds__(TPB) void side_force_kernel(Geom g, Sched q, long V
   }
   SB();
 #undef SB
-  axpy<false>(f, 1.0, ld(force, V, s));
-  st(force, V, s, f);
+  axpy<false>(f, 1.0, ld_nt(force, V, s));
+  st_nt(force, V, s, f);
 }
 
 // After the Lepage/7-staple side_force (lib/asqtad_force_fnmat_p.c:268-277):
@@ -376,9 +398,9 @@ __global__ __launch_bounds__(TPB) void naik_deriv_kernel(Geom g, Sched q, long V
   axpy<false>(f, 1.0, mm<false, true>(a, Up1));
   const M3 b = mm<false, false>(ld(U, V, m2), Um1);  // [U(x-2mu) U(x-mu)]^+ = U(x-mu)^+ U(x-2mu)^+
   axpy<false>(f, 1.0, mm<true, false>(b, ld(m, V, m2)));
-  M3 d = ld(D, V, s);
+  M3 d = ld_nt(D, V, s);
   axpy<false>(d, cn, f);
-  st(D, V, s, d);
+  st_nt(D, V, s, d);
 }
 
 // Lepage fix-up (lib/asqtad_force_fnmat_p.c:317-339), first half:
@@ -411,9 +433,9 @@ __global__ __launch_bounds__(TPB) void lepage_apply_kernel(Geom g, Sched q, long
   const M3 u = ld(U, V, s);
   M3 t = mm<false, false>(ld(acc, V, s), u);
   axpy<false>(t, -1.0, mm<false, false>(u, ld(acc, V, nbr(g, x, mu, 1, 0, 0))));
-  M3 d = ld(D, V, s);
+  M3 d = ld_nt(D, V, s);
   axpy<false>(d, cf, t);
-  st(D, V, s, d);
+  st_nt(D, V, s, d);
 }
 
 // force (raw layout) += fs TA(U D^+), TA the traceless antihermitian part:
@@ -466,10 +488,10 @@ __global__ __launch_bounds__(TPB) void fat_staple_kernel(Geom g, Sched q, long V
     const M3 t = mm<true, false>(ld(K, V, xmnu), ld(I, V, xmnu));
     axpy<false>(r, 1.0, mm<false, false>(t, ld(K, V, xmnupmu)));
   }
-  if constexpr (STORE) st(out, V, s, r);
-  M3 f = ld(fat, V, s);
+  if constexpr (STORE) st_nt(out, V, s, r);
+  M3 f = ld_nt(fat, V, s);
   axpy<false>(f, c, r);
-  st(fat, V, s, f);
+  st_nt(fat, V, s, f);
 }
 
 // Naik link L(x) = c U(x) U(x+mu) U(x+2mu) (lib/asqtad_dslash_p.c:600-620)
