@@ -253,30 +253,48 @@ __global__ __launch_bounds__(TPB) void mid_kernel(Geom g, Sched q, long V, const
   const long Vh = g.Vh;
   const double sg = s >= Vh ? -scale : scale;
   auto vidx = [&](long i, int c) { return i + (i >= Vh ? 2 * Vh : 0) + c * Vh; };
-  for (int m = 0; m < nmid; m++) {
-    const unsigned j = nbr(g, x, m & 3, m < 4 ? 1 : 3, 0, 0);
-    M3 acc = zero3();
+  // four mids per pass (one- and three-hop of two directions when nmid = 8),
+  // so x_t(x) is loaded once per pass instead of once per mid
+  for (int g0 = 0; g0 < nmid; g0 += 4) {
+    int mi[4];
+    unsigned j[4];
+    M3 acc[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      mi[k] = nmid == 8 ? (g0 / 2 + (k >> 1)) + 4 * (k & 1) : g0 + k;
+      j[k] = nbr(g, x, mi[k] & 3, mi[k] < 4 ? 1 : 3, 0, 0);
+      acc[k] = zero3();
+    }
     for (int t = 0; t < n; t++) {
       const T2 *v = xs[t];
-      double2 a[3], b[3];
+      double2 a[3];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const T2 va = v[vidx(s, c)], vb = v[vidx(j, c)];
+        const T2 va = v[vidx(s, c)];
         a[c] = make_double2(va.x, va.y);
-        b[c] = make_double2(vb.x, vb.y);
       }
       const double w = sg * eps[t];
 #pragma unroll
-      for (int r = 0; r < 3; r++)
+      for (int k = 0; k < 4; k++) {
+        double2 b[3];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          const double re = fma(a[r].x, b[c].x, a[r].y * b[c].y);
-          const double im = fma(a[r].y, b[c].x, -a[r].x * b[c].y);
-          acc.e[3 * r + c].x = fma(w, re, acc.e[3 * r + c].x);
-          acc.e[3 * r + c].y = fma(w, im, acc.e[3 * r + c].y);
+          const T2 vb = v[vidx(j[k], c)];
+          b[c] = make_double2(vb.x, vb.y);
         }
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            const double re = fma(a[r].x, b[c].x, a[r].y * b[c].y);
+            const double im = fma(a[r].y, b[c].x, -a[r].x * b[c].y);
+            acc[k].e[3 * r + c].x = fma(w, re, acc[k].e[3 * r + c].x);
+            acc[k].e[3 * r + c].y = fma(w, im, acc[k].e[3 * r + c].y);
+          }
+      }
     }
-    st(mid + (long)m * 9 * V, V, s, acc);
+#pragma unroll
+    for (int k = 0; k < 4; k++) st_nt(mid + (long)mi[k] * 9 * V, V, s, acc[k]);
   }
 }
 
