@@ -340,25 +340,15 @@ __global__ __launch_bounds__(TPB) void side_force_kernel(Geom g, Sched q, long V
   const unsigned xpmu = nbr(g, x, mu, 1, nu, 0), xpnu = nbr(g, x, nu, 1, mu, 0);
   // SB() keeps the scheduler from hoisting every matrix load to the top:
   // with all 14 in flight a thread needs ~500 registers (one wave per SIMD).
+  // B(x+nu) S(x+mu)^+ and S(x+mu) are loaded once and serve both the staple
+  // and the force terms at +mu.
 #define SB() __builtin_amdgcn_sched_barrier(0)
-  {
-    const unsigned xmnu = nbr(g, x, nu, -1, mu, 0), xmnupmu = nbr(g, x, nu, -1, mu, 1);
-    const M3 bs = mm<false, true>(ld<HASB>(bot, V, xpnu), ld(side, V, xpmu));  // B(x+nu) S(x+mu)^+
-    SB();
-    M3 sp = mm<false, false>(ld(side, V, s), bs);
-    SB();
-    const M3 t = mm<true, false>(ld(side, V, xmnu), ld<HASB>(bot, V, xmnu));
-    SB();
-    axpy<false>(sp, 1.0, mm<false, false>(t, ld(side, V, xmnupmu)));
-    if constexpr (ACC) axpy<false>(sp, 1.0, ld_nt(stpl, V, s));
-    st_nt(stpl, V, s, sp);
-  }
-  SB();
-  const unsigned xmmu = nbr(g, x, mu, -1, nu, 0), xmmupnu = nbr(g, x, mu, -1, nu, 1);
-  M3 f;
+  M3 f, sp;
   {
     const M3 Smu = ld(side, V, xpmu);
-    const M3 bs = mm<false, true>(ld<HASB>(bot, V, xpnu), Smu);
+    const M3 bs = mm<false, true>(ld<HASB>(bot, V, xpnu), Smu);  // B(x+nu) S(x+mu)^+
+    SB();
+    sp = mm<false, false>(ld(side, V, s), bs);
     SB();
     f = mm<false, true>(ld(top, V, s), bs);  // T S(x+mu) B(x+nu)^+
     SB();
@@ -367,6 +357,16 @@ __global__ __launch_bounds__(TPB) void side_force_kernel(Geom g, Sched q, long V
     axpy<false>(f, 1.0, mm<false, true>(b2, ld(top, V, xpnu)));
   }
   SB();
+  {
+    const unsigned xmnu = nbr(g, x, nu, -1, mu, 0), xmnupmu = nbr(g, x, nu, -1, mu, 1);
+    const M3 t = mm<true, false>(ld(side, V, xmnu), ld<HASB>(bot, V, xmnu));
+    SB();
+    axpy<false>(sp, 1.0, mm<false, false>(t, ld(side, V, xmnupmu)));
+    if constexpr (ACC) axpy<false>(sp, 1.0, ld_nt(stpl, V, s));
+    st_nt(stpl, V, s, sp);
+  }
+  SB();
+  const unsigned xmmu = nbr(g, x, mu, -1, nu, 0), xmmupnu = nbr(g, x, mu, -1, nu, 1);
   {
     const M3 Sm = ld(side, V, xmmu);
     const M3 t1 = mm<true, false>(ld<HASB>(bot, V, xmmu), Sm);  // B(x-mu)^+ S(x-mu)
