@@ -329,10 +329,18 @@ __global__ __launch_bounds__(TPB) void staple_kernel(Geom g, Sched q, long V, do
 //           + B(x-mu)^+ S(x-mu) T(x-mu+nu) + T(x-mu)^+ S(x-mu) B(x-mu+nu)
 // stpl(x) = [stpl(x) if acc] + S(x) B(x+nu) S(x+mu)^+ + S(x-nu)^+ B(x-nu) S(x-nu+mu)
 // with B = bot, S = side (the link along nu), T = top.
-template <bool ACC, bool HASB>
+// MIX (with ACC = false): the staple is not stored as is but mixed as the
+// reference does next (lib/asqtad_force_fnmat_p.c:268-277): mid5 += cl S
+// (mid5 may be null), stpl = c7 S + c5 ms.
+struct Mix {
+  double2 *mid5;
+  double cl, c7, c5;
+  const double2 *ms;
+};
+template <bool ACC, bool HASB, bool MIX = false>
 __global__ __launch_bounds__(TPB) void side_force_kernel(Geom g, Sched q, long V, double2 *force, double2 *stpl,
                                                          Lin bot, const double2 *side,
-                                                         const double2 *top, int mu, int nu) {
+                                                         const double2 *top, int mu, int nu, Mix mx) {
   const int s0 = site_index(q, g.Vh);
   if (s0 < 0) return;
   const unsigned s = (unsigned)s0;
@@ -340,15 +348,14 @@ __global__ __launch_bounds__(TPB) void side_force_kernel(Geom g, Sched q, long V
   const unsigned xpmu = nbr(g, x, mu, 1, nu, 0), xpnu = nbr(g, x, nu, 1, mu, 0);
   // SB() keeps the scheduler from hoisting every matrix load to the top:
   // with all 14 in flight a thread needs ~500 registers (one wave per SIMD).
-  // B(x+nu) S(x+mu)^+ and S(x+mu) are loaded once and serve both the staple
-  // and the force terms at +mu.
+  // B(x+nu) S(x+mu)^+ and S(x+mu) are loaded once and serve both the force
+  // terms at +mu and the staple; the staple is formed last, after the force
+  // is stored, so the mix (MIX) has the registers of the force terms.
 #define SB() __builtin_amdgcn_sched_barrier(0)
-  M3 f, sp;
+  M3 f, bs;
   {
     const M3 Smu = ld(side, V, xpmu);
-    const M3 bs = mm<false, true>(ld<HASB>(bot, V, xpnu), Smu);  // B(x+nu) S(x+mu)^+
-    SB();
-    sp = mm<false, false>(ld(side, V, s), bs);
+    bs = mm<false, true>(ld<HASB>(bot, V, xpnu), Smu);  // B(x+nu) S(x+mu)^+
     SB();
     f = mm<false, true>(ld(top, V, s), bs);  // T S(x+mu) B(x+nu)^+
     SB();
@@ -358,16 +365,7 @@ __global__ __launch_bounds__(TPB) void side_force_kernel(Geom g, Sched q, long V
   }
   SB();
   {
-    const unsigned xmnu = nbr(g, x, nu, -1, mu, 0), xmnupmu = nbr(g, x, nu, -1, mu, 1);
-    const M3 t = mm<true, false>(ld(side, V, xmnu), ld<HASB>(bot, V, xmnu));
-    SB();
-    axpy<false>(sp, 1.0, mm<false, false>(t, ld(side, V, xmnupmu)));
-    if constexpr (ACC) axpy<false>(sp, 1.0, ld_nt(stpl, V, s));
-    st_nt(stpl, V, s, sp);
-  }
-  SB();
-  const unsigned xmmu = nbr(g, x, mu, -1, nu, 0), xmmupnu = nbr(g, x, mu, -1, nu, 1);
-  {
+    const unsigned xmmu = nbr(g, x, mu, -1, nu, 0), xmmupnu = nbr(g, x, mu, -1, nu, 1);
     const M3 Sm = ld(side, V, xmmu);
     const M3 t1 = mm<true, false>(ld<HASB>(bot, V, xmmu), Sm);  // B(x-mu)^+ S(x-mu)
     SB();
@@ -378,25 +376,39 @@ __global__ __launch_bounds__(TPB) void side_force_kernel(Geom g, Sched q, long V
     axpy<false>(f, 1.0, mm<false, false>(t2, ld<HASB>(bot, V, xmmupnu)));
   }
   SB();
-#undef SB
   axpy<false>(f, 1.0, ld_nt(force, V, s));
   st_nt(force, V, s, f);
-}
-
-// After the Lepage/7-staple side_force (lib/asqtad_force_fnmat_p.c:268-277):
-// mid5 += cl mid3 (if mid5), mid3 = c7 mid3 + c5 ms.
-__global__ __launch_bounds__(TPB) void mix_kernel(double2 *mid5, double cl, double2 *mid3, double c7, double c5,
-                           const double2 *ms, long n) {
-  const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const double2 t = mid3[k], m = ms[k];
-  if (mid5) {
-    double2 a = mid5[k];
-    a.x = fma(cl, t.x, a.x);
-    a.y = fma(cl, t.y, a.y);
-    mid5[k] = a;
+  SB();
+  {
+    const unsigned xmnu = nbr(g, x, nu, -1, mu, 0), xmnupmu = nbr(g, x, nu, -1, mu, 1);
+    M3 sp = mm<false, false>(ld(side, V, s), bs);
+    SB();
+    const M3 t = mm<true, false>(ld(side, V, xmnu), ld<HASB>(bot, V, xmnu));
+    SB();
+    axpy<false>(sp, 1.0, mm<false, false>(t, ld(side, V, xmnupmu)));
+    if constexpr (ACC) axpy<false>(sp, 1.0, ld_nt(stpl, V, s));
+    if constexpr (MIX) {
+      SB();
+      if (mx.mid5) {
+        M3 a = ld_nt(mx.mid5, V, s);
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+          a.e[e].x = fma(mx.cl, sp.e[e].x, a.e[e].x);
+          a.e[e].y = fma(mx.cl, sp.e[e].y, a.e[e].y);
+        }
+        st_nt(mx.mid5, V, s, a);
+      }
+      SB();
+      const M3 m = ld(mx.ms, V, s);
+#pragma unroll
+      for (int e = 0; e < 9; e++) {
+        sp.e[e].x = fma(mx.c7, sp.e[e].x, mx.c5 * m.e[e].x);
+        sp.e[e].y = fma(mx.c7, sp.e[e].y, mx.c5 * m.e[e].y);
+      }
+    }
+    st_nt(stpl, V, s, sp);
   }
-  mid3[k] = make_double2(fma(c7, t.x, c5 * m.x), fma(c7, t.y, c5 * m.y));
+#undef SB
 }
 
 // Naik part of QOP_asqtad_deriv (lib/asqtad_force_fnmat_p.c:395-442):
@@ -765,14 +777,21 @@ class ForceRun {
   }
   void side_force(double2 *force, Lin bot, const double2 *side, const double2 *top, int mu, int nu,
                   double2 *stpl, int acc) {
+    const Mix mx{};
     if (acc && bot.b)
-      hipLaunchKernelGGL((side_force_kernel<true, true>), gx_, b_, 0, st_, g_, sq_, V_, force, stpl, bot, side, top, mu, nu);
+      hipLaunchKernelGGL((side_force_kernel<true, true>), gx_, b_, 0, st_, g_, sq_, V_, force, stpl, bot, side, top, mu, nu, mx);
     else if (acc)
-      hipLaunchKernelGGL((side_force_kernel<true, false>), gx_, b_, 0, st_, g_, sq_, V_, force, stpl, bot, side, top, mu, nu);
+      hipLaunchKernelGGL((side_force_kernel<true, false>), gx_, b_, 0, st_, g_, sq_, V_, force, stpl, bot, side, top, mu, nu, mx);
     else if (bot.b)
-      hipLaunchKernelGGL((side_force_kernel<false, true>), gx_, b_, 0, st_, g_, sq_, V_, force, stpl, bot, side, top, mu, nu);
+      hipLaunchKernelGGL((side_force_kernel<false, true>), gx_, b_, 0, st_, g_, sq_, V_, force, stpl, bot, side, top, mu, nu, mx);
     else
-      hipLaunchKernelGGL((side_force_kernel<false, false>), gx_, b_, 0, st_, g_, sq_, V_, force, stpl, bot, side, top, mu, nu);
+      hipLaunchKernelGGL((side_force_kernel<false, false>), gx_, b_, 0, st_, g_, sq_, V_, force, stpl, bot, side, top, mu, nu, mx);
+  }
+  // side_force with a single bottom field and the mix folded in (stpl = c7 S + c5 ms)
+  void side_force_mix(double2 *force, Lin bot, const double2 *side, const double2 *top, int mu, int nu,
+                      double2 *stpl, const Mix &mx) {
+    hipLaunchKernelGGL((side_force_kernel<false, false, true>), gx_, b_, 0, st_, g_, sq_, V_, force, stpl, bot, side, top, mu,
+                       nu, mx);
   }
 
   const Lattice *lat_;
@@ -826,11 +845,10 @@ void ForceRun::deriv(double2 *const U[4], double2 *D[4], const double coef[6], d
         }
         if (coefL != 0) nfl += 36;
         if (coefL != 0 || coef7 != 0) {
-          side_force(D[rho], Lin{mid[sig], 1.0, nullptr}, U[rho], stpl5, sig, rho, mid3, 0);
+          side_force_mix(D[rho], Lin{mid[sig], 1.0, nullptr}, U[rho], stpl5, sig, rho, mid3,
+                         Mix{coefL != 0 ? mid5[rho] : nullptr, coefL, coef7, coef5, mid[sig]});
           nfl += SIDE_FORCE_FLOPS;
           if (coefL != 0) nfl += 36;
-          hipLaunchKernelGGL(mix_kernel, ge_, b_, 0, st_, coefL != 0 ? mid5[rho] : nullptr, coefL, mid3,
-                             coef7, coef5, (const double2 *)mid[sig], n9_);
         } else {
           lincomb(mid3, 0.0, coef5, mid[sig], 0.0, nullptr);
         }
