@@ -329,15 +329,22 @@ __global__ __launch_bounds__(TPB) void staple_kernel(Geom g, Sched q, long V, do
 //           + B(x-mu)^+ S(x-mu) T(x-mu+nu) + T(x-mu)^+ S(x-mu) B(x-mu+nu)
 // stpl(x) = [stpl(x) if acc] + S(x) B(x+nu) S(x+mu)^+ + S(x-nu)^+ B(x-nu) S(x-nu+mu)
 // with B = bot, S = side (the link along nu), T = top.
-// MIX (with ACC = false): the staple is not stored as is but mixed as the
-// reference does next (lib/asqtad_force_fnmat_p.c:268-277): mid5 += cl S
+// EPI: what is done with the staple S at the end.
+// EPI_STORE: stpl = S.
+// EPI_MIX (the 7-staple/Lepage side_force): S is mixed as the reference does
+// next (lib/asqtad_force_fnmat_p.c:268-277): mid5 = [mid5 if m5acc] + cl S
 // (mid5 may be null), stpl = c7 S + c5 ms.
+// EPI_FIN (the last 3-staple side_force of a direction sig): S is not
+// stored but added to the derivative of sig (:309-314):
+// d = d + S [+ c1 ms if ms].
+enum { EPI_STORE = 0, EPI_MIX = 1, EPI_FIN = 2 };
 struct Mix {
-  double2 *mid5;
-  double cl, c7, c5;
+  double2 *mid5;  // EPI_FIN: the derivative d
+  double cl, c7, c5;  // EPI_FIN: c5 = c1
   const double2 *ms;
+  int m5acc;
 };
-template <bool ACC, bool HASB, bool MIX = false>
+template <bool ACC, bool HASB, int EPI = EPI_STORE>
 __global__ __launch_bounds__(TPB) void side_force_kernel(Geom g, Sched q, long V, double2 *force, double2 *stpl,
                                                          Lin bot, const double2 *side,
                                                          const double2 *top, int mu, int nu, Mix mx) {
@@ -350,7 +357,7 @@ __global__ __launch_bounds__(TPB) void side_force_kernel(Geom g, Sched q, long V
   // with all 14 in flight a thread needs ~500 registers (one wave per SIMD).
   // B(x+nu) S(x+mu)^+ and S(x+mu) are loaded once and serve both the force
   // terms at +mu and the staple; the staple is formed last, after the force
-  // is stored, so the mix (MIX) has the registers of the force terms.
+  // is stored, so its epilogue (EPI) has the registers of the force terms.
 #define SB() __builtin_amdgcn_sched_barrier(0)
   M3 f, bs;
   {
@@ -387,10 +394,21 @@ __global__ __launch_bounds__(TPB) void side_force_kernel(Geom g, Sched q, long V
     SB();
     axpy<false>(sp, 1.0, mm<false, false>(t, ld(side, V, xmnupmu)));
     if constexpr (ACC) axpy<false>(sp, 1.0, ld_nt(stpl, V, s));
-    if constexpr (MIX) {
+    if constexpr (EPI == EPI_FIN) {
+      SB();
+      M3 d = ld_nt(mx.mid5, V, s);
+      axpy<false>(d, 1.0, sp);
+      if (mx.ms) {
+        SB();
+        axpy<false>(d, mx.c5, ld(mx.ms, V, s));
+      }
+      st_nt(mx.mid5, V, s, d);
+      return;
+    }
+    if constexpr (EPI == EPI_MIX) {
       SB();
       if (mx.mid5) {
-        M3 a = ld_nt(mx.mid5, V, s);
+        M3 a = mx.m5acc ? ld_nt(mx.mid5, V, s) : zero3();
 #pragma unroll
         for (int e = 0; e < 9; e++) {
           a.e[e].x = fma(mx.cl, sp.e[e].x, a.e[e].x);
@@ -790,8 +808,19 @@ class ForceRun {
   // side_force with a single bottom field and the mix folded in (stpl = c7 S + c5 ms)
   void side_force_mix(double2 *force, Lin bot, const double2 *side, const double2 *top, int mu, int nu,
                       double2 *stpl, const Mix &mx) {
-    hipLaunchKernelGGL((side_force_kernel<false, false, true>), gx_, b_, 0, st_, g_, sq_, V_, force, stpl, bot, side, top, mu,
-                       nu, mx);
+    hipLaunchKernelGGL((side_force_kernel<false, false, EPI_MIX>), gx_, b_, 0, st_, g_, sq_, V_, force, stpl, bot, side, top,
+                       mu, nu, mx);
+  }
+  // accumulating side_force whose staple goes into d (+ c1 ms) instead of a field
+  void side_force_fin(double2 *force, Lin bot, const double2 *side, const double2 *top, int mu, int nu,
+                      double2 *stpl, double2 *d, double c1, const double2 *ms) {
+    const Mix mx{d, 0.0, 0.0, c1, ms, 0};
+    if (bot.b)
+      hipLaunchKernelGGL((side_force_kernel<true, true, EPI_FIN>), gx_, b_, 0, st_, g_, sq_, V_, force, stpl, bot, side, top,
+                         mu, nu, mx);
+    else
+      hipLaunchKernelGGL((side_force_kernel<true, false, EPI_FIN>), gx_, b_, 0, st_, g_, sq_, V_, force, stpl, bot, side, top,
+                         mu, nu, mx);
   }
 
   const Lattice *lat_;
@@ -821,9 +850,10 @@ void ForceRun::deriv(double2 *const U[4], double2 *D[4], const double coef[6], d
       for (int mu = 0; mu < 4; mu++) {
         if (mu == sig) continue;
         staple(stpl3[mu], 0, 1.0, U[sig], U[mu], sig, mu, 0.0, nullptr);
-        lincomb(mid5[mu], 0.0, 0.0, nullptr, 0.0, nullptr);
         nfl += STAPLE_FLOPS;
       }
+      // mid5[nu] starts at zero (:243-247): its first update overwrites
+      bool m5set[4] = {false, false, false, false};
       for (int rho = 0; rho < 4; rho++) {
         if (rho == sig) continue;
         // stpl5 = coef7 sum 7-staples + coefL stpl3[rho]
@@ -846,7 +876,8 @@ void ForceRun::deriv(double2 *const U[4], double2 *D[4], const double coef[6], d
         if (coefL != 0) nfl += 36;
         if (coefL != 0 || coef7 != 0) {
           side_force_mix(D[rho], Lin{mid[sig], 1.0, nullptr}, U[rho], stpl5, sig, rho, mid3,
-                         Mix{coefL != 0 ? mid5[rho] : nullptr, coefL, coef7, coef5, mid[sig]});
+                         Mix{coefL != 0 ? mid5[rho] : nullptr, coefL, coef7, coef5, mid[sig], m5set[rho]});
+          if (coefL != 0) m5set[rho] = true;
           nfl += SIDE_FORCE_FLOPS;
           if (coefL != 0) nfl += 36;
         } else {
@@ -857,27 +888,34 @@ void ForceRun::deriv(double2 *const U[4], double2 *D[4], const double coef[6], d
           if (mu == sig || mu == rho) continue;
           for (int nu = 0; nu < 4; nu++) {
             if (nu == sig || nu == rho || nu == mu) continue;
-            side_force(D[mu], Lin{mid3, 1.0, nullptr}, U[mu], stpl3[nu], sig, mu, mid5[nu], 1);
+            side_force(D[mu], Lin{mid3, 1.0, nullptr}, U[mu], stpl3[nu], sig, mu, mid5[nu], m5set[nu]);
+            m5set[nu] = true;
             nfl += SIDE_FORCE_FLOPS;
           }
         }
       }
+      for (int mu = 0; mu < 4; mu++)
+        if (mu != sig && !m5set[mu]) lincomb(mid5[mu], 0.0, 0.0, nullptr, 0.0, nullptr);
     }
     if (have3) {
-      int first = 1;
+      // the last one adds its staple sum (mid3) and coef1 mid to D[sig]
+      int k = 0;
       for (int mu = 0; mu < 4; mu++) {
         if (mu == sig) continue;
         nfl += have5 ? 36 : 18;
-        side_force(D[mu], Lin{mid[sig], coef3, have5 ? mid5[mu] : nullptr}, U[mu], U[sig], sig, mu, mid3,
-                   !first);
-        first = 0;
+        const Lin bot{mid[sig], coef3, have5 ? mid5[mu] : nullptr};
+        if (++k < 3)
+          side_force(D[mu], bot, U[mu], U[sig], sig, mu, mid3, k > 1);
+        else
+          side_force_fin(D[mu], bot, U[mu], U[sig], sig, mu, mid3, D[sig], coef1,
+                         coef1 != 0 ? mid[sig] : nullptr);
         nfl += SIDE_FORCE_FLOPS;
       }
       nfl += 18;
+    } else if (coef1 != 0) {
+      lincomb(D[sig], 1.0, 0.0, nullptr, coef1, mid[sig]);
     }
     if (coef1 != 0) nfl += 36;
-    if (have3 || coef1 != 0)
-      lincomb(D[sig], 1.0, have3 ? 1.0 : 0.0, have3 ? mid3 : nullptr, coef1, coef1 != 0 ? mid[sig] : nullptr);
   }
   if (coefL != 0) {
     // Lepage fix-up (:317-339)
