@@ -300,18 +300,14 @@ __global__ __launch_bounds__(TPB) void mid_kernel(Geom g, Sched q, long V, const
 
 // staple (lib/asqtad_force_fnmat_p.c:93-127) with its scaling folded in:
 // out(x) = [out(x) if acc] + sc [K(x) I(x+nu) K(x+mu)^+ + K(x-nu)^+ I(x-nu) K(x-nu+mu)]
-//          + te E(x)
-// I is the field along mu, K the link along nu.
-template <bool ACC, bool HASE>
-__global__ __launch_bounds__(TPB) void staple_kernel(Geom g, Sched q, long V, double2 *out, double sc, const double2 *I,
-                              const double2 *K, int mu, int nu, double te, const double2 *E) {
-  const int s0 = site_index(q, g.Vh);
-  if (s0 < 0) return;
-  const unsigned s = (unsigned)s0;
-  const Site x = site_of(g, s);
+//          + te E(x) [+ the same staple of I2, K2 along nu2 if TWO]
+// I is the field along mu, K the link along nu.  TWO computes the two
+// 7-staples of a (sig, rho) pair in one pass, in the order in which the
+// reference accumulates them (:252-262).
+__device__ __forceinline__ void add_staple(M3 &r, double sc, const double2 *I, const double2 *K, long V,
+                                           const Geom &g, const Site &x, unsigned s, int mu, int nu) {
   const unsigned xpnu = nbr(g, x, nu, 1, mu, 0), xpmu = nbr(g, x, mu, 1, nu, 0);
   const unsigned xmnu = nbr(g, x, nu, -1, mu, 0), xmnupmu = nbr(g, x, nu, -1, mu, 1);
-  M3 r = ACC ? ld_nt(out, V, s) : zero3();
   {
     const M3 t = mm<false, true>(ld(I, V, xpnu), ld(K, V, xpmu));
     axpy<false>(r, sc, mm<false, false>(ld(K, V, s), t));
@@ -320,7 +316,22 @@ __global__ __launch_bounds__(TPB) void staple_kernel(Geom g, Sched q, long V, do
     const M3 t = mm<true, false>(ld(K, V, xmnu), ld(I, V, xmnu));
     axpy<false>(r, sc, mm<false, false>(t, ld(K, V, xmnupmu)));
   }
+}
+template <bool ACC, bool HASE, bool TWO = false>
+__global__ __launch_bounds__(TPB) void staple_kernel(Geom g, Sched q, long V, double2 *out, double sc, const double2 *I,
+                                                     const double2 *K, int mu, int nu, double te, const double2 *E,
+                                                     const double2 *I2, const double2 *K2, int nu2) {
+  const int s0 = site_index(q, g.Vh);
+  if (s0 < 0) return;
+  const unsigned s = (unsigned)s0;
+  const Site x = site_of(g, s);
+  M3 r = ACC ? ld_nt(out, V, s) : zero3();
+  add_staple(r, sc, I, K, V, g, x, s, mu, nu);
   if constexpr (HASE) axpy<false>(r, te, ld(E, V, s));
+  if constexpr (TWO) {
+    __builtin_amdgcn_sched_barrier(0);
+    add_staple(r, sc, I2, K2, V, g, x, s, mu, nu2);
+  }
   st_nt(out, V, s, r);
 }
 
@@ -784,14 +795,25 @@ class ForceRun {
  private:
   void staple(double2 *out, int acc, double sc, const double2 *I, const double2 *K, int mu, int nu,
               double te, const double2 *E) {
+    const double2 *n2 = nullptr;
     if (acc && E)
-      hipLaunchKernelGGL((staple_kernel<true, true>), gx_, b_, 0, st_, g_, sq_, V_, out, sc, I, K, mu, nu, te, E);
+      hipLaunchKernelGGL((staple_kernel<true, true>), gx_, b_, 0, st_, g_, sq_, V_, out, sc, I, K, mu, nu, te, E, n2, n2, 0);
     else if (acc)
-      hipLaunchKernelGGL((staple_kernel<true, false>), gx_, b_, 0, st_, g_, sq_, V_, out, sc, I, K, mu, nu, te, E);
+      hipLaunchKernelGGL((staple_kernel<true, false>), gx_, b_, 0, st_, g_, sq_, V_, out, sc, I, K, mu, nu, te, E, n2, n2, 0);
     else if (E)
-      hipLaunchKernelGGL((staple_kernel<false, true>), gx_, b_, 0, st_, g_, sq_, V_, out, sc, I, K, mu, nu, te, E);
+      hipLaunchKernelGGL((staple_kernel<false, true>), gx_, b_, 0, st_, g_, sq_, V_, out, sc, I, K, mu, nu, te, E, n2, n2, 0);
     else
-      hipLaunchKernelGGL((staple_kernel<false, false>), gx_, b_, 0, st_, g_, sq_, V_, out, sc, I, K, mu, nu, te, E);
+      hipLaunchKernelGGL((staple_kernel<false, false>), gx_, b_, 0, st_, g_, sq_, V_, out, sc, I, K, mu, nu, te, E, n2, n2, 0);
+  }
+  // out = sc S(I, K, nu) [+ te E] + sc S(I2, K2, nu2)
+  void staple2(double2 *out, double sc, const double2 *I, const double2 *K, int mu, int nu, double te,
+               const double2 *E, const double2 *I2, const double2 *K2, int nu2) {
+    if (E)
+      hipLaunchKernelGGL((staple_kernel<false, true, true>), gx_, b_, 0, st_, g_, sq_, V_, out, sc, I, K, mu, nu, te, E, I2,
+                         K2, nu2);
+    else
+      hipLaunchKernelGGL((staple_kernel<false, false, true>), gx_, b_, 0, st_, g_, sq_, V_, out, sc, I, K, mu, nu, te, E, I2,
+                         K2, nu2);
   }
   void side_force(double2 *force, Lin bot, const double2 *side, const double2 *top, int mu, int nu,
                   double2 *stpl, int acc) {
@@ -858,17 +880,19 @@ void ForceRun::deriv(double2 *const U[4], double2 *D[4], const double coef[6], d
         if (rho == sig) continue;
         // stpl5 = coef7 sum 7-staples + coefL stpl3[rho]
         if (coef7 != 0) {
-          int first = 1;
+          // the two (mu, nu) pairs of directions other than sig and rho
+          int pm[2], pn[2], np = 0;
           for (int mu = 0; mu < 4; mu++) {
             if (mu == sig || mu == rho) continue;
             for (int nu = 0; nu < 4; nu++) {
               if (nu == sig || nu == rho || nu == mu) continue;
-              staple(stpl5, !first, coef7, stpl3[mu], U[nu], sig, nu, first && coefL != 0 ? coefL : 0.0,
-                     first && coefL != 0 ? stpl3[rho] : nullptr);
-              first = 0;
+              pm[np] = mu;
+              pn[np++] = nu;
               nfl += STAPLE_FLOPS;
             }
           }
+          staple2(stpl5, coef7, stpl3[pm[0]], U[pn[0]], sig, pn[0], coefL != 0 ? coefL : 0.0,
+                  coefL != 0 ? stpl3[rho] : nullptr, stpl3[pm[1]], U[pn[1]], pn[1]);
           nfl += 18;
         } else {
           lincomb(stpl5, 0.0, coefL, stpl3[rho], 0.0, nullptr);
