@@ -206,12 +206,21 @@ struct Ptr4 {
   const double2 *p[4];
 };
 
-// raw [V][18] -> site-innermost [9][V]
+// raw [V][18] -> site-innermost [9][V].  A block's TPB sites are one
+// contiguous run of 9 TPB complex numbers in the raw layout; it is read
+// whole into LDS and written out element-plane by element-plane, so both
+// sides are contiguous.
 __global__ __launch_bounds__(TPB) void to_soa_kernel(const double2 *raw, double2 *f, long V) {
-  const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (s >= V) return;
+  __shared__ double2 t[9 * TPB];
+  const long s0 = (long)blockIdx.x * TPB;
+  const int n = V - s0 < TPB ? (int)(V - s0) : TPB;
+  const double2 *src = raw + 9 * s0;
+  for (int i = threadIdx.x; i < 9 * n; i += TPB) t[i] = src[i];
+  __syncthreads();
+  if ((int)threadIdx.x < n) {
 #pragma unroll
-  for (int e = 0; e < 9; e++) f[e * V + s] = raw[9 * s + e];
+    for (int e = 0; e < 9; e++) f[e * V + s0 + threadIdx.x] = t[9 * threadIdx.x + e];
+  }
 }
 
 // o = co o + ca A + cb B; co == 0 does not read o, null A/B are zero
@@ -570,12 +579,18 @@ __global__ __launch_bounds__(TPB) void naik_link_kernel(Geom g, Sched q, long V,
   st(L, V, s, o);
 }
 
-// site-innermost [9][V] -> raw [V][18]
+// site-innermost [9][V] -> raw [V][18], through LDS as to_soa_kernel
 __global__ __launch_bounds__(TPB) void from_soa_kernel(const double2 *f, double2 *raw, long V) {
-  const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (s >= V) return;
+  __shared__ double2 t[9 * TPB];
+  const long s0 = (long)blockIdx.x * TPB;
+  const int n = V - s0 < TPB ? (int)(V - s0) : TPB;
+  if ((int)threadIdx.x < n) {
 #pragma unroll
-  for (int e = 0; e < 9; e++) raw[9 * s + e] = f[e * V + s];
+    for (int e = 0; e < 9; e++) t[9 * threadIdx.x + e] = f[e * V + s0 + threadIdx.x];
+  }
+  __syncthreads();
+  double2 *dst = raw + 9 * s0;
+  for (int i = threadIdx.x; i < 9 * n; i += TPB) dst[i] = t[i];
 }
 
 // Derivative of the U(3) projection W = V (V^+V)^-1/2 contracted with F2,
