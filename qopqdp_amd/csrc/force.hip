@@ -511,28 +511,39 @@ __global__ __launch_bounds__(TPB) void lepage_apply_kernel(Geom g, Sched q, long
 // HISQ force (factor 2 of lib/hisq_force_fnmat_p.c:171-181, then :211-219)
 __global__ __launch_bounds__(TPB) void force_kernel(long V, double2 *F, const double2 *U, const double2 *D,
                                                     double fs) {
-  const long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (s >= V) return;
-  const M3 a = mm<false, true>(ld(U, V, s), ld(D, V, s));
-  M3 t;
+  // the raw force of the block's TPB sites is one contiguous run, moved
+  // through LDS so that it is read and written contiguously
+  __shared__ double2 ft[9 * TPB];
+  const long s0 = (long)blockIdx.x * TPB;
+  const int n = V - s0 < TPB ? (int)(V - s0) : TPB;
+  double2 *Fb = F + 9 * s0;
+  for (int i = threadIdx.x; i < 9 * n; i += TPB) ft[i] = Fb[i];
+  __syncthreads();
+  if ((int)threadIdx.x < n) {
+    const long s = s0 + threadIdx.x;
+    const M3 a = mm<false, true>(ld(U, V, s), ld(D, V, s));
+    M3 t;
 #pragma unroll
-  for (int i = 0; i < 3; i++)
+    for (int i = 0; i < 3; i++)
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-      const double2 p = a.e[3 * i + j], q = a.e[3 * j + i];
-      t.e[3 * i + j] = make_double2(0.5 * (p.x - q.x), 0.5 * (p.y + q.y));
+      for (int j = 0; j < 3; j++) {
+        const double2 p = a.e[3 * i + j], q = a.e[3 * j + i];
+        t.e[3 * i + j] = make_double2(0.5 * (p.x - q.x), 0.5 * (p.y + q.y));
+      }
+    const double tr = (t.e[0].y + t.e[4].y + t.e[8].y) / 3.0;
+    t.e[0].y -= tr;
+    t.e[4].y -= tr;
+    t.e[8].y -= tr;
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+      double2 f = ft[9 * threadIdx.x + e];
+      f.x = fma(fs, t.e[e].x, f.x);
+      f.y = fma(fs, t.e[e].y, f.y);
+      ft[9 * threadIdx.x + e] = f;
     }
-  const double tr = (t.e[0].y + t.e[4].y + t.e[8].y) / 3.0;
-  t.e[0].y -= tr;
-  t.e[4].y -= tr;
-  t.e[8].y -= tr;
-#pragma unroll
-  for (int e = 0; e < 9; e++) {
-    double2 f = F[9 * s + e];
-    f.x = fma(fs, t.e[e].x, f.x);
-    f.y = fma(fs, t.e[e].y, f.y);
-    F[9 * s + e] = f;
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 9 * n; i += TPB) Fb[i] = ft[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -736,7 +747,7 @@ class ForceRun {
  public:
   ForceRun(const Lattice *lat, int nfields, const double *eps, const DevVec *const *xs, int n,
            hipStream_t st)
-      : lat_(lat), g_(lat->g), V_(lat->V), n9_(9 * lat->V), st_(st), sq_(make_sched(lat)),
+      : g_(lat->g), V_(lat->V), n9_(9 * lat->V), st_(st), sq_(make_sched(lat)),
         b_(TPB), gs_(grid(lat->V)), ge_(grid(9 * lat->V)), gx_(8 * sq_.per) {
     if (V_ >= (1L << 27))
       throw std::runtime_error("fermion force: lattice volume of 2^27 sites or more (32-bit offsets)");
@@ -860,7 +871,6 @@ class ForceRun {
                          mu, nu, mx);
   }
 
-  const Lattice *lat_;
   Geom g_;
   long V_, n9_;
   hipStream_t st_;
