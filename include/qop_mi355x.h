@@ -208,6 +208,14 @@ void QOP_D3_extract_G_to_raw(QOP_D_Real *dest[], QOP_D3_GaugeField *src, QOP_eve
 void QOP_F3_extract_G_to_raw(QOP_F_Real *dest[], QOP_F3_GaugeField *src, QOP_evenodd_t evenodd);
 void QOP_D3_destroy_G(QOP_D3_GaugeField *field); /* common_p.c:147-160 */
 void QOP_F3_destroy_G(QOP_F3_GaugeField *field);
+/* the field keeps the caller's raw arrays until convert_G_to_raw, which
+ * writes the field back into them and returns the same pointer array (an
+ * array of 4 malloc'd fields when the field was not converted from raw);
+ * the object is destroyed (common_p.c:199-208, 244-257) */
+QOP_D3_GaugeField *QOP_D3_convert_G_from_raw(QOP_D_Real *links[], QOP_evenodd_t evenodd); /* common_p.c:199-208 */
+QOP_F3_GaugeField *QOP_F3_convert_G_from_raw(QOP_F_Real *links[], QOP_evenodd_t evenodd);
+QOP_D_Real **QOP_D3_convert_G_to_raw(QOP_D3_GaugeField *src, QOP_evenodd_t evenodd); /* common_p.c:244-257 */
+QOP_F_Real **QOP_F3_convert_G_to_raw(QOP_F3_GaugeField *src, QOP_evenodd_t evenodd);
 /* staggered signs and boundary phases in place (common_p.c:464-526) */
 void QOP_D3_rephase_G(QOP_D3_GaugeField *links, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign);
 void QOP_F3_rephase_G(QOP_F3_GaugeField *links, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign);
@@ -220,6 +228,10 @@ void QOP_D3_extract_F_to_raw(QOP_D_Real *dest[], QOP_D3_Force *src, QOP_evenodd_
 void QOP_F3_extract_F_to_raw(QOP_F_Real *dest[], QOP_F3_Force *src, QOP_evenodd_t evenodd);
 void QOP_D3_destroy_F(QOP_D3_Force *field); /* common_p.c:160-169 */
 void QOP_F3_destroy_F(QOP_F3_Force *field);
+QOP_D3_Force *QOP_D3_convert_F_from_raw(QOP_D_Real *force[], QOP_evenodd_t evenodd); /* common_p.c:212-221 */
+QOP_F3_Force *QOP_F3_convert_F_from_raw(QOP_F_Real *force[], QOP_evenodd_t evenodd);
+QOP_D_Real **QOP_D3_convert_F_to_raw(QOP_D3_Force *src, QOP_evenodd_t evenodd); /* common_p.c:260-273 */
+QOP_F_Real **QOP_F3_convert_F_to_raw(QOP_F3_Force *src, QOP_evenodd_t evenodd);
 
 /* ---- asqtad links (include/qop_pc.h:217-271; lib/asqtad_dslash_p.c) -- */
 QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_raw(QOP_D_Real *fatlinks[], QOP_D_Real *longlinks[], QOP_evenodd_t evenodd); /* :307-329 */
@@ -227,13 +239,31 @@ QOP_F3_FermionLinksAsqtad *QOP_F3_asqtad_create_L_from_raw(QOP_F_Real *fatlinks[
 QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_G(QOP_info_t *info, QOP_asqtad_coeffs_t *coeffs, QOP_D3_GaugeField *gauge); /* :642-664 */
 QOP_F3_FermionLinksAsqtad *QOP_F3_asqtad_create_L_from_G(QOP_info_t *info, QOP_asqtad_coeffs_t *coeffs, QOP_F3_GaugeField *gauge);
 QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_G2(QOP_info_t *info, QOP_asqtad_coeffs_t *coeffs, QOP_D3_GaugeField *gFat, QOP_D3_GaugeField *gLong); /* :668-689 */
+QOP_F3_FermionLinksAsqtad *QOP_F3_asqtad_create_L_from_G2(QOP_info_t *info, QOP_asqtad_coeffs_t *coeffs, QOP_F3_GaugeField *gFat, QOP_F3_GaugeField *gLong);
 void QOP_D3_asqtad_extract_L_to_raw(QOP_D_Real *fatlinks[], QOP_D_Real *longlinks[], QOP_D3_FermionLinksAsqtad *src, QOP_evenodd_t evenodd); /* :916-938 */
 void QOP_F3_asqtad_extract_L_to_raw(QOP_F_Real *fatlinks[], QOP_F_Real *longlinks[], QOP_F3_FermionLinksAsqtad *src, QOP_evenodd_t evenodd);
+/* The link object takes the caller's raw arrays: they hold the links x 1/2
+ * from here on (the reference scales them in place, :152-223, 281-304) and
+ * must not be changed or freed until convert_L_to_raw, which writes the
+ * current links back into them (user normalisation, whole lattice), stores
+ * the caller's pointer arrays in *fatlinks / *longlinks (malloc'd arrays of
+ * malloc'd fields when the object was not converted from raw; *longlinks
+ * NULL without Naik links) and destroys the object.  The reference declares
+ * convert_L_to_raw (include/qop_pc.h:247-250) but defines no body. */
+QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_convert_L_from_raw(QOP_D_Real *fatlinks[], QOP_D_Real *longlinks[], QOP_evenodd_t evenodd); /* :281-304 */
+QOP_F3_FermionLinksAsqtad *QOP_F3_asqtad_convert_L_from_raw(QOP_F_Real *fatlinks[], QOP_F_Real *longlinks[], QOP_evenodd_t evenodd);
+void QOP_D3_asqtad_convert_L_to_raw(QOP_D_Real ***fatlinks, QOP_D_Real ***longlinks, QOP_D3_FermionLinksAsqtad *src, QOP_evenodd_t evenodd); /* include/qop_pc.h:247-250 */
+void QOP_F3_asqtad_convert_L_to_raw(QOP_F_Real ***fatlinks, QOP_F_Real ***longlinks, QOP_F3_FermionLinksAsqtad *src, QOP_evenodd_t evenodd);
 void QOP_D3_asqtad_load_L_from_raw(QOP_D3_FermionLinksAsqtad *asqtad, QOP_D_Real *fatlinks[], QOP_D_Real *longlinks[], QOP_evenodd_t evenodd); /* :332-360 */
+void QOP_F3_asqtad_load_L_from_raw(QOP_F3_FermionLinksAsqtad *asqtad, QOP_F_Real *fatlinks[], QOP_F_Real *longlinks[], QOP_evenodd_t evenodd);
 void QOP_D3_asqtad_load_L_from_G(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *asqtad, QOP_asqtad_coeffs_t *coeffs, QOP_D3_GaugeField *gauge); /* :857-875 */
+void QOP_F3_asqtad_load_L_from_G(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *asqtad, QOP_asqtad_coeffs_t *coeffs, QOP_F3_GaugeField *gauge);
+void QOP_D3_asqtad_load_L_from_G2(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *asqtad, QOP_asqtad_coeffs_t *coeffs, QOP_D3_GaugeField *gFat, QOP_D3_GaugeField *gLong); /* :878-896 */
+void QOP_F3_asqtad_load_L_from_G2(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *asqtad, QOP_asqtad_coeffs_t *coeffs, QOP_F3_GaugeField *gFat, QOP_F3_GaugeField *gLong);
 void QOP_D3_asqtad_destroy_L(QOP_D3_FermionLinksAsqtad *field); /* :941-968 */
 void QOP_F3_asqtad_destroy_L(QOP_F3_FermionLinksAsqtad *field);
 void QOP_D3_asqtad_rephase_L(QOP_D3_FermionLinksAsqtad *fla, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign); /* :1448-1482 */
+void QOP_F3_asqtad_rephase_L(QOP_F3_FermionLinksAsqtad *fla, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign);
 /* ---- HISQ links (include/qop_d3.h:474-485; lib/hisq_links_p.c:62-301) --
  * fat7 smearing, U(3) projection, asqtad smearing of the projected links:
  * get_asqtad_links_from_hisq returns n_naiks asqtad link objects (owned by
@@ -262,6 +292,20 @@ void QOP_D3_asqtad_dslash(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *asqtad, Q
 void QOP_F3_asqtad_dslash(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *asqtad, QOP_F_Real mass,
                           QOP_F3_ColorVector *out, QOP_F3_ColorVector *in, QOP_evenodd_t eo_out,
                           QOP_evenodd_t eo_in);
+/* One-direction transport (asqtad_dslash_p.c:1140-1191, 1338-1377), used
+ * for interpolating operators: on the sites of eo_out,
+ *   fb > 0: out += wtfat F(x)/2 in(x+dir) + wtlong L(x)/2 in(x+3 dir)
+ *   fb < 0: out += s [wtfat F(x-dir)^+/2 in(x-dir) + wtlong L(x-3dir)^+/2 in(x-3dir)]
+ * with s = -1 for the double-stored styles (1, 3; the default) and s = +1
+ * for the single-stored ones (0, 2), as the reference's two code paths do.
+ * out is accumulated into, not overwritten.  Without Naik links only the fat
+ * term is applied. */
+void QOP_D3_asqtad_dslash_dir(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *asqtad, int dir, int fb,
+                              double wtfat, double wtlong, QOP_D3_ColorVector *out,
+                              QOP_D3_ColorVector *in, QOP_evenodd_t eo_out);
+void QOP_F3_asqtad_dslash_dir(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *asqtad, int dir, int fb,
+                              double wtfat, double wtlong, QOP_F3_ColorVector *out,
+                              QOP_F3_ColorVector *in, QOP_evenodd_t eo_out);
 /* out = in/mass on eo (asqtad_dslash_p.c:1037-1104) */
 void QOP_D3_asqtad_diaginv(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *asqtad, QOP_D_Real mass,
                            QOP_D3_ColorVector *out, QOP_D3_ColorVector *in, QOP_evenodd_t eo);
@@ -322,6 +366,15 @@ int QOPX_D3_asqtad_invert_multi_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqta
                                       double masses[], int nmass, QOP_D3_ColorVector *out[],
                                       QOP_D3_ColorVector *in[], int nsrc, int capacity, int prec,
                                       double ms_tol, int *ms_iters);
+
+/* The hop out = D_{eo_out, 1-eo_out} in as the sloppy solvers apply it, in
+ * precision prec (64, 32, or 16: FP16 vectors with a per-site float norm and
+ * FP16 links with a per-link float scale), on n (1..64) right-hand sides in
+ * one multi-RHS launch of the autotuned shape; in[i] is rounded to the stored
+ * precision on the source parity, out[i] receives the result on eo_out in
+ * FP64.  Returns 0, or -1 when the device memory is exhausted. */
+int QOPX_D3_asqtad_hop(QOP_D3_FermionLinksAsqtad *fla, int prec, int n, QOP_D3_ColorVector *out[],
+                       QOP_D3_ColorVector *in[], QOP_evenodd_t eo_out);
 
 /* Mean ms per launch of the even->odd hop kernel (precision 64/32/16, nrhs
  * right-hand sides, autotuned shape (rhs per thread, block, XCD remap,
@@ -481,10 +534,18 @@ const char *QOPX_last_error(void);
 #  define QOP_extract_G_to_raw QOP_D3_extract_G_to_raw
 #  define QOP_destroy_G QOP_D3_destroy_G
 #  define QOP_rephase_G QOP_D3_rephase_G
+#  define QOP_convert_G_from_raw QOP_D3_convert_G_from_raw
+#  define QOP_convert_G_to_raw QOP_D3_convert_G_to_raw
+#  define QOP_convert_F_from_raw QOP_D3_convert_F_from_raw
+#  define QOP_convert_F_to_raw QOP_D3_convert_F_to_raw
 #  define QOP_asqtad_create_L_from_raw QOP_D3_asqtad_create_L_from_raw
 #  define QOP_asqtad_create_L_from_G QOP_D3_asqtad_create_L_from_G
 #  define QOP_asqtad_create_L_from_G2 QOP_D3_asqtad_create_L_from_G2
 #  define QOP_asqtad_extract_L_to_raw QOP_D3_asqtad_extract_L_to_raw
+#  define QOP_asqtad_convert_L_from_raw QOP_D3_asqtad_convert_L_from_raw
+#  define QOP_asqtad_convert_L_to_raw QOP_D3_asqtad_convert_L_to_raw
+#  define QOP_asqtad_load_L_from_G2 QOP_D3_asqtad_load_L_from_G2
+#  define QOP_asqtad_dslash_dir QOP_D3_asqtad_dslash_dir
 #  define QOP_asqtad_load_L_from_raw QOP_D3_asqtad_load_L_from_raw
 #  define QOP_asqtad_load_L_from_G QOP_D3_asqtad_load_L_from_G
 #  define QOP_asqtad_destroy_L QOP_D3_asqtad_destroy_L
@@ -519,9 +580,21 @@ const char *QOPX_last_error(void);
 #  define QOP_extract_G_to_raw QOP_F3_extract_G_to_raw
 #  define QOP_destroy_G QOP_F3_destroy_G
 #  define QOP_rephase_G QOP_F3_rephase_G
+#  define QOP_convert_G_from_raw QOP_F3_convert_G_from_raw
+#  define QOP_convert_G_to_raw QOP_F3_convert_G_to_raw
+#  define QOP_convert_F_from_raw QOP_F3_convert_F_from_raw
+#  define QOP_convert_F_to_raw QOP_F3_convert_F_to_raw
 #  define QOP_asqtad_create_L_from_raw QOP_F3_asqtad_create_L_from_raw
 #  define QOP_asqtad_create_L_from_G QOP_F3_asqtad_create_L_from_G
 #  define QOP_asqtad_extract_L_to_raw QOP_F3_asqtad_extract_L_to_raw
+#  define QOP_asqtad_convert_L_from_raw QOP_F3_asqtad_convert_L_from_raw
+#  define QOP_asqtad_convert_L_to_raw QOP_F3_asqtad_convert_L_to_raw
+#  define QOP_asqtad_load_L_from_G2 QOP_F3_asqtad_load_L_from_G2
+#  define QOP_asqtad_dslash_dir QOP_F3_asqtad_dslash_dir
+#  define QOP_asqtad_create_L_from_G2 QOP_F3_asqtad_create_L_from_G2
+#  define QOP_asqtad_load_L_from_raw QOP_F3_asqtad_load_L_from_raw
+#  define QOP_asqtad_load_L_from_G QOP_F3_asqtad_load_L_from_G
+#  define QOP_asqtad_rephase_L QOP_F3_asqtad_rephase_L
 #  define QOP_asqtad_destroy_L QOP_F3_asqtad_destroy_L
 #  define QOP_asqtad_dslash QOP_F3_asqtad_dslash
 #  define QOP_asqtad_diaginv QOP_F3_asqtad_diaginv

@@ -15,6 +15,10 @@
  *                                                          (examples/asqtad_test.c:37-81), brute-force dense
  *                                                          matrix on 4^4, anti-hermiticity, free-field spectrum,
  *                                                          gauge covariance
+ *   or_dslash_dir         lib/asqtad_dslash_p.c:1140-1377  sum over directions and fb of the one-direction
+ *                                                          transports = the hop and = check_op (m = 0)
+ *   or_links_set_single   lib/pcstubs/asqtad_invert_f3.c   stored values FP32-representable; FP32 solves
+ *                                                          agree with FP64 to FP32 accuracy
  *   or_ddagd_norm2        lib/asqtad_dslash_p.c:1121-1138  <p,Ap> identity, free-field spectrum
  *   or_invert (EVEN/ODD)  lib/asqtad_invert_p.c:246-647    free-field exact solution; check_resids identity
  *   or_invert (EVENODD)   lib/asqtad_invert_p.c:649-851    full-D residual ||in - D out|| and free field
@@ -305,7 +309,22 @@ struct or_links_s {
   int nlinks;
   double *dbl[16]; /* W_k(x), k = 4mu+{0:fat fwd, 1:long fwd, 2:fat bck, 3:long bck} */
   int disp[16], dir[16];
+  int single; /* FP32 storage of links and vectors (the F3 build), or_links_set_single */
 };
+
+/* A value as the link object stores it: rounded to FP32 in single mode. */
+static inline double ST(const or_links_t *l, double v) { return l->single ? (double)(float)v : v; }
+
+/* Single-precision restatement (lib/pcstubs/asqtad_invert_f3.c instantiates
+ * the same drivers with QOP_Precision 'F'): links and every vector the
+ * operator and the solvers store are rounded to FP32; sums, dot products
+ * and the solver scalars stay FP64 (QLA_D_Real, invert_cg_p.c:13-16). */
+void or_links_set_single(or_links_t *l, int on) {
+  l->single = on;
+  if (!on) return;
+  for (int k = 0; k < l->nlinks; k++)
+    for (long i = 0; i < 18 * l->V; i++) l->dbl[k][i] = (double)(float)l->dbl[k][i];
+}
 
 or_links_t *or_links_create(const int L[4], double *const fat[4],
                             double *const lng[4]) {
@@ -377,7 +396,7 @@ static void hop(or_links_t *l, double *dest, const double *src, int eo) {
       long j = shift_index(l->L, x, l->dir[k], l->disp[k]);
       mv_acc(acc, l->dbl[k] + 18 * i, tsrc + 6 * j);
     }
-    memcpy(dest + 6 * i, acc, sizeof acc);
+    for (int c = 0; c < 6; c++) dest[6 * i + c] = ST(l, acc[c]);
   }
   free(tsrc);
 }
@@ -387,7 +406,7 @@ static void mass_term(or_links_t *l, double *out, const double *in, double mass,
                       int add, int eo) {
   long b, e;
   sub_range(l, eo, &b, &e);
-  for (long i = 6 * b; i < 6 * e; i++) out[i] = mass * in[i] + (add ? out[i] : 0.0);
+  for (long i = 6 * b; i < 6 * e; i++) out[i] = ST(l, mass * in[i] + (add ? out[i] : 0.0));
 }
 
 void or_dslash(or_links_t *l, double mass, double *out, const double *in,
@@ -411,6 +430,36 @@ void or_dslash(or_links_t *l, double mass, double *out, const double *in,
     else if (eo_in == OR_ODD) mass_term(l, out, in, mass, 0, OR_ODD);
     else { hop(l, out, in, OR_ODD); mass_term(l, out, in, mass, 1, OR_ODD); }
   }
+}
+
+/* QOP_asqtad_dslash_dir (asqtad_dslash_p.c:1140-1191, style 1 :1338-1377):
+ * out(x) += r_j W_k(x) in(x + d_k) on the sites of eo_out for the fat and
+ * long link of direction dir, forward (fb > 0) or backward; bsign multiplies
+ * the backward terms: +1 with the stored -U^+ of the double-stored styles,
+ * -1 for style 0, whose backward term enters as +U^+ (:1280-1336).  Without
+ * long links only the fat term (the reference indexes past its links there). */
+void or_dslash_dir(or_links_t *l, int dir, int fb, double wtfat, double wtlong, double *out,
+                   const double *in, int eo_out, double bsign) {
+  long b, e;
+  sub_range(l, eo_out, &b, &e);
+  double *tsrc = malloc(6 * l->V * sizeof(double));
+  memcpy(tsrc, in, 6 * l->V * sizeof(double));
+  const int nper = l->nlinks == 16 ? 2 : 1;
+#pragma omp parallel for
+  for (long i = b; i < e; i++) {
+    int x[4];
+    or_raw_coords(l->L, i, x);
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < nper; j++) {
+      int k = l->nlinks == 16 ? 4 * dir + (fb > 0 ? 0 : 2) + j : 2 * dir + (fb > 0 ? 0 : 1);
+      long jn = shift_index(l->L, x, dir, l->disp[k]);
+      double w = (j ? wtlong : wtfat) * (fb > 0 ? 1.0 : bsign), t[6] = {0, 0, 0, 0, 0, 0};
+      mv_acc(t, l->dbl[k] + 18 * i, tsrc + 6 * jn);
+      for (int c = 0; c < 6; c++) acc[c] += w * t[c];
+    }
+    for (int c = 0; c < 6; c++) out[6 * i + c] = ST(l, out[6 * i + c] + acc[c]);
+  }
+  free(tsrc);
 }
 
 /* check_op restatement (examples/asqtad_test.c:37-81):
@@ -491,7 +540,7 @@ static void vaxpy(or_links_t *l, double *d, double a, const double *x,
   long b, e;
   sub_range(l, eo, &b, &e);
 #pragma omp parallel for
-  for (long i = 6 * b; i < 6 * e; i++) d[i] = a * x[i] + y[i];
+  for (long i = 6 * b; i < 6 * e; i++) d[i] = ST(l, a * x[i] + y[i]);
 }
 /* d += (cr + i ci) * x */
 static void vcaxpy(or_links_t *l, double *d, double cr, double ci,
@@ -502,8 +551,8 @@ static void vcaxpy(or_links_t *l, double *d, double cr, double ci,
   for (long i = b; i < e; i++)
     for (int c = 0; c < 3; c++) {
       double xr = x[6 * i + 2 * c], xi = x[6 * i + 2 * c + 1];
-      d[6 * i + 2 * c] += cr * xr - ci * xi;
-      d[6 * i + 2 * c + 1] += cr * xi + ci * xr;
+      d[6 * i + 2 * c] = ST(l, d[6 * i + 2 * c] + (cr * xr - ci * xi));
+      d[6 * i + 2 * c + 1] = ST(l, d[6 * i + 2 * c + 1] + (cr * xi + ci * xr));
     }
 }
 
@@ -518,7 +567,7 @@ double or_ddagd_norm2(or_links_t *l, double mass, double *out,
   or_dslash(l, mass, out, tmp, eo, oppsub(eo));
   long b, e;
   sub_range(l, eo, &b, &e);
-  for (long i = 6 * b; i < 6 * e; i++) out[i] = m2 * in[i] - out[i];
+  for (long i = 6 * b; i < 6 * e; i++) out[i] = ST(l, m2 * in[i] - out[i]);
   double in2 = norm2(l, in, eo), Ain2 = norm2(l, tmp, oppsub(eo));
   free(tmp);
   return Ain2 + m2 * in2;
@@ -741,7 +790,7 @@ static void invert_cgms(linop_ctx *lc, or_invert_arg_t *ia, or_resid_arg_t **ra,
       double zi = zn[i], ai = a[i];
       double *pmi = pm[i];
 #pragma omp parallel for
-      for (long k = 6 * bb; k < 6 * ee; k++) pmi[k] = ai * pmi[k] + zi * r[k];
+      for (long k = 6 * bb; k < 6 * ee; k++) pmi[k] = ST(l, ai * pmi[k] + zi * r[k]);
     }
     for (int i = 0; i < ns; i++) { bo[i] = b[i]; zo[i] = z[i]; z[i] = zn[i]; }
   }
@@ -824,6 +873,18 @@ static void refine(or_links_t *l, double **out, double *const *in, double **r,
   }
 }
 
+/* FP32 copy of an FP64 link object (QOP_FD_asqtad_create_L_from_L). */
+static or_links_t *links_copy_single(const or_links_t *l) {
+  or_links_t *f = calloc(1, sizeof *f);
+  *f = *l;
+  for (int k = 0; k < l->nlinks; k++) {
+    f->dbl[k] = malloc(18 * l->V * sizeof(double));
+    for (long i = 0; i < 18 * l->V; i++) f->dbl[k][i] = (double)(float)l->dbl[k][i];
+  }
+  f->single = 1;
+  return f;
+}
+
 static void solve_multi(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t **ra,
                         const double *masses, double **out, double *const *in,
                         int nsolve, double *final_flop) {
@@ -860,7 +921,7 @@ static void solve_multi(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t **ra,
   double m20 = m2s[0];
   linop_ctx lc = {l, sqrt(m20), ineo};
   for (int i = 0; i < nm; i++) {
-    m2s[i] -= m20;
+    m2s[i] = ST(l, m2s[i] - m20); /* QLA_Real shifts (:356) */
     x[i] = vnew(l);
     Ax[i] = vnew(l);
   }
@@ -897,10 +958,23 @@ static void solve_multi(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t **ra,
     if (ra[imax]->rsqmin >= 1) ra[imax]->rsqmin = in2max * r2stopmax / r2max;
     ra[imax]->rsqmin *= rsqminfac;
     ia->max_iter = max_iter - iter;
-    /* mixed_rsq: the reference runs an FP32 inner solve when r2 >
-     * mixed_rsq*in2 (:510-538); the oracle is FP64 only, which is the
-     * default (mixed_rsq = 1e9). */
-    invert_cgms(&lc, ia, xresarg, m2s, nm, x, r[imax], cgp);
+    if (!l->single && r2max > ia->mixed_rsq * in2max) {
+      /* FP32 inner multi-shift CG on an FP32 copy of the links, the residual
+       * and the shifts, when r2 > mixed_rsq in2 (:510-538; the links copy is
+       * QOP_FD_asqtad_create_L_from_L, asqtad_dslash_p.c:742-790) */
+      or_links_t *lf = links_copy_single(l);
+      if (ia->mixed_rsq > rsqminold) ra[imax]->rsqmin = ia->mixed_rsq * in2max / r2max;
+      double m2sf[nm], *rf = vnew(l), *xf[nm], *cgpf = vnew(l);
+      for (int i = 0; i < nm; i++) { m2sf[i] = (double)(float)m2s[i]; xf[i] = vnew(l); }
+      for (long q = 0; q < 6 * l->V; q++) rf[q] = (double)(float)r[imax][q];
+      linop_ctx lcf = {lf, sqrt(m20), ineo};
+      invert_cgms(&lcf, ia, xresarg, m2sf, nm, xf, rf, cgpf);
+      for (int i = 0; i < nm; i++) { vcopy(l, x[i], xf[i], ineo); free(xf[i]); }
+      free(rf); free(cgpf);
+      or_links_destroy(lf);
+    } else {
+      invert_cgms(&lc, ia, xresarg, m2s, nm, x, r[imax], cgp);
+    }
     iter += ra[imax]->final_iter;
     nrestart++;
     ra[imax]->rsqmin = rsqminold;
@@ -984,7 +1058,7 @@ static void invert_eo(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t *ra, do
   {
     long b, e;
     sub_range(l, cgeo, &b, &e);
-    for (long i = 6 * b; i < 6 * e; i++) qin[i] = mass * tmp2[i] - qin[i];
+    for (long i = 6 * b; i < 6 * e; i++) qin[i] = ST(l, mass * tmp2[i] - qin[i]);
   }
   if (ineo == cgeo) vzero(l, qin, oppsub(cgeo));
   else vcopy(l, qin, in, oppsub(cgeo));
@@ -1010,7 +1084,7 @@ static void invert_eo(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t *ra, do
       long b, e;
       sub_range(l, oe, &b, &e);
       double f = 1.0 / mass;
-      for (long i = 6 * b; i < 6 * e; i++) qout[i] = f * qout[i];
+      for (long i = 6 * b; i < 6 * e; i++) qout[i] = ST(l, f * qout[i]);
     }
     /* full residual (:797-799) */
     or_dslash(l, mass, cgr, qout, ineo, OR_EVENODD);
@@ -1067,7 +1141,7 @@ void or_invert_multi(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t *ra,
       if (shifts[i] < shifts[imin]) imin = i;
     }
     double gm = masses[imin];
-    for (int i = 0; i < 2; i++) shifts[i] -= gm * gm;
+    for (int i = 0; i < 2; i++) shifts[i] = ST(l, shifts[i] - gm * gm);
     linop_ctx lc = {l, gm, cgeo};
     double st = 1 / (shifts[1] - shifts[0]);
     double *x0 = vnew(l), *src = vnew(l);
@@ -1075,7 +1149,7 @@ void or_invert_multi(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t *ra,
     {
       long b, e;
       sub_range(l, cgeo, &b, &e);
-      for (long q = 6 * b; q < 6 * e; q++) x0[q] = st * x0[q];
+      for (long q = 6 * b; q < 6 * e; q++) x0[q] = ST(l, st * x0[q]);
     }
     vcopy(l, cgp, x0, cgeo);
     lin_plain(&lc, src, cgp);
@@ -1106,7 +1180,7 @@ void or_invert_multi(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t *ra,
         rap[j] = &ra[k0 + j];
       }
       double gm = masses[k0 + jmin];
-      for (int j = 0; j < nm; j++) shifts[j] -= gm * gm;
+      for (int j = 0; j < nm; j++) shifts[j] = ST(l, shifts[j] - gm * gm);
       linop_ctx lc = {l, gm, cgeo};
       invert_cgms(&lc, ia, rap, shifts, nm, out + k0, in[i], cgp);
       flop += (nflop + nflopm * (nm - 1)) * ra[k0].final_iter * (double)l->V;
@@ -1118,7 +1192,7 @@ void or_invert_multi(or_links_t *l, or_invert_arg_t *ia, or_resid_arg_t *ra,
     for (int j = 0; j < nmass[i]; j++, k++) {
       long b, e;
       sub_range(l, cgeo, &b, &e);
-      for (long q = 6 * b; q < 6 * e; q++) out[k][q] *= masses[k];
+      for (long q = 6 * b; q < 6 * e; q++) out[k][q] = ST(l, out[k][q] * masses[k]);
     }
   if (final_flop) *final_flop = flop;
   free(cgp);

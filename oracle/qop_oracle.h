@@ -56,6 +56,14 @@ void or_links_destroy(or_links_t *l);
 /* QOP_asqtad_dslash_qdp semantics (asqtad_dslash_p.c:1048-1092). */
 void or_dslash(or_links_t *l, double mass, double *out, const double *in,
                int eo_out, int eo_in);
+/* QOP_asqtad_dslash_dir (asqtad_dslash_p.c:1140-1377): out += one-direction
+ * transport on eo_out; bsign multiplies the backward terms (+1 style 1,
+ * -1 style 0). */
+void or_dslash_dir(or_links_t *l, int dir, int fb, double wtfat, double wtlong, double *out,
+                   const double *in, int eo_out, double bsign);
+/* FP32 storage of links and of every vector the operator and the solvers
+ * store (the single-precision build, lib/pcstubs/asqtad_invert_f3.c). */
+void or_links_set_single(or_links_t *l, int on);
 /* Independent restatement of the full operator from the user links,
  * following examples/asqtad_test.c:37-81 (check_op). Writes all sites. */
 void or_dslash_check_op(const int L[4], double *const fat[4],

@@ -145,6 +145,10 @@ SIGNATURES = [
     ("QOP_F3_extract_G_to_raw", None, [_PP, _VP, C.c_int]),
     ("QOP_D3_destroy_G", None, [_VP]),
     ("QOP_F3_destroy_G", None, [_VP]),
+    ("QOP_D3_convert_G_from_raw", _VP, [_PP, C.c_int]),
+    ("QOP_F3_convert_G_from_raw", _VP, [_PP, C.c_int]),
+    ("QOP_D3_convert_G_to_raw", C.c_void_p, [_VP, C.c_int]),
+    ("QOP_F3_convert_G_to_raw", C.c_void_p, [_VP, C.c_int]),
     ("QOP_D3_rephase_G", None, [_VP, C.POINTER(C.c_int), C.POINTER(QOPBc), C.POINTER(QOPSign)]),
     ("QOP_F3_rephase_G", None, [_VP, C.POINTER(C.c_int), C.POINTER(QOPBc), C.POINTER(QOPSign)]),
     ("QOP_D3_create_F_from_raw", _VP, [_PP, C.c_int]),
@@ -153,6 +157,10 @@ SIGNATURES = [
     ("QOP_F3_extract_F_to_raw", None, [_PP, _VP, C.c_int]),
     ("QOP_D3_destroy_F", None, [_VP]),
     ("QOP_F3_destroy_F", None, [_VP]),
+    ("QOP_D3_convert_F_from_raw", _VP, [_PP, C.c_int]),
+    ("QOP_F3_convert_F_from_raw", _VP, [_PP, C.c_int]),
+    ("QOP_D3_convert_F_to_raw", C.c_void_p, [_VP, C.c_int]),
+    ("QOP_F3_convert_F_to_raw", C.c_void_p, [_VP, C.c_int]),
     ("QOP_asqtad_force_set_opts", C.c_int, [C.POINTER(QOPOpt), C.c_int]),
     ("QOP_D3_asqtad_force", None, [C.POINTER(QOPInfo), _VP, _VP, C.POINTER(QOPAsqtadCoeffs), C.c_double, _VP]),
     ("QOP_F3_asqtad_force", None, [C.POINTER(QOPInfo), _VP, _VP, C.POINTER(QOPAsqtadCoeffs), C.c_float, _VP]),
@@ -165,13 +173,23 @@ SIGNATURES = [
     ("QOP_D3_asqtad_create_L_from_G", _VP, [C.POINTER(QOPInfo), C.POINTER(QOPAsqtadCoeffs), _VP]),
     ("QOP_F3_asqtad_create_L_from_G", _VP, [C.POINTER(QOPInfo), C.POINTER(QOPAsqtadCoeffs), _VP]),
     ("QOP_D3_asqtad_create_L_from_G2", _VP, [C.POINTER(QOPInfo), C.POINTER(QOPAsqtadCoeffs), _VP, _VP]),
+    ("QOP_F3_asqtad_create_L_from_G2", _VP, [C.POINTER(QOPInfo), C.POINTER(QOPAsqtadCoeffs), _VP, _VP]),
     ("QOP_D3_asqtad_extract_L_to_raw", None, [_PP, _PP, _VP, C.c_int]),
     ("QOP_F3_asqtad_extract_L_to_raw", None, [_PP, _PP, _VP, C.c_int]),
+    ("QOP_D3_asqtad_convert_L_from_raw", _VP, [_PP, _PP, C.c_int]),
+    ("QOP_F3_asqtad_convert_L_from_raw", _VP, [_PP, _PP, C.c_int]),
+    ("QOP_D3_asqtad_convert_L_to_raw", None, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), _VP, C.c_int]),
+    ("QOP_F3_asqtad_convert_L_to_raw", None, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), _VP, C.c_int]),
     ("QOP_D3_asqtad_load_L_from_raw", None, [_VP, _PP, _PP, C.c_int]),
+    ("QOP_F3_asqtad_load_L_from_raw", None, [_VP, _PP, _PP, C.c_int]),
     ("QOP_D3_asqtad_load_L_from_G", None, [C.POINTER(QOPInfo), _VP, C.POINTER(QOPAsqtadCoeffs), _VP]),
+    ("QOP_F3_asqtad_load_L_from_G", None, [C.POINTER(QOPInfo), _VP, C.POINTER(QOPAsqtadCoeffs), _VP]),
+    ("QOP_D3_asqtad_load_L_from_G2", None, [C.POINTER(QOPInfo), _VP, C.POINTER(QOPAsqtadCoeffs), _VP, _VP]),
+    ("QOP_F3_asqtad_load_L_from_G2", None, [C.POINTER(QOPInfo), _VP, C.POINTER(QOPAsqtadCoeffs), _VP, _VP]),
     ("QOP_D3_asqtad_destroy_L", None, [_VP]),
     ("QOP_F3_asqtad_destroy_L", None, [_VP]),
     ("QOP_D3_asqtad_rephase_L", None, [_VP, C.POINTER(C.c_int), C.POINTER(QOPBc), C.POINTER(QOPSign)]),
+    ("QOP_F3_asqtad_rephase_L", None, [_VP, C.POINTER(C.c_int), C.POINTER(QOPBc), C.POINTER(QOPSign)]),
     ("QOP_hisq_links_set_opts", C.c_int, [C.POINTER(QOPOpt), C.c_int]),
     ("QOP_D3_hisq_create_L_from_G", _VP, [C.POINTER(QOPInfo), C.POINTER(QOPHisqCoeffs), _VP]),
     ("QOP_F3_hisq_create_L_from_G", _VP, [C.POINTER(QOPInfo), C.POINTER(QOPHisqCoeffs), _VP]),
@@ -190,6 +208,10 @@ SIGNATURES = [
     ("QOP_DF3_asqtad_create_L_from_L", _VP, [_VP]),
     ("QOP_D3_asqtad_dslash", None, [C.POINTER(QOPInfo), _VP, C.c_double, _VP, _VP, C.c_int, C.c_int]),
     ("QOP_F3_asqtad_dslash", None, [C.POINTER(QOPInfo), _VP, C.c_float, _VP, _VP, C.c_int, C.c_int]),
+    ("QOP_D3_asqtad_dslash_dir", None, [C.POINTER(QOPInfo), _VP, C.c_int, C.c_int, C.c_double, C.c_double,
+                                        _VP, _VP, C.c_int]),
+    ("QOP_F3_asqtad_dslash_dir", None, [C.POINTER(QOPInfo), _VP, C.c_int, C.c_int, C.c_double, C.c_double,
+                                        _VP, _VP, C.c_int]),
     ("QOP_D3_asqtad_diaginv", None, [C.POINTER(QOPInfo), _VP, C.c_double, _VP, _VP, C.c_int]),
     ("QOP_F3_asqtad_diaginv", None, [C.POINTER(QOPInfo), _VP, C.c_float, _VP, _VP, C.c_int]),
     ("QOP_D3_asqtad_invert", None, [C.POINTER(QOPInfo), _VP, C.POINTER(QOPInvertArg),
@@ -208,6 +230,7 @@ SIGNATURES = [
     ("QOPX_D3_asqtad_invert_multi_batch", C.c_int, [C.POINTER(QOPInfo), _VP, C.POINTER(QOPInvertArg), _PP,
                                                     C.POINTER(C.c_double), C.c_int, _PP, _PP, C.c_int,
                                                     C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int)]),
+    ("QOPX_D3_asqtad_hop", C.c_int, [_VP, C.c_int, C.c_int, _PP, _PP, C.c_int]),
     ("QOPX_D3_asqtad_hop_time", C.c_double, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     ("QOPX_copy_time", C.c_double, [C.c_ulonglong, C.c_int]),
     ("QOPX_hop_shape_override", None, [C.POINTER(C.c_int)]),

@@ -409,6 +409,43 @@ void QOP_F3_destroy_G(QOP_F3_GaugeField *g) {
   for (int i = 0; i < 4; i++) if (g->U[i]) QOP_CHECK_HIP(hipFree(g->U[i]));
   delete g;
 }
+// common_p.c:199-208, 244-257: the field keeps the caller's arrays; to_raw
+// writes it back into them (or into new ones) and destroys it.
+extern "C++" {
+namespace {
+template <class GT, typename T>
+T **gauge_to_raw(GT *g, double *const dev[4], int eo, void (*destroy)(GT *)) {
+  T **r = g->raw;
+  const bool own = !r;
+  if (own) {
+    r = (T **)malloc(4 * sizeof(T *));
+    for (int i = 0; i < 4; i++) r[i] = (T *)calloc((size_t)g->lat->V * 18, sizeof(T));
+  }
+  sync();
+  for (int i = 0; i < 4; i++) download_mat(r[i], dev[i], g->lat, eo);
+  destroy(g);
+  return r;
+}
+}  // namespace
+}  // extern "C++"
+QOP_D3_GaugeField *QOP_D3_convert_G_from_raw(QOP_D_Real *links[], QOP_evenodd_t eo) {
+  QOP_D3_GaugeField *g = QOP_D3_create_G_from_raw(links, eo);
+  if (g) g->raw = links;
+  return g;
+}
+QOP_F3_GaugeField *QOP_F3_convert_G_from_raw(QOP_F_Real *links[], QOP_evenodd_t eo) {
+  QOP_F3_GaugeField *g = QOP_F3_create_G_from_raw(links, eo);
+  if (g) g->raw = links;
+  return g;
+}
+QOP_D_Real **QOP_D3_convert_G_to_raw(QOP_D3_GaugeField *src, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  return gauge_to_raw<QOP_D3_GaugeField, double>(src, src->U, eo, QOP_D3_destroy_G);
+}
+QOP_F_Real **QOP_F3_convert_G_to_raw(QOP_F3_GaugeField *src, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  return gauge_to_raw<QOP_F3_GaugeField, float>(src, src->U, eo, QOP_F3_destroy_G);
+}
 // common_p.c:455-526: sign uses absolute coordinates, bc on the last slice
 void QOP_D3_rephase_G(QOP_D3_GaugeField *g, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign) {
   CHECK_INIT;
@@ -501,6 +538,25 @@ void QOP_F3_extract_F_to_raw(QOP_F_Real *dest[], QOP_F3_Force *src, QOP_evenodd_
 }
 void QOP_D3_destroy_F(QOP_D3_Force *f) { destroy_force(f); }
 void QOP_F3_destroy_F(QOP_F3_Force *f) { destroy_force(f); }
+// common_p.c:212-221, 260-273
+QOP_D3_Force *QOP_D3_convert_F_from_raw(QOP_D_Real *force[], QOP_evenodd_t eo) {
+  QOP_D3_Force *f = QOP_D3_create_F_from_raw(force, eo);
+  if (f) f->raw = force;
+  return f;
+}
+QOP_F3_Force *QOP_F3_convert_F_from_raw(QOP_F_Real *force[], QOP_evenodd_t eo) {
+  QOP_F3_Force *f = QOP_F3_create_F_from_raw(force, eo);
+  if (f) f->raw = force;
+  return f;
+}
+QOP_D_Real **QOP_D3_convert_F_to_raw(QOP_D3_Force *src, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  return gauge_to_raw<QOP_D3_Force, double>(src, src->F, eo, QOP_D3_destroy_F);
+}
+QOP_F_Real **QOP_F3_convert_F_to_raw(QOP_F3_Force *src, QOP_evenodd_t eo) {
+  CHECK_INIT;
+  return gauge_to_raw<QOP_F3_Force, float>(src, src->F, eo, QOP_F3_destroy_F);
+}
 
 // lib/asqtad_force.c:22-43: the tags are validated and kept; every nsrc runs
 // the fnmat form here.
@@ -585,6 +641,15 @@ QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_create_L_from_G2(QOP_info_t *info, QOP_
   }
   return f;
 }
+QOP_F3_FermionLinksAsqtad *QOP_F3_asqtad_create_L_from_G2(QOP_info_t *info, QOP_asqtad_coeffs_t *c, QOP_F3_GaugeField *gf, QOP_F3_GaugeField *gl) {
+  CHECK_INIT;
+  auto *f = new QOP_F3_FermionLinksAsqtad;
+  if (!guard_oom(__func__, info, [&] { fatten(f->L, gf->lat, gf->U, gl->U, c, info); })) {
+    QOP_F3_asqtad_destroy_L(f);
+    return nullptr;
+  }
+  return f;
+}
 static void extract_links(Links &L, double *fat[], double *lng[], int eo, float *ffat[], float *flng[]) {
   Lattice *lat = L.lat;
   DevBuf<double> fb[4], lb[4];
@@ -621,6 +686,25 @@ void QOP_D3_asqtad_load_L_from_G(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *f,
   CHECK_INIT;
   guard_oom(__func__, info, [&] { fatten(f->L, g->lat, g->U, g->U, c, info); });
 }
+void QOP_F3_asqtad_load_L_from_raw(QOP_F3_FermionLinksAsqtad *f, QOP_F_Real *fat[], QOP_F_Real *lng[], QOP_evenodd_t eo) {
+  CHECK_INIT;
+  guard_oom(__func__, nullptr, [&] { build_links_from_raw(f->L, fat, lng, eo); });
+}
+void QOP_F3_asqtad_load_L_from_G(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *f, QOP_asqtad_coeffs_t *c, QOP_F3_GaugeField *g) {
+  CHECK_INIT;
+  guard_oom(__func__, info, [&] { fatten(f->L, g->lat, g->U, g->U, c, info); });
+}
+// asqtad_dslash_p.c:878-896: fat links from gFat, Naik links from gLong
+void QOP_D3_asqtad_load_L_from_G2(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *f, QOP_asqtad_coeffs_t *c,
+                                  QOP_D3_GaugeField *gf, QOP_D3_GaugeField *gl) {
+  CHECK_INIT;
+  guard_oom(__func__, info, [&] { fatten(f->L, gf->lat, gf->U, gl->U, c, info); });
+}
+void QOP_F3_asqtad_load_L_from_G2(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *f, QOP_asqtad_coeffs_t *c,
+                                  QOP_F3_GaugeField *gf, QOP_F3_GaugeField *gl) {
+  CHECK_INIT;
+  guard_oom(__func__, info, [&] { fatten(f->L, gf->lat, gf->U, gl->U, c, info); });
+}
 static void destroy_links(Links &L) {
   sync();
   L.free_all();
@@ -635,13 +719,94 @@ void QOP_F3_asqtad_destroy_L(QOP_F3_FermionLinksAsqtad *f) {
   destroy_links(f->L);
   delete f;
 }
+// ---- link objects that take the caller's raw arrays -----------------------
+// asqtad_dslash_p.c:281-304 -> 152-223: the object takes the arrays and scales
+// them in place by 1/2 (:201-206); the device keeps its own double-stored
+// copy, the arrays are written back by convert_L_to_raw.
+extern "C++" {
+namespace {
+template <typename T>
+void scale_raw(T *const a[4], double s) {
+  const size_t n = (size_t)G.lat->V * 18;
+  for (int i = 0; i < 4; i++)
+    for (size_t e = 0; e < n; e++) a[i][e] = (T)(s * a[i][e]);
+}
+template <class LT, typename T>
+LT *convert_links_from_raw(T *fat[], T *lng[], const char *fn) {
+  auto *f = new LT;
+  f->L.lat = G.lat;
+  if (!guard_oom(fn, nullptr, [&] { build_links_from_raw(f->L, fat, lng, QOP_EVENODD); })) {
+    destroy_links(f->L);
+    delete f;
+    return nullptr;
+  }
+  scale_raw(fat, 0.5);
+  if (lng) scale_raw(lng, 0.5);
+  f->raw_fat = fat;
+  f->raw_lng = lng;
+  return f;
+}
+template <typename T>
+T **alloc_raw4() {
+  T **a = (T **)malloc(4 * sizeof(T *));
+  for (int i = 0; i < 4; i++) a[i] = (T *)calloc((size_t)G.lat->V * 18, sizeof(T));
+  return a;
+}
+// The current links (user normalisation) go back into the arrays the object
+// was converted from, or into new ones; the object is destroyed.  On a device
+// allocation failure the object stays valid and *fat = *lng = NULL.
+template <class LT, typename T>
+void convert_links_to_raw(T ***fat, T ***lng, LT *src, const char *fn) {
+  T **rf = src->raw_fat, **rl = src->raw_lng;
+  const bool naik = src->L.nl == 16;
+  const bool own_f = !rf, own_l = naik && !rl;
+  if (own_f) rf = alloc_raw4<T>();
+  if (own_l) rl = alloc_raw4<T>();
+  bool ok;
+  if constexpr (sizeof(T) == 8)
+    ok = guard_oom(fn, nullptr, [&] { extract_links(src->L, rf, naik ? rl : nullptr, QOP_EVENODD, nullptr, nullptr); });
+  else
+    ok = guard_oom(fn, nullptr, [&] { extract_links(src->L, nullptr, nullptr, QOP_EVENODD, rf, naik ? rl : nullptr); });
+  if (!ok) {
+    auto drop = [](T **a) { for (int i = 0; i < 4; i++) free(a[i]); free(a); };
+    if (own_f) drop(rf);
+    if (own_l) drop(rl);
+    *fat = nullptr;
+    if (lng) *lng = nullptr;
+    return;
+  }
+  // long-link arrays of an object reloaded without Naik links: undo the 1/2
+  if (!naik && rl) scale_raw(rl, 2.0);
+  *fat = rf;
+  if (lng) *lng = rl;
+  destroy_links(src->L);
+  delete src;
+}
+}  // namespace
+}  // extern "C++"
+
+QOP_D3_FermionLinksAsqtad *QOP_D3_asqtad_convert_L_from_raw(QOP_D_Real *fat[], QOP_D_Real *lng[], QOP_evenodd_t) {
+  CHECK_INIT;
+  return convert_links_from_raw<QOP_D3_FermionLinksAsqtad>(fat, lng, __func__);
+}
+QOP_F3_FermionLinksAsqtad *QOP_F3_asqtad_convert_L_from_raw(QOP_F_Real *fat[], QOP_F_Real *lng[], QOP_evenodd_t) {
+  CHECK_INIT;
+  return convert_links_from_raw<QOP_F3_FermionLinksAsqtad>(fat, lng, __func__);
+}
+void QOP_D3_asqtad_convert_L_to_raw(QOP_D_Real ***fat, QOP_D_Real ***lng, QOP_D3_FermionLinksAsqtad *src, QOP_evenodd_t) {
+  CHECK_INIT;
+  convert_links_to_raw(fat, lng, src, __func__);
+}
+void QOP_F3_asqtad_convert_L_to_raw(QOP_F_Real ***fat, QOP_F_Real ***lng, QOP_F3_FermionLinksAsqtad *src, QOP_evenodd_t) {
+  CHECK_INIT;
+  convert_links_to_raw(fat, lng, src, __func__);
+}
+
 // asqtad_dslash_p.c:1385-1482: sign relative to r0, bc on the last slice for
 // fat links and the last three slices for long links; then re-double-store.
-void QOP_D3_asqtad_rephase_L(QOP_D3_FermionLinksAsqtad *f, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign) {
-  CHECK_INIT;
-  Links &L = f->L;
+static void rephase_links(Links &L, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign, const char *fn) {
   Lattice *lat = L.lat;
-  guard_oom(__func__, nullptr, [&] {
+  guard_oom(fn, nullptr, [&] {
     DevBuf<double> fb[4], lb[4];
     double *fd[4], *ld[4];
     size_t n = (size_t)lat->V * 18 * sizeof(double);
@@ -656,6 +821,14 @@ void QOP_D3_asqtad_rephase_L(QOP_D3_FermionLinksAsqtad *f, int *r0, QOP_bc_t *bc
     const double *fc[4] = {fd[0], fd[1], fd[2], fd[3]}, *lc[4] = {ld[0], ld[1], ld[2], ld[3]};
     build_links64(L, fc, L.nl == 16 ? lc : nullptr, stream());
   });
+}
+void QOP_D3_asqtad_rephase_L(QOP_D3_FermionLinksAsqtad *f, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign) {
+  CHECK_INIT;
+  rephase_links(f->L, r0, bc, sign, __func__);
+}
+void QOP_F3_asqtad_rephase_L(QOP_F3_FermionLinksAsqtad *f, int *r0, QOP_bc_t *bc, QOP_staggered_sign_t *sign) {
+  CHECK_INIT;
+  rephase_links(f->L, r0, bc, sign, __func__);
 }
 // asqtad_dslash_p.c:742-790: the single-precision copy of double links
 QOP_F3_FermionLinksAsqtad *QOP_FD3_asqtad_create_L_from_L(QOP_D3_FermionLinksAsqtad *src) {
@@ -732,6 +905,40 @@ void QOP_F3_asqtad_dslash(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *f, QOP_F_
     sync();
   });
 }
+// asqtad_dslash_p.c:1140-1191 (asqtad_hop_dir), :1338-1377 (style 1) and
+// :1280-1336 (style 0, whose backward term enters with + instead of -)
+static void dslash_dir_dev(Links &L, int dir, int fb, double wtfat, double wtlong, DevVec &out,
+                           const DevVec &in, int eo_out) {
+  QOP_REQUIRE(dir >= 0 && dir < 4, "dslash_dir: dir must be 0..3");
+  QOP_REQUIRE(out.prec == in.prec, "dslash_dir: precision mismatch");
+  const int prec = out.prec;
+  LinkSet &ls = L.get(prec);
+  const double bsign = (options().style & 1) ? 1.0 : -1.0;
+  std::unique_ptr<TmpVec> tcopy;  // the reference gathers from a copy of the source too
+  const DevVec *src = &in;
+  if (out.data == in.data) {
+    tcopy.reset(new TmpVec(in.lat, in.prec));
+    copy(tcopy->v, in, 2);
+    src = &tcopy->v;
+  }
+  for (int p = 0; p < 2; p++) {
+    if (eo_out != 2 && eo_out != p) continue;
+    launch_hop_dir(L.lat, ls, prec, p, dir, fb, wtfat, wtlong, bsign, out.half(p), src->half(1 - p), stream());
+  }
+  sync();
+}
+void QOP_D3_asqtad_dslash_dir(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *f, int dir, int fb, double wtfat,
+                              double wtlong, QOP_D3_ColorVector *out, QOP_D3_ColorVector *in,
+                              QOP_evenodd_t eo_out) {
+  CHECK_INIT;
+  guard_oom(__func__, info, [&] { dslash_dir_dev(f->L, dir, fb, wtfat, wtlong, out->v, in->v, eo_out); });
+}
+void QOP_F3_asqtad_dslash_dir(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *f, int dir, int fb, double wtfat,
+                              double wtlong, QOP_F3_ColorVector *out, QOP_F3_ColorVector *in,
+                              QOP_evenodd_t eo_out) {
+  CHECK_INIT;
+  guard_oom(__func__, info, [&] { dslash_dir_dev(f->L, dir, fb, wtfat, wtlong, out->v, in->v, eo_out); });
+}
 void QOP_D3_asqtad_diaginv(QOP_info_t *, QOP_D3_FermionLinksAsqtad *, QOP_D_Real mass,
                            QOP_D3_ColorVector *out, QOP_D3_ColorVector *in, QOP_evenodd_t eo) {
   CHECK_INIT;
@@ -783,8 +990,9 @@ void QOP_D3_asqtad_invert(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *f, QOP_in
   if (info) { info->final_sec = sec; info->final_flop = flop; info->status = QOP_SUCCESS; }
 }
 
-// Single precision: the solve runs with FP64 vectors on the FP64 master of
-// the links and the result is rounded to FP32.
+// Single precision: the solve iterates on the FP32 vectors with the FP32
+// links, as lib/pcstubs/asqtad_invert_f3.c instantiates lib/asqtad_invert_p.c
+// with QOP_Precision 'F' (scalars and reductions stay FP64).
 void QOP_F3_asqtad_invert(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *f, QOP_invert_arg_t *inv_arg,
                           QOP_resid_arg_t *res_arg, QOP_F_Real mass, QOP_F3_ColorVector *out,
                           QOP_F3_ColorVector *in) {
@@ -792,15 +1000,8 @@ void QOP_F3_asqtad_invert(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *f, QOP_in
   InvArg ia = to_ia(inv_arg);
   ResArg ra = to_ra(res_arg);
   double flop = 0, sec = 0;
-  const bool ok = guard_oom(__func__, info, [&] {
-    TmpVec o(out->v.lat, P64), i(in->v.lat, P64);
-    copy(o.v, out->v, 2);
-    copy(i.v, in->v, 2);
-    invert_dispatch(f->L, &ia, &ra, mass, o.v, i.v, &flop, &sec);
-    copy(out->v, o.v, inv_arg->evenodd);
-    sync();
-  });
-  if (!ok) return;
+  if (!guard_oom(__func__, info, [&] { invert_dispatch(f->L, &ia, &ra, mass, out->v, in->v, &flop, &sec); }))
+    return;
   from_ra(res_arg, ra);
   if (info) { info->final_sec = sec; info->final_flop = flop; info->status = QOP_SUCCESS; }
 }
@@ -848,35 +1049,26 @@ void QOP_F3_asqtad_invert_multi(QOP_info_t *info, QOP_F3_FermionLinksAsqtad *f, 
   std::vector<ResArg> ra(nsolve);
   std::vector<ResArg *> rap(nsolve);
   std::vector<double> ms(nsolve);
-  std::vector<std::unique_ptr<TmpVec>> ov(nsolve), iv(nsrc);
   std::vector<DevVec *> out(nsolve), in(nsrc);
-  double flop = 0, sec = 0;
-  const bool ok = guard_oom(__func__, info, [&] {
-    int k = 0;
-    for (int i = 0; i < nsrc; i++) {
-      iv[i].reset(new TmpVec(in_pt[i]->v.lat, P64));
-      copy(iv[i]->v, in_pt[i]->v, 2);
-      in[i] = &iv[i]->v;
-      for (int j = 0; j < nmass[i]; j++, k++) {
-        ra[k] = to_ra(res_arg[i][j]);
-        rap[k] = &ra[k];
-        ms[k] = masses[i][j];
-        ov[k].reset(new TmpVec(out_pt[i][j]->v.lat, P64));
-        copy(ov[k]->v, out_pt[i][j]->v, 2);
-        out[k] = &ov[k]->v;
-      }
+  int k = 0;
+  for (int i = 0; i < nsrc; i++) {
+    in[i] = &in_pt[i]->v;
+    for (int j = 0; j < nmass[i]; j++, k++) {
+      ra[k] = to_ra(res_arg[i][j]);
+      rap[k] = &ra[k];
+      ms[k] = masses[i][j];
+      out[k] = &out_pt[i][j]->v;
     }
-    invert_multi_dispatch(f->L, &ia, rap.data(), ms.data(), nmass, out.data(), in.data(), nsrc, &flop, &sec);
-    int cg = inv_arg->evenodd == QOP_EVENODD ? QOP_EVEN : inv_arg->evenodd;
-    k = 0;
-    for (int i = 0; i < nsrc; i++)
-      for (int j = 0; j < nmass[i]; j++, k++) {
-        copy(out_pt[i][j]->v, ov[k]->v, cg);
-        from_ra(res_arg[i][j], ra[k]);
-      }
-    sync();
-  });
-  if (!ok) return;
+  }
+  double flop = 0, sec = 0;
+  if (!guard_oom(__func__, info, [&] {
+        invert_multi_dispatch(f->L, &ia, rap.data(), ms.data(), nmass, out.data(), in.data(), nsrc, &flop,
+                              &sec);
+      }))
+    return;
+  k = 0;
+  for (int i = 0; i < nsrc; i++)
+    for (int j = 0; j < nmass[i]; j++, k++) from_ra(res_arg[i][j], ra[k]);
   if (info) { info->final_sec = sec; info->final_flop = flop; info->status = QOP_SUCCESS; }
 }
 

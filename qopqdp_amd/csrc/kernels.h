@@ -77,6 +77,14 @@ void launch_dot(const DotArgs &a, int pu, int pv, int mode, hipStream_t st);
 void launch_finalize(const double *partial, int m, int n, double *result,
                      hipStream_t st, const int *status);
 
+// One-direction transport onto out_parity (QOP_asqtad_dslash_dir):
+// out(x) += wf W_a(x) in(x+d_a) + wl W_b(x) in(x+d_b) with the double-stored
+// links of direction dir (a, b = fat and long, forward for fb > 0 and
+// backward otherwise; bsign multiplies the backward terms).  Only the fat
+// term without Naik links.
+void launch_hop_dir(const Lattice *lat, const LinkSet &ls, int prec, int out_parity, int dir, int fb,
+                    double wf, double wl, double bsign, VecView out, VecView in, hipStream_t st);
+
 // streaming copy of `bytes` (multiple of 16)
 void launch_copy(const void *in, void *out, size_t bytes, hipStream_t st);
 

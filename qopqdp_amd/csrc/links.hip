@@ -220,4 +220,57 @@ void launch_fatten(const Lattice *lat, const double *gf[4], const double *gl[4],
   fatten_dev(lat, gf, gl, c, fat, lng, st);
 }
 
+
+
+// ---------------------------------------------------------------------------
+// one-direction transport (lib/asqtad_dslash_p.c:1140-1191, 1338-1377): one
+// thread per output site; not on the solvers' path
+template <typename C>
+__global__ void hop_dir_kernel(Geom g, int parity, const C *__restrict__ links, int nl, int dir, int fb,
+                               double wf, double wl, double bsign, C *out, const C *__restrict__ in) {
+  const long h = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (h >= g.Vh) return;
+  const long Vh = g.Vh;
+  int x[4];
+  raw_coords_d(g, h + parity * Vh, x);
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const int nterm = nl == 16 ? 2 : 1;
+  for (int j = 0; j < nterm; j++) {
+    const int k = nl == 16 ? 4 * dir + (fb > 0 ? 0 : 2) + j : 2 * dir + (fb > 0 ? 0 : 1);
+    const int d = (j ? 3 : 1) * (fb > 0 ? 1 : -1);
+    const long nb = shift_d(g, x, dir, d) - (long)(1 - parity) * Vh;
+    const double w = (j ? wl : wf) * (fb > 0 ? 1.0 : bsign);
+    const C *W = links + (long)k * 9 * Vh + h;
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) {
+        const C m = W[(long)(3 * r + c) * Vh];
+        const C v = in[(long)c * Vh + nb];
+        acc[2 * r] += w * ((double)m.x * v.x - (double)m.y * v.y);
+        acc[2 * r + 1] += w * ((double)m.x * v.y + (double)m.y * v.x);
+      }
+  }
+  for (int r = 0; r < 3; r++) {
+    C o = out[(long)r * Vh + h];
+    o.x = o.x + acc[2 * r];
+    o.y = o.y + acc[2 * r + 1];
+    out[(long)r * Vh + h] = o;
+  }
+}
+
+void launch_hop_dir(const Lattice *lat, const LinkSet &ls, int prec, int out_parity, int dir, int fb,
+                    double wf, double wl, double bsign, VecView out, VecView in, hipStream_t st) {
+  QOP_REQUIRE(prec == P64 || prec == P32, "dslash_dir: FP64 or FP32 vectors");
+  QOP_REQUIRE(ls.prec == prec && (ls.nl == 16 || ls.nl == 8), "dslash_dir: link set");
+  dim3 g((lat->Vh + 255) / 256), b(256);
+  if (prec == P64)
+    hipLaunchKernelGGL(hop_dir_kernel<double2>, g, b, 0, st, lat->g, out_parity,
+                       (const double2 *)ls.parity_data(lat, out_parity), ls.nl, dir, fb, wf, wl, bsign,
+                       (double2 *)out.data, (const double2 *)in.data);
+  else
+    hipLaunchKernelGGL(hop_dir_kernel<float2>, g, b, 0, st, lat->g, out_parity,
+                       (const float2 *)ls.parity_data(lat, out_parity), ls.nl, dir, fb, wf, wl, bsign,
+                       (float2 *)out.data, (const float2 *)in.data);
+  QOP_CHECK_HIP(hipGetLastError());
+}
+
 }  // namespace qop

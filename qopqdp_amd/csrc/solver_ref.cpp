@@ -5,9 +5,11 @@
 //   solve_multi  -- lib/asqtad_invert_p.c:246-647 (with refine)
 //   invert       -- lib/asqtad_invert_p.c:134-144, 649-851
 //   invert_multi -- lib/asqtad_invert_p.c:853-1059
-// Vectors are FP64 device vectors; FP64 scalars throughout.  The FP32 branch
-// of solve_multi (mixed_rsq) runs the same multi-shift CG on FP32 vectors and
-// FP32 links, as the reference does.
+// The vectors of a solve have the precision of the caller's vectors: FP64 for
+// the QOP_D3 entry points, FP32 for QOP_F3 (lib/pcstubs/asqtad_invert_f3.c
+// instantiates the same drivers in single precision); the scalars are FP64
+// throughout.  The FP32 branch of an FP64 solve_multi (mixed_rsq) runs the
+// same multi-shift CG on FP32 vectors and FP32 links, as the reference does.
 #include <memory>
 #include <chrono>
 #include <cmath>
@@ -392,6 +394,7 @@ void solve_multi(Links &L, InvArg *ia, ResArg **ra, const double *masses, DevVec
                  DevVec *const *in, int nsolve, double *final_flop, double *final_sec) {
   Lattice *lat = L.lat;
   const int ineo = ia->evenodd;
+  const int bp = in[0]->prec;  // vector precision of this solve
   std::vector<double> in2(nsolve), r2(nsolve), r2stop(nsolve), mass2(nsolve), m2s(nsolve);
   std::vector<Cplx> scale(nsolve);
   std::vector<int> oi(nsolve), xi(nsolve);
@@ -400,7 +403,7 @@ void solve_multi(Links &L, InvArg *ia, ResArg **ra, const double *masses, DevVec
   const double mixedrsq = ia->mixed_rsq;
   double dtime = 0;
   std::vector<std::unique_ptr<TmpVec>> rv(nsolve), av(nsolve);
-  TmpVec tv(lat, P64);
+  TmpVec tv(lat, bp);
   for (int i = 0; i < nsolve; i++) {
     in2[i] = norm2(*in[i], ineo);
     r2stop[i] = ra[i]->rsqmin * in2[i];
@@ -413,8 +416,8 @@ void solve_multi(Links &L, InvArg *ia, ResArg **ra, const double *masses, DevVec
       m2s[j] = mass2[i];
       nm++;
     }
-    rv[i].reset(new TmpVec(lat, P64));
-    av[i].reset(new TmpVec(lat, P64));
+    rv[i].reset(new TmpVec(lat, bp));
+    av[i].reset(new TmpVec(lat, bp));
   }
   for (int i = 0; i < nsolve; i++) {
     int j = 0;
@@ -428,8 +431,9 @@ void solve_multi(Links &L, InvArg *ia, ResArg **ra, const double *masses, DevVec
   double gl_mass = std::sqrt(m20);
   for (int i = 0; i < nm; i++) {
     m2s[i] -= m20;
-    xv[i].reset(new TmpVec(lat, P64));
-    axv[i].reset(new TmpVec(lat, P64));
+    if (bp == P32) m2s[i] = (float)m2s[i];  // QLA_Real shifts of the F build (:356)
+    xv[i].reset(new TmpVec(lat, bp));
+    axv[i].reset(new TmpVec(lat, bp));
   }
   std::vector<DevVec *> r(nsolve), Aout(nsolve), x(nm), Ax(nm);
   for (int i = 0; i < nsolve; i++) { r[i] = &rv[i]->v; Aout[i] = &av[i]->v; }
@@ -467,7 +471,8 @@ void solve_multi(Links &L, InvArg *ia, ResArg **ra, const double *masses, DevVec
     if (ra[imax]->rsqmin >= 1) ra[imax]->rsqmin = in2max * r2stopmax / r2max;
     ra[imax]->rsqmin *= rsqminfac;
     ia->max_iter = max_iter - iter;
-    const bool mixed = r2max > mixedrsq * in2max;
+    // the FP32 inner branch exists in the double-precision build only (:510)
+    const bool mixed = bp == P64 && r2max > mixedrsq * in2max;
     double t0 = wall_time();
     if (mixed) {
       // FP32 inner multi-shift CG (:510-538)
@@ -482,7 +487,7 @@ void solve_multi(Links &L, InvArg *ia, ResArg **ra, const double *masses, DevVec
       invert_cgms(L, P32, gl_mass, ineo, ia, xresarg.data(), m2sf.data(), nm, xfp.data(), rf.v);
       for (int i = 0; i < nm; i++) { copy(*x[i], xf[i]->v, ineo); sync(); xf[i].reset(); }
     } else {
-      invert_cgms(L, P64, gl_mass, ineo, ia, xresarg.data(), m2s.data(), nm, x.data(), *r[imax]);
+      invert_cgms(L, bp, gl_mass, ineo, ia, xresarg.data(), m2s.data(), nm, x.data(), *r[imax]);
     }
     dtime += wall_time() - t0;
     iter += ra[imax]->final_iter;
@@ -530,6 +535,7 @@ void invert(Links &L, InvArg *ia, ResArg *ra, double mass, DevVec &out, const De
   Lattice *lat = L.lat;
   double nflop = 0.5 * (20 * 3 + 2 * 8 * 9 * L.nl);
   const int ineo = ia->evenodd;
+  const int bp = in.prec;
   int iter = 0, nrestart = -1;
   const int max_iter_old = ia->max_iter, max_restarts_old = ia->max_restarts;
   int max_restarts = ia->max_restarts;
@@ -540,7 +546,7 @@ void invert(Links &L, InvArg *ia, ResArg *ra, double mass, DevVec &out, const De
   double insq = insqe + insqo;
   const int cgeo = insqe >= insqo ? 0 : 1;
   const int oe = oppsub(cgeo);
-  TmpVec qin(lat, P64), qout(lat, P64), cgr(lat, P64), tmp2(lat, P64), gtmp(lat, P64);
+  TmpVec qin(lat, bp), qout(lat, bp), cgr(lat, bp), tmp2(lat, bp), gtmp(lat, bp);
   copy(tmp2.v, in, ineo);
   if (ineo != 2) zero(tmp2.v, oppsub(ineo));
   // project: qin = mass*tmp2 - D tmp2 on cgeo (:79-89)
@@ -564,7 +570,8 @@ void invert(Links &L, InvArg *ia, ResArg *ra, double mass, DevVec &out, const De
   double dtime = 0;
   do {
     double t0 = wall_time();
-    if (options().cgtype == 1) {
+    // eigCG keeps its deflation space in FP64; F3 solves run the plain CG
+    if (options().cgtype == 1 && bp == P64) {
       // eigCG inner solves (asqtad_invert_p.c:683-687, 767-769); the sizes
       // are taken from the options at every call, as the reference does
       if (!L.eig) L.eig = new EigcgState;
@@ -573,7 +580,7 @@ void invert(Links &L, InvArg *ia, ResArg *ra, double mass, DevVec &out, const De
       L.eig->nev = options().eigcg_nev;
       invert_eigcg(L, mass, cgeo, ia, ra, qout.v, qin.v, *L.eig);
     } else {
-      invert_cg(L, P64, mass, cgeo, ia, ra, qout.v, qin.v);
+      invert_cg(L, bp, mass, cgeo, ia, ra, qout.v, qin.v);
     }
     sync();
     dtime += wall_time() - t0;
@@ -625,6 +632,7 @@ void invert_multi(Links &L, InvArg *ia, ResArg **ra, const double *masses, const
     return;
   }
   double nflop = 0.5 * (20 * 3 + 2 * 8 * 9 * L.nl), nflopm = 0.5 * 10 * 3;
+  const int bp = in[0]->prec;
   const int cgeo = 0;  // EVENODD "not supported yet": uses EVEN (:938-941)
   fprintf(stderr, "warning: QOP_asqtad_invert_multi on ALL subset not supported yet!\n");
   double flop = 0;
@@ -639,11 +647,11 @@ void invert_multi(Links &L, InvArg *ia, ResArg **ra, const double *masses, const
     double gm = masses[imin];
     for (int i = 0; i < 2; i++) shifts[i] -= gm * gm;
     double st = 1 / (shifts[1] - shifts[0]);
-    TmpVec x0(lat, P64), src(lat, P64), cgp(lat, P64), tmp(lat, P64);
+    TmpVec x0(lat, bp), src(lat, bp), cgp(lat, bp), tmp(lat, bp);
     axpy(x0.v, -1.0, *in[0], *in[1], cgeo);
     scal(x0.v, Cplx{st, 0}, x0.v, cgeo);
     copy(cgp.v, x0.v, cgeo);
-    ddagd_norm2(L, P64, gm, src.v, cgp.v, cgeo, tmp.v);
+    ddagd_norm2(L, bp, gm, src.v, cgp.v, cgeo, tmp.v);
     axpy(src.v, -1.0, src.v, *in[imin], cgeo);
     double rsq1 = norm2(src.v, cgeo), rsqsave[2];
     for (int i = 0; i < 2; i++) {
@@ -651,7 +659,7 @@ void invert_multi(Links &L, InvArg *ia, ResArg **ra, const double *masses, const
       rsqsave[i] = ra[i]->rsqmin;
       ra[i]->rsqmin *= rsq0 / rsq1;
     }
-    invert_cgms(L, P64, gm, cgeo, ia, ra, shifts, 2, out, src.v);
+    invert_cgms(L, bp, gm, cgeo, ia, ra, shifts, 2, out, src.v);
     flop += (nflop + nflopm) * ra[imin]->final_iter * (double)lat->V;
     for (int i = 0; i < 2; i++) {
       ra[i]->rsqmin = rsqsave[i];
@@ -669,7 +677,7 @@ void invert_multi(Links &L, InvArg *ia, ResArg **ra, const double *masses, const
       }
       double gm = masses[k0 + jmin];
       for (int j = 0; j < nm; j++) shifts[j] -= gm * gm;
-      invert_cgms(L, P64, gm, cgeo, ia, ra + k0, shifts.data(), nm, out + k0, *in[i]);
+      invert_cgms(L, bp, gm, cgeo, ia, ra + k0, shifts.data(), nm, out + k0, *in[i]);
       flop += (nflop + nflopm * (nm - 1)) * ra[k0]->final_iter * (double)lat->V;
       k0 += nm;
     }

@@ -118,7 +118,73 @@ static double hop_time_impl(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, 
   return ms / nrep;
 }
 
+// the body of QOPX_D3_asqtad_hop (below)
+static void hop_impl(QOP_D3_FermionLinksAsqtad *fla, int prec, int n, QOP_D3_ColorVector *out[],
+                     QOP_D3_ColorVector *in[], int eo_out) {
+  Links &L = fla->L;
+  Lattice *lat = L.lat;
+  QOP_REQUIRE(prec == 64 || prec == 32 || prec == 16, "hop: precision must be 64, 32 or 16");
+  QOP_REQUIRE(n >= 1 && n <= 64, "hop: 1..64 vectors");
+  QOP_REQUIRE(eo_out == QOP_EVEN || eo_out == QOP_ODD, "hop: output subset must be EVEN or ODD");
+  const int p = prec == 64 ? P64 : prec == 32 ? P32 : P16;
+  const int src_par = 1 - eo_out;
+  DevBatch src, dst;
+  struct Free {
+    DevBatch *b[2];
+    ~Free() {
+      for (DevBatch *x : b) devbatch_free(*x);
+    }
+  } fr{{&src, &dst}};
+  devbatch_alloc(src, lat, p, n);
+  devbatch_alloc(dst, lat, p, n);
+  int pin[1] = {P64};
+  for (int b = 0; b < n; b++) {  // FP64 -> stored precision (FP16: per-site norm)
+    LincombArgs a;
+    a.Vh = (int)lat->Vh;
+    a.out = src.slot(b);
+    a.v[0] = in[b]->v.half(src_par);
+    a.cr[0] = 1.0;
+    launch_lincomb(a, p, pin, 1, false, stream());
+  }
+  HopShape hs = tune_hop(L, p, n, src, dst, eo_out, RED_NONE);
+  LinkSet &ls = L.get(p);
+  HopArgs a;
+  a.g = lat->g;
+  a.parity = eo_out;
+  a.links = ls.parity_data(lat, eo_out);
+  a.lscale = ls.parity_scale(lat, eo_out);
+  a.src = src.slot(0);
+  a.dst = dst.slot(0);
+  a.sstride = a.dstride = 3 * lat->Vh;
+  a.nstride = lat->Vh;
+  a.n = n;
+  hs.apply(a);
+  launch_hop_args(a, p, ls.nl, hs.nb, hs.block, RED_NONE, false, stream());
+  int pd[1] = {p};
+  for (int b = 0; b < n; b++) {  // stored precision -> FP64
+    LincombArgs c;
+    c.Vh = (int)lat->Vh;
+    c.out = out[b]->v.half(eo_out);
+    c.v[0] = dst.slot(b);
+    c.cr[0] = 1.0;
+    launch_lincomb(c, P64, pd, 1, false, stream());
+  }
+  sync();
+}
+
 extern "C" {
+
+// The hop out = D_{eo_out, 1-eo_out} in of the sloppy solvers, in precision
+// prec (64/32/16: FP16 vectors with a per-site norm and FP16 links with a
+// per-link scale) on n right-hand sides in one multi-RHS launch with the
+// autotuned shape (the kernel the batched solvers run): each in[i] is
+// rounded to the stored precision on the source parity, the result is
+// written to out[i] on eo_out in FP64.  Returns 0, or -1 when the device
+// memory is exhausted.
+int QOPX_D3_asqtad_hop(QOP_D3_FermionLinksAsqtad *fla, int prec, int n, QOP_D3_ColorVector *out[],
+                       QOP_D3_ColorVector *in[], QOP_evenodd_t eo_out) {
+  return guard_oom(__func__, nullptr, [&] { hop_impl(fla, prec, n, out, in, eo_out); }) ? 0 : -1;
+}
 
 // Time `nrep` launches of the even->odd hop kernel in precision prec
 // (64/32/16) on nrhs right-hand sides with the autotuned launch shape.

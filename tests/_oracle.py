@@ -71,6 +71,9 @@ class Oracle:
             L.or_links_destroy.argtypes = [C.c_void_p]
             L.or_dslash.argtypes = [C.c_void_p, C.c_double, _D, _D, C.c_int, C.c_int]
             L.or_dslash_check_op.argtypes = [_IV, _PP, _PP, C.c_double, _D, _D]
+            L.or_dslash_dir.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, _D, _D,
+                                        C.c_int, C.c_double]
+            L.or_links_set_single.argtypes = [C.c_void_p, C.c_int]
             L.or_ddagd_norm2.restype = C.c_double
             L.or_ddagd_norm2.argtypes = [C.c_void_p, C.c_double, _D, _D, C.c_int]
             L.or_invert.argtypes = [C.c_void_p, C.POINTER(InvArg), C.POINTER(ResArg), C.c_double,
@@ -162,6 +165,16 @@ class Oracle:
         self.lib.or_dslash(h, mass, out.reshape(-1), vin.reshape(-1), eo_out, eo_in)
         return out
 
+    def dslash_dir(self, h, dir, fb, wtfat, wtlong, vin, eo_out, out, bsign=1.0):
+        """out += one-direction transport (QOP_asqtad_dslash_dir); out is updated in place."""
+        vin = np.ascontiguousarray(vin, dtype=np.float64)
+        self.lib.or_dslash_dir(h, dir, fb, wtfat, wtlong, out.reshape(-1), vin.reshape(-1), eo_out, bsign)
+        return out
+
+    def set_single(self, h, on=True):
+        """FP32 storage of the links and of every stored vector (the F3 build)."""
+        self.lib.or_links_set_single(h, 1 if on else 0)
+
     def check_op(self, L, fat, lng, mass, vin):
         out = np.zeros_like(vin)
         self.lib.or_dslash_check_op(_iv(L), _pa(fat), _pa(lng) if lng else None, mass,
@@ -209,12 +222,12 @@ class Oracle:
         return out, ra
 
     def invert_multi(self, h, masses, vins, evenodd, rsqmin, max_iter=2000, restart=1000,
-                     max_restarts=5):
+                     max_restarts=5, relmin=0.0, mixed_rsq=1e9):
         """masses: list (per source) of lists of masses."""
-        ia = InvArg(1e9, max_iter, restart, max_restarts, evenodd)
+        ia = InvArg(mixed_rsq, max_iter, restart, max_restarts, evenodd)
         nmass = [len(m) for m in masses]
         ns = sum(nmass)
-        ra = (ResArg * ns)(*[ResArg(rsqmin, 0, 0, 0, 0, 0) for _ in range(ns)])
+        ra = (ResArg * ns)(*[ResArg(rsqmin, 0, relmin, 0, 0, 0) for _ in range(ns)])
         flat = np.array([m for ms in masses for m in ms], dtype=np.float64)
         outs = [np.zeros_like(vins[0]) for _ in range(ns)]
         ins = [np.ascontiguousarray(v) for v in vins]

@@ -131,3 +131,37 @@ def test_c_client_solves_on_device(tmp_path):
     words = r.stdout.split()
     assert words[0] == "iters", r.stdout
     assert int(words[1]) > 0 and float(words[3]) <= 1e-16
+
+
+# Families of the reference's non-QDP prototypes that are outside the asqtad
+# hot path, with the reason; everything else in the list must be exported in
+# both precisions.
+OUT_OF_SCOPE = {
+    "_D_": "DiracFermion fields serve the Wilson / domain-wall actions",
+    "symanzik_": "gauge action and gauge force",
+    "wilson": "Wilson / clover fermions",
+    "dw_": "domain-wall fermions",
+    "asqtad_invert_threaded": "compiled out in the reference (lib/asqtad_invert_p.c:146-195)",
+}
+
+
+def test_every_reference_asqtad_prototype_is_exported():
+    """Every per-precision prototype of include/qop_pc.h that takes no QDP
+    type (fixture written by tests/golden/make_qop_pc_names.py) and belongs
+    to the asqtad path (fields, links, Dslash, solvers, forces, HISQ links)
+    is exported as QOP_D3_* and QOP_F3_*."""
+    import json
+    fx = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                     "qop_pc_names.json")))
+    lib = q.lib()
+    names = fx["non_qdp"]
+    assert len(names) >= 60
+    checked = 0
+    for n in names:
+        probe = "_" + n + "_" if len(n) > 2 else n
+        if any(k in probe or n.startswith(k) for k in OUT_OF_SCOPE):
+            continue
+        for p in ("D3", "F3"):
+            assert hasattr(lib, f"QOP_{p}_{n}"), f"QOP_{p}_{n} not exported"
+        checked += 1
+    assert checked >= 35

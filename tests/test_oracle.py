@@ -427,3 +427,97 @@ def test_invert_multi_refine_quirk(oracle):
     for k in (2, 3):
         assert ras[k].final_rsq == pytest.approx(tr[k] ** 2, rel=1e-6)
     oracle.free_links(h)
+
+
+def test_dslash_dir_sums_to_hop(oracle):
+    """The eight one-direction transports (4 directions x fb) with unit
+    weights add up to the hop (style 1), which is also the independent
+    check_op stencil at m = 0; each call accumulates into out."""
+    L = [4, 4, 4, 6]
+    V = int(np.prod(L))
+    fat, lng = random_links(L, 81)
+    h = oracle.links(L, fat, lng)
+    v = rand_vec(L, 82)
+    for eo_out in (EVEN, ODD):
+        sl = slice(0, V // 2) if eo_out == EVEN else slice(V // 2, V)
+        start = rand_vec(L, 83)
+        out = start.copy()
+        for mu in range(4):
+            for fb in (1, -1):
+                oracle.dslash_dir(h, mu, fb, 1.0, 1.0, v, eo_out, out)
+        hop = oracle.dslash(h, 0.0, v, eo_out, 1 - eo_out)
+        ref = oracle.check_op(L, fat, lng, 0.0, v)
+        np.testing.assert_allclose(out[sl] - start[sl], hop[sl], rtol=0, atol=1e-12)
+        np.testing.assert_allclose(out[sl] - start[sl], ref[sl], rtol=0, atol=1e-12)
+        other = slice(V // 2, V) if eo_out == EVEN else slice(0, V // 2)
+        assert np.array_equal(out[other], start[other])  # only eo_out is touched
+    # weights: fat only (x2) equals twice the hop of the links without Naik
+    h8 = oracle.links(L, fat, None)
+    out = np.zeros((V, 3, 2))
+    for mu in range(4):
+        for fb in (1, -1):
+            oracle.dslash_dir(h, mu, fb, 2.0, 0.0, v, EVEN, out)
+    hop8 = oracle.dslash(h8, 0.0, v, EVEN, ODD)
+    np.testing.assert_allclose(out[: V // 2], 2 * hop8[: V // 2], rtol=0, atol=1e-12)
+    # style 0 sign (bsign -1): forward terms minus the stored backward ones
+    fwd = np.zeros((V, 3, 2))
+    for mu in range(4):
+        oracle.dslash_dir(h, mu, 1, 1.0, 1.0, v, EVEN, fwd)
+    s0 = np.zeros((V, 3, 2))
+    for mu in range(4):
+        for fb in (1, -1):
+            oracle.dslash_dir(h, mu, fb, 1.0, 1.0, v, EVEN, s0, bsign=-1.0)
+    hop = oracle.dslash(h, 0.0, v, EVEN, ODD)
+    np.testing.assert_allclose(s0[: V // 2], 2 * fwd[: V // 2] - hop[: V // 2], rtol=0, atol=1e-12)
+    oracle.free_links(h)
+    oracle.free_links(h8)
+
+
+def test_single_precision_mode(oracle):
+    """FP32 storage mode (the F3 build): every stored value is FP32
+    representable, the operator agrees with FP64 to FP32 accuracy, and an
+    EVEN solve reaches the FP32-limited residual with the FP64 solution."""
+    L = [4, 4, 4, 6]
+    V = int(np.prod(L))
+    fat, lng = smeared_links(oracle, L, 91)
+    h64 = oracle.links(L, fat, lng)
+    h32 = oracle.links(L, fat, lng)
+    oracle.set_single(h32)
+    v = rand_vec(L, 92).astype(np.float32).astype(np.float64)
+    d64 = oracle.dslash(h64, 0.3, v, EVENODD, EVENODD)
+    d32 = oracle.dslash(h32, 0.3, v, EVENODD, EVENODD)
+    assert np.array_equal(d32, d32.astype(np.float32).astype(np.float64))
+    err = np.linalg.norm(d32 - d64) / np.linalg.norm(d64)
+    assert 1e-9 < err < 1e-6
+    b = rand_vec(L, 93, EVEN).astype(np.float32).astype(np.float64)
+    x64, r64, _ = oracle.invert(h64, 0.1, b, EVEN, 1e-10)
+    x32, r32, _ = oracle.invert(h32, 0.1, b, EVEN, 1e-10)
+    assert np.array_equal(x32, x32.astype(np.float32).astype(np.float64))
+    tr = oracle.true_resid_even(h64, 0.1, x32, b)
+    assert tr < 1e-4
+    d = np.linalg.norm(x32[: V // 2] - x64[: V // 2]) / np.linalg.norm(x64[: V // 2])
+    assert d < 1e-4
+    assert abs(r32.final_iter - r64.final_iter) <= max(3, 0.1 * r64.final_iter)
+    oracle.free_links(h64)
+    oracle.free_links(h32)
+
+
+def test_mixed_rsq_branch(oracle):
+    """solve_multi's FP32 inner branch (asqtad_invert_p.c:510-538): with a
+    small mixed_rsq the first outer iteration runs in FP32 down to mixed_rsq,
+    the FP64 refinement finishes; the result is the FP64 solution."""
+    L = [4, 4, 4, 6]
+    V = int(np.prod(L))
+    fat, lng = smeared_links(oracle, L, 95)
+    h = oracle.links(L, fat, lng)
+    b = rand_vec(L, 96, EVEN)
+    x64, r64, _ = oracle.invert(h, 0.1, b, EVEN, 1e-20)
+    xm, rm, _ = oracle.invert(h, 0.1, b, EVEN, 1e-20, mixed_rsq=1e-8)
+    assert rm.final_restart >= 1  # an FP32 outer iteration and at least one FP64 one
+    assert oracle.true_resid_even(h, 0.1, xm, b) < 2e-9
+    d = np.linalg.norm(xm[: V // 2] - x64[: V // 2]) / np.linalg.norm(x64[: V // 2])
+    assert d < 1e-8
+    # the FP32 stage is real: stopping it at 1e-8 costs more outer work than
+    # the plain FP64 solve, which needs a single outer iteration
+    assert r64.final_restart == 0
+    oracle.free_links(h)
