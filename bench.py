@@ -69,7 +69,7 @@ def main():
     ap.add_argument("--shape", default=None,
                     help="--roofline-only: force the hop launch shape (8 comma-separated ints: "
                          "RHS per thread, block, XCD remap, non-temporal, RHS-innermost, x0 tile, "
-                         "LDS staging, x0 walk) instead of the autotuned one")
+                         "LDS staging, 0) instead of the autotuned one")
     ap.add_argument("--serve", action="store_true",
                     help="serving mix instead: Poisson arrivals over several lattices")
     ap.add_argument("--serve-lattices", default="16,16,16,16;24,24,24,48;32,32,32,64")

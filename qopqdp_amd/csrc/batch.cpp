@@ -35,14 +35,9 @@ static bool shape_valid(const Lattice *lat, int prec, int nl, int nrhs, const Ho
   if (s.block != 64 && s.block != 128 && s.block != 256) return false;
   if (s.tile && (s.tile % s.block != 0 || slab % s.tile != 0 || s.tile >= slab)) return false;
   if (s.stage) {
-    if (s.stage != 1 || prec == P64 || nl != 16 || s.nb > 8 || s.nt || s.walk) return false;
+    if (s.stage != 1 || prec == P64 || nl != 16 || s.nb > 8 || s.nt) return false;
     const size_t el = prec == P16 ? 16 : 24;  // stored bytes per site and RHS
     if ((size_t)s.nb * s.block * el > 65536) return false;
-  }
-  if (s.walk) {
-    const int L0 = lat->L[0];
-    if (L0 < 8 || (L0 / 2) % s.walk || slab % 64 || s.nt || s.tile) return false;
-    if (s.nb != 1 && s.nb != 2 && s.nb != 4) return false;
   }
   return true;
 }
@@ -139,17 +134,6 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
         if (shape_valid(lat, prec, L.nl, nrhs, c)) time_shape(c);
       }
   }
-  // stage 4: the x0-walking kernel (register ring of x0 neighbours)
-  const int L0 = lat->L[0];
-  if (L0 >= 8 && slab % 64 == 0)
-    for (int wnb : {1, 2, 4}) {
-      if (wnb > 1 && wnb > nrhs) continue;
-      for (int wblk : blocks)
-        for (int seg : {4, 8, 16}) {
-          if ((L0 / 2) % seg) continue;
-          time_shape(HopShape{wnb, wblk, 1, 0, nrhs > wnb ? 1 : 0, 0, 0, seg});
-        }
-    }
   QOP_CHECK_HIP(hipEventDestroy(e0));
   QOP_CHECK_HIP(hipEventDestroy(e1));
   if (scratch) {
@@ -157,9 +141,9 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
     QOP_CHECK_HIP(hipFree(scratch));
   }
   if (options().verbose)
-    printf("TUNE: prec %d nrhs %d -> nb %d block %d remap %d nt %d inner %d tile %d stage %d walk %d "
+    printf("TUNE: prec %d nrhs %d -> nb %d block %d remap %d nt %d inner %d tile %d stage %d "
            "(%.3f ms/hop)\n", prec, nrhs, best.nb, best.block, best.remap, best.nt, best.rhs_inner,
-           best.tile, best.stage, best.walk, best_ms / reps);
+           best.tile, best.stage, best_ms / reps);
   cache[key] = best;
   return best;
 }
@@ -236,7 +220,7 @@ BatchSolver::~BatchSolver() { release(); }
 void BatchSolver::release() {
   QOP_CHECK_HIP(hipStreamSynchronize(st_));
   if (ev_) (void)hipEventDestroy(ev_);
-  if (graph_) (void)hipGraphExecDestroy(graph_);
+  graph_.reset();
   for (DevBatch *b : {&x64_, &r64_, &b64_, &ax64_, &t64_, &rs_, &ps_, &ts_, &xs_}) devbatch_free(*b);
   (void)hipFree(rhs_);
   (void)hipHostFree(rhs_h_);
@@ -463,14 +447,9 @@ void BatchSolver::enqueue_iteration() {
 void BatchSolver::build_graph() {
   const int chunk = options().chunk > 0 ? options().chunk : 16;
   if (graph_ && graph_chunk_ == chunk) return;
-  if (graph_) QOP_CHECK_HIP(hipGraphExecDestroy(graph_));
-  hipStream_t st = st_;
-  hipGraph_t g;
-  QOP_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  graph_.begin(st_);
   for (int c = 0; c < chunk; c++) enqueue_iteration();
-  QOP_CHECK_HIP(hipStreamEndCapture(st, &g));
-  QOP_CHECK_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
-  QOP_CHECK_HIP(hipGraphDestroy(g));
+  graph_.end(st_);
   graph_chunk_ = chunk;
 }
 
@@ -481,7 +460,7 @@ void BatchSolver::launch_chunk() {
   if (!any) return;
   if (options().use_graph) {
     build_graph();
-    QOP_CHECK_HIP(hipGraphLaunch(graph_, st_));
+    graph_.launch(st_);
   } else {
     const int chunk = options().chunk > 0 ? options().chunk : 16;
     for (int c = 0; c < chunk; c++) enqueue_iteration();

@@ -70,14 +70,13 @@ void bs_launch_sum(const double *part, int nch, int n, const int *sel,
 // Tuned hop launch shape for (lattice, precision, nl, nrhs), timed on first use
 // and cached (batch.cpp).
 struct HopShape {
-  int nb = 1, block = 256, remap = 1, nt = 0, rhs_inner = 0, tile = 0, stage = 0, walk = 0;
+  int nb = 1, block = 256, remap = 1, nt = 0, rhs_inner = 0, tile = 0, stage = 0;
   void apply(HopArgs &a) const {
     a.remap = remap;
     a.nt = nt;
     a.rhs_inner = rhs_inner;
     a.tile = tile;
     a.stage = stage;
-    a.walk = walk;
   }
 };
 struct SlotResult {
@@ -155,7 +154,7 @@ class BatchSolver {
   double *sums_ = nullptr, *sums_h_ = nullptr;
   std::vector<int> loaded_;
   std::vector<double> mass_;
-  hipGraphExec_t graph_ = nullptr;
+  CapturedGraph graph_;
   int graph_chunk_ = 0;
   HopShape shape_;     // tuned launch shape of the first sloppy hop
   HopShape shape2_;    // tuned launch shape of the second (fused CG update)

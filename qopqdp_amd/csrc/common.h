@@ -70,6 +70,37 @@ struct DevBuf {
   }
 };
 
+// A stream-captured hipGraph and its executable instance, owned together:
+// the captured graph lives as long as the instance made from it and both are
+// destroyed in one place (instance first).  Profilers that trace graph
+// launches look the instance's nodes up in the captured graph at every
+// launch, so a captured graph destroyed right after instantiation leaves
+// them reading freed memory.
+struct CapturedGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  CapturedGraph() = default;
+  CapturedGraph(const CapturedGraph &) = delete;
+  CapturedGraph &operator=(const CapturedGraph &) = delete;
+  ~CapturedGraph() { reset(); }
+  explicit operator bool() const { return exec != nullptr; }
+  void begin(hipStream_t st) {
+    reset();
+    QOP_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  }
+  void end(hipStream_t st) {
+    QOP_CHECK_HIP(hipStreamEndCapture(st, &graph));
+    QOP_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  }
+  void launch(hipStream_t st) const { QOP_CHECK_HIP(hipGraphLaunch(exec, st)); }
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+};
+
 // Lattice geometry passed by value to kernels.
 struct Geom {
   int L[4];

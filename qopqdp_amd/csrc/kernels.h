@@ -30,7 +30,6 @@ struct HopArgs {
   int nt = 0;                    // non-temporal link loads
   int rhs_inner = 0;             // RHS groups innermost in the block order
   int tile = 0;                  // x0-tiled site order: sites per tile of an x0 slab (0: plain)
-  int walk = 0;                  // x0-walking kernel: sites per thread along x0 (0: off)
   int stage = 0;                 // LDS staging: the block's own source-parity sites are
                                  // loaded once into LDS and the neighbours inside that
                                  // range (x3 and most x2 hops) are read from there

@@ -437,155 +437,6 @@ hop_kernel(const HopArgs a) {
   }
 }
 
-// x0-walking hop: each thread walks `seg` output sites of one x0 column (fixed
-// slab position s, x0 = c, c+2, ..., every other x0 so the parity is kept)
-// and keeps the column's x0-neighbour source values, at x0-3, x0-1, x0+1 and
-// x0+3, in a register ring: the next site (x0+2) needs only x0+5 besides
-// three values already held.  The x0 gathers, whose reuse distance in the
-// plain order is one x0 slab (beyond L2), drop from four per site to about
-// one.  A wave covers 64 consecutive slab positions at one x0, i.e. one
-// 64-site reduction chunk per step, and every site is computed with the same
-// arithmetic in the same order as hop_kernel.
-template <int P, int NL, int NB, int RED, bool HASX>
-__global__ void __launch_bounds__(256)
-hop_walk_kernel(const HopArgs a) {
-  using R = typename PT<P>::R;
-  const Geom g = a.g;
-  const int Vh = g.Vh;
-  const int L0 = g.L[0];
-  const int S = Vh / L0;
-  const int seg = a.walk;
-  const int nseg = (L0 / 2) / seg;
-  const int lb = logical_block(a.remap);
-  const int wb = a.rhs_inner ? lb / a.ngroups : lb;
-  const int b0 = (a.rhs_inner ? lb % a.ngroups : (int)blockIdx.y) * NB;
-  if (a.status && *a.status != 0) return;
-  const long wid = (long)wb * blockDim.x + threadIdx.x;
-  const bool valid = wid < 2L * S * nseg;
-  const int s = valid ? (int)(wid % S) : 0;
-  const int rest = valid ? (int)(wid / S) : 0;
-  const int cls = rest & 1, jseg = rest >> 1;
-
-  bool act[NB];
-  bool any = false;
-#pragma unroll
-  for (int b = 0; b < NB; b++) {
-    act[b] = (b0 + b < a.n) && (!a.active || a.active[b0 + b]);
-    any |= act[b];
-  }
-  const bool work = valid && any;
-  constexpr int PER = NL / 4;
-  R ring[4][NB][6];
-  auto load_src = [&](int x0p, R (&v)[NB][6]) {
-    x0p = x0p < 0 ? x0p + L0 : (x0p >= L0 ? x0p - L0 : x0p);
-    const long hs = (long)x0p * S + s;
-#pragma unroll
-    for (int b = 0; b < NB; b++) {
-      if (!act[b]) continue;
-      const char *sd = (const char *)a.src.data + (long)(b0 + b) * a.sstride * sizeof(typename PT<P>::S);
-      const float *sn = a.src.norm ? a.src.norm + (long)(b0 + b) * a.nstride : nullptr;
-      vload<P>(sd, sn, hs, Vh, v[b]);
-    }
-  };
-  int x0 = cls + 2 * jseg * seg;
-  if (work) {
-    load_src(x0 - 3, ring[0]);
-    load_src(x0 - 1, ring[1]);
-    load_src(x0 + 1, ring[2]);
-  }
-#pragma unroll 1
-  for (int t = 0; t < seg; t++, x0 += 2) {
-    double red[NB];
-#pragma unroll
-    for (int b = 0; b < NB; b++) red[b] = 0.0;
-    const int h = x0 * S + s;
-    if (work) {
-      load_src(x0 + 3, ring[3]);
-      int x[4], lex;
-      site_coords(g, h, a.parity, x, lex);
-      R acc[NB][6];
-#pragma unroll
-      for (int b = 0; b < NB; b++)
-#pragma unroll
-        for (int i = 0; i < 6; i++) acc[b][i] = 0;
-      auto term = [&](int k, const R (&w)[18], const R (&v)[6], R (&ac)[6]) {
-#pragma unroll
-        for (int r = 0; r < 3; r++) {
-          R re = ac[2 * r], im = ac[2 * r + 1];
-#pragma unroll
-          for (int c = 0; c < 3; c++) {
-            const R wr = w[6 * r + 2 * c], wi = w[6 * r + 2 * c + 1];
-            re = fma(wr, v[2 * c], re);
-            re = fma(-wi, v[2 * c + 1], re);
-            im = fma(wr, v[2 * c + 1], im);
-            im = fma(wi, v[2 * c], im);
-          }
-          ac[2 * r] = re; ac[2 * r + 1] = im;
-        }
-        (void)k;
-      };
-      auto load_link = [&](int k, R (&w)[18]) {
-        const long lbase = (long)k * 9 * Vh + h;
-        float scl = 1.0f;
-        if constexpr (P == P16) scl = a.lscale[(long)k * Vh + h];
-#pragma unroll
-        for (int e = 0; e < 9; e++) ld_c<P, false>(a.links, lbase + (long)e * Vh, scl, w[2 * e], w[2 * e + 1]);
-      };
-      // mu = 0 from the ring: d = +1 -> x0+1, +3 -> x0+3, -1 -> x0-1, -3 -> x0-3
-#pragma unroll
-      for (int j = 0; j < PER; j++) {
-        const int d = PER == 4 ? (j == 0 ? 1 : j == 1 ? 3 : j == 2 ? -1 : -3) : (j == 0 ? 1 : -1);
-        const int ri = d == 1 ? 2 : d == 3 ? 3 : d == -1 ? 1 : 0;
-        R w[18];
-        load_link(j, w);
-#pragma unroll
-        for (int b = 0; b < NB; b++)
-          if (act[b]) term(j, w, ring[ri][b], acc[b]);
-      }
-#pragma unroll 1
-      for (int mu = 1; mu < 4; mu++) {
-#pragma unroll
-        for (int j = 0; j < PER; j++) {
-          const int d = PER == 4 ? (j == 0 ? 1 : j == 1 ? 3 : j == 2 ? -1 : -3) : (j == 0 ? 1 : -1);
-          const int k = PER * mu + j;
-          const int nb = nbr(g, x, lex, mu, d);
-          R w[18];
-          load_link(k, w);
-#pragma unroll
-          for (int b = 0; b < NB; b++) {
-            if (!act[b]) continue;
-            const char *sd = (const char *)a.src.data + (long)(b0 + b) * a.sstride * sizeof(typename PT<P>::S);
-            const float *sn = a.src.norm ? a.src.norm + (long)(b0 + b) * a.nstride : nullptr;
-            R v[6];
-            vload<P>(sd, sn, nb, Vh, v);
-            term(k, w, v, acc[b]);
-          }
-        }
-      }
-      hop_epilogue<P, NB, RED, HASX, false>(a, h, Vh, b0, act, acc, red);
-      // slide the ring by one x0 step of 2
-#pragma unroll
-      for (int b = 0; b < NB; b++)
-#pragma unroll
-        for (int i = 0; i < 6; i++) {
-          ring[0][b][i] = ring[1][b][i];
-          ring[1][b][i] = ring[2][b][i];
-          ring[2][b][i] = ring[3][b][i];
-        }
-    }
-    if constexpr (RED != RED_NONE) {
-      const long nchunk = (Vh + 63) / 64;
-      const int chunk = h >> 6;
-#pragma unroll
-      for (int b = 0; b < NB; b++) {
-        double sm = wave_sum(red[b]);
-        if ((threadIdx.x & 63) == 0 && valid && b0 + b < a.n)
-          a.partial[(long)(b0 + b) * nchunk + chunk] = sm;
-      }
-    }
-  }
-}
-
 int hop_partial_blocks(const Lattice *lat) { return (int)((lat->Vh + 63) / 64); }
 
 // LDS bytes of a staged block: NB RHS x blockDim sites x (3 stored complex
@@ -658,23 +509,6 @@ static void hop_dispatch_nb(const HopArgs &a, int nb, dim3 grid, dim3 block,
   }
 }
 
-template <int P, int NL>
-static void hop_walk_dispatch(const HopArgs &a, int nb, dim3 grid, dim3 block, int red, bool hasx,
-                              hipStream_t st) {
-#define QOP_WLAUNCH(NB_, RD, HX) \
-  hipLaunchKernelGGL((hop_walk_kernel<P, NL, NB_, RD, HX>), grid, block, 0, st, a)
-#define QOP_WRED(NB_)                                                                  \
-  if (red == RED_NONE) { if (hasx) QOP_WLAUNCH(NB_, RED_NONE, true); else QOP_WLAUNCH(NB_, RED_NONE, false); } \
-  else if (red == RED_OUT2) { if (hasx) QOP_WLAUNCH(NB_, RED_OUT2, true); else QOP_WLAUNCH(NB_, RED_OUT2, false); } \
-  else if (red == RED_CG) { QOP_WLAUNCH(NB_, RED_CG, true); }                          \
-  else { QOP_WLAUNCH(NB_, RED_X2, true); }
-  if (nb == 1) { QOP_WRED(1) }
-  else if (nb == 2) { QOP_WRED(2) }
-  else { QOP_WRED(4) }
-#undef QOP_WRED
-#undef QOP_WLAUNCH
-}
-
 void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
                      bool hasx, hipStream_t st) {
   QOP_REQUIRE(nb == 1 || nb == 2 || nb == 4 || nb == 8 || nb == 16, "hop: rhs per thread must be 1,2,4,8,16");
@@ -683,7 +517,7 @@ void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
   QOP_REQUIRE(red != RED_CG || (a.cg_alpha && a.xs.data && prec != P64), "hop: RED_CG needs alpha and xs");
   QOP_REQUIRE(a.n >= 1, "hop: n >= 1");
   if (a.stage) {
-    QOP_REQUIRE(prec != P64 && nl == 16 && nb <= 8 && !a.single && !a.nt && !a.walk,
+    QOP_REQUIRE(prec != P64 && nl == 16 && nb <= 8 && !a.single && !a.nt,
                 "hop: LDS staging needs FP32/FP16 Naik links, at most 8 RHS per thread and plain "
                 "link loads");
     QOP_REQUIRE(stage_bytes(prec, nb, block) <= 65536, "hop: LDS staging tile above 64 KiB");
@@ -693,16 +527,6 @@ void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
     QOP_REQUIRE(a.tile % block == 0 && slab % a.tile == 0, "hop: tile must divide the x0 slab");
   }
   int nblk = (a.g.Vh + block - 1) / block;
-  if (a.walk) {
-    const int L0 = a.g.L[0], slab = a.g.Vh / L0;
-    QOP_REQUIRE(!a.single && !a.nt && !a.tile, "hop: the x0 walk excludes single-stored links, "
-                "non-temporal loads and tiles");
-    QOP_REQUIRE(L0 >= 8 && (L0 / 2) % a.walk == 0 && slab % 64 == 0,
-                "hop: the x0 walk needs L0 >= 8, walk dividing L0/2 and 64 dividing the slab");
-    QOP_REQUIRE(nb == 1 || nb == 2 || nb == 4, "hop: the x0 walk takes 1, 2 or 4 RHS per thread");
-    const long nw = 2L * slab * ((L0 / 2) / a.walk);
-    nblk = (int)((nw + block - 1) / block);
-  }
   a.ngroups = (a.n + nb - 1) / nb;
   dim3 grid, blk(block);
   if (a.rhs_inner) {
@@ -712,20 +536,6 @@ void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
   } else {
     if (a.remap) nblk = (nblk + 7) / 8 * 8;
     grid = dim3(nblk, a.ngroups);
-  }
-  if (a.walk) {
-    if (prec == P64) {
-      if (nl == 16) hop_walk_dispatch<P64, 16>(a, nb, grid, blk, red, hasx, st);
-      else hop_walk_dispatch<P64, 8>(a, nb, grid, blk, red, hasx, st);
-    } else if (prec == P32) {
-      if (nl == 16) hop_walk_dispatch<P32, 16>(a, nb, grid, blk, red, hasx, st);
-      else hop_walk_dispatch<P32, 8>(a, nb, grid, blk, red, hasx, st);
-    } else {
-      if (nl == 16) hop_walk_dispatch<P16, 16>(a, nb, grid, blk, red, hasx, st);
-      else hop_walk_dispatch<P16, 8>(a, nb, grid, blk, red, hasx, st);
-    }
-    QOP_CHECK_HIP(hipGetLastError());
-    return;
   }
   if (prec == P64) {
     if (nl == 16) hop_dispatch_nb<P64, 16>(a, nb, grid, blk, red, hasx, st);

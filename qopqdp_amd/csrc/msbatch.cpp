@@ -69,7 +69,7 @@ MultiShiftBatch::~MultiShiftBatch() { release(); }
 
 void MultiShiftBatch::release() {
   sync();
-  if (graph_) (void)hipGraphExecDestroy(graph_);
+  graph_.reset();
   for (DevBatch *b : {&b64_, &rs_, &ps_, &ts_, &pm_, &xm_, &x64_, &t64_, &ax64_}) devbatch_free(*b);
   (void)hipFree(ru_);
   (void)hipFree(slot_);
@@ -232,18 +232,15 @@ void MultiShiftBatch::run() {
   hipStream_t st = stream();
   const int chunk = options().chunk > 0 ? options().chunk : 16;
   if (!graph_) {
-    hipGraph_t g;
-    QOP_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    graph_.begin(st);
     for (int c = 0; c < chunk; c++) enqueue_iteration();
-    QOP_CHECK_HIP(hipStreamEndCapture(st, &g));
-    QOP_CHECK_HIP(hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0));
-    QOP_CHECK_HIP(hipGraphDestroy(g));
+    graph_.end(st);
   }
   while (true) {
     bool any = false;
     for (int i = 0; i < cap_; i++) any |= active_h_[i] != 0;
     if (!any) break;
-    QOP_CHECK_HIP(hipGraphLaunch(graph_, st));
+    graph_.launch(st);
     QOP_CHECK_HIP(hipMemcpyAsync(active_h_, active_, cap_ * sizeof(int), hipMemcpyDeviceToHost, st));
     sync();
   }

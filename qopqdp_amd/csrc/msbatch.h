@@ -97,7 +97,7 @@ class MultiShiftBatch {
   double *partA_ = nullptr, *partB_ = nullptr, *partC_ = nullptr;
   double *coef_ = nullptr, *sums_ = nullptr;
   int *zero_status_ = nullptr;
-  hipGraphExec_t graph_ = nullptr;
+  CapturedGraph graph_;
   HopShape shape_;
 };
 

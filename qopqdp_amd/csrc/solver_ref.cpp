@@ -180,21 +180,18 @@ void invert_cgms(Links &L, int prec, double mass, int eo, InvArg *ia, ResArg **r
   };
 
   const int chunk = options().chunk > 0 ? options().chunk : 16;
-  hipGraphExec_t gexec = nullptr;
+  CapturedGraph gexec;
   if (options().use_graph) {
-    hipGraph_t graph;
-    QOP_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    gexec.begin(st);
     for (int c = 0; c < chunk; c++) enqueue_iteration();
-    QOP_CHECK_HIP(hipStreamEndCapture(st, &graph));
-    QOP_CHECK_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-    QOP_CHECK_HIP(hipGraphDestroy(graph));
+    gexec.end(st);
   }
 
   double yAy = 0;
   auto linop = [&](DevVec &o, const DevVec &i) { return ddagd_norm2(L, prec, mass, o, i, eo, tmp.v); };
   DevVec &xmin = *out[imin];
   while (true) {
-    if (gexec) QOP_CHECK_HIP(hipGraphLaunch(gexec, st));
+    if (gexec) gexec.launch(st);
     else for (int c = 0; c < chunk; c++) enqueue_iteration();
     B.pull(st);
     if (S.status == 2) break;
@@ -262,7 +259,7 @@ void invert_cgms(Links &L, int prec, double mass, int eo, InvArg *ia, ResArg **r
     B.push(st);
     cgms_launch_pupdate(B.d, prec, Vh, p.v.half(eo), r.v.half(eo), ns, st);
   }
-  if (gexec) QOP_CHECK_HIP(hipGraphExecDestroy(gexec));
+  gexec.reset();
   copy(xmin, y.v, eo);  // :378
   sync();
   for (int i = 0; i < ns; i++) {

@@ -34,7 +34,8 @@ def env(oracle):
     oracle.rephase_milc(L, U)
     fat, lng = oracle.make_imp_links(L, U, MILC)
     s = q.Session(L)
-    yield dict(L=L, s=s, U=U, U0=U_unphased, fat=fat, lng=lng)
+    saved = [x.copy() for x in U + U_unphased + fat + lng]
+    yield dict(L=L, s=s, U=U, U0=U_unphased, fat=fat, lng=lng, saved=saved)
     s.close()
 
 
@@ -98,8 +99,8 @@ def test_load_L_from_raw_replaces_links(env, oracle, prec):
     lng2 = [-0.7 * x for x in lng]
     fla = links(s, fat, lng, prec)
     dt = np.float64 if prec == "D" else np.float32
-    a = [np.ascontiguousarray(x, dtype=dt) for x in fat2]
-    b = [np.ascontiguousarray(x, dtype=dt) for x in lng2]
+    a = [np.array(x, dtype=dt) for x in fat2]
+    b = [np.array(x, dtype=dt) for x in lng2]
     fn = s.lib.QOP_D3_asqtad_load_L_from_raw if prec == "D" else s.lib.QOP_F3_asqtad_load_L_from_raw
     fn(fla, q.ptr_array(a), q.ptr_array(b), EVENODD)
     ref_fat = [x.astype(dt).astype(np.float64) for x in fat2]
@@ -242,8 +243,9 @@ def test_convert_L_takes_the_arrays(env, oracle, prec):
     returns them."""
     s, L = env["s"], env["L"]
     dt = np.float64 if prec == "D" else np.float32
-    fat = [np.ascontiguousarray(x, dtype=dt) for x in env["fat"]]
-    lng = [np.ascontiguousarray(x, dtype=dt) for x in env["lng"]]
+    # copies: the library scales and rewrites these arrays in place
+    fat = [np.array(x, dtype=dt) for x in env["fat"]]
+    lng = [np.array(x, dtype=dt) for x in env["lng"]]
     orig_f = [x.astype(np.float64) for x in fat]
     orig_l = [x.astype(np.float64) for x in lng]
     pf, pl = q.ptr_array(fat), q.ptr_array(lng)
@@ -366,7 +368,8 @@ def test_convert_G_and_F_round_trip(env, oracle, prec):
     s, L = env["s"], env["L"]
     dt = np.float64 if prec == "D" else np.float32
     T = "D3" if prec == "D" else "F3"
-    U = [np.ascontiguousarray(u, dtype=dt) for u in env["U0"]]
+    # copies: the field's values are written back into these arrays
+    U = [np.array(u, dtype=dt) for u in env["U0"]]
     Uref = [u.astype(np.float64) for u in U]
     pu = q.ptr_array(U)
     g = getattr(s.lib, f"QOP_{T}_convert_G_from_raw")(pu, EVENODD)
@@ -377,7 +380,7 @@ def test_convert_G_and_F_round_trip(env, oracle, prec):
     oracle.rephase_milc(L, Uref)
     for mu in range(4):
         np.testing.assert_array_equal(U[mu], Uref[mu].astype(dt))
-    F = [np.ascontiguousarray(u, dtype=dt) for u in env["U"]]
+    F = [np.array(u, dtype=dt) for u in env["U"]]
     Fref = [x.copy() for x in F]
     pf = q.ptr_array(F)
     fo = getattr(s.lib, f"QOP_{T}_convert_F_from_raw")(pf, EVENODD)
@@ -520,9 +523,17 @@ def test_f3_invert_multi(env, oracle, f3, eo):
         for j, m in enumerate(ms):
             sl = sub(s.V, EVEN)
             assert rel(res[i][j][sl], ores[k][sl]) < 1e-4
-            # EVENODD runs one CGMS per source and stops on the lightest shift
-            # (asqtad_invert_p.c:1012-1038), the heavier ones a little looser
-            assert oracle.true_resid_even(f3["h64"], m, res[i][j], srcs[i]) < (1e-4 if eo == EVEN else 3e-4)
+            tr = oracle.true_resid_even(f3["h64"], m, res[i][j], srcs[i])
+            if eo == EVEN:
+                assert tr < 1e-4
+            else:
+                # EVENODD runs one CGMS per source (asqtad_invert_p.c:1012-1038)
+                # whose stopping test takes the shift with the largest rsqmin
+                # and checks its true residual with the unshifted operator
+                # (invert_cg_p.c:164-172, 315-323): masses may stop short of
+                # the tolerance, and the GPU must then match the oracle
+                otr = oracle.true_resid_even(f3["h64"], m, ores[k], srcs[i])
+                assert tr == pytest.approx(otr, rel=2e-2)
             assert abs(ras[i][j].final_iter - oras[k].final_iter) <= max(3, 0.05 * oras[k].final_iter)
             k += 1
 
@@ -584,6 +595,20 @@ def test_two_source_invert_multi_matches_oracle(env, oracle, d3):
     ores, oras, _ = oracle.invert_multi(d3["h"], masses, srcs, EVENODD, 1e-20)
     sl = sub(s.V, EVEN)
     for i in range(2):
+        print(f"two-source {i}: GPU iters {ras[i][0].final_iter} rsq {ras[i][0].final_rsq:.3e} "
+              f"true {oracle.true_resid_even(d3['h'], masses[i][0], res[i][0], srcs[i]):.3e}; "
+              f"oracle iters {oras[i].final_iter} rsq {oras[i].final_rsq:.3e} "
+              f"true {oracle.true_resid_even(d3['h'], masses[i][0], ores[i], srcs[i]):.3e}; "
+              f"rel {rel(res[i][0][sl], ores[i][sl]):.3e}", flush=True)
+    for i in range(2):
         assert rel(res[i][0][sl], ores[i][sl]) < 1e-8
         assert oracle.true_resid_even(d3["h"], masses[i][0], res[i][0], srcs[i]) < 1e-7
         assert ras[i][0].final_iter == pytest.approx(oras[i].final_iter, abs=max(2, 0.02 * oras[i].final_iter))
+
+
+def test_shared_inputs_unchanged(env):
+    """No test above wrote into the module's shared gauge field and links
+    (entry points that take the caller's arrays are given copies)."""
+    now = env["U"] + env["U0"] + env["fat"] + env["lng"]
+    for a, b in zip(now, env["saved"]):
+        assert np.array_equal(a, b)

@@ -36,7 +36,7 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.fixture(scope="module")
+@pytest.fixture(scope="class")
 def c2(oracle):
     """32^3x64 gauge field, the oracle's fattened links and link object, and
     the library's links built from the same raw fat/long links."""
@@ -66,42 +66,46 @@ def gpu_hop(s, fla, prec, srcs, eo_out=ODD):
     return out
 
 
-def test_hop_fp32_fp16_against_oracle_32x32x32x64(c2, oracle):
-    s, fla, h = c2["s"], c2["fla"], c2["h"]
-    V = s.V
-    srcs = [q.gen_source(L2, 2000 + i, EVEN) for i in range(16)]
-    refs = [oracle.dslash(h, 0.0, x, ODD, EVEN)[V // 2:] for x in srcs[:4]]
-    errs = {}
-    for prec, bound in ((32, 1e-6), (16, 1e-3)):
-        one = gpu_hop(s, fla, prec, srcs[:1])[0][V // 2:]
-        many = gpu_hop(s, fla, prec, srcs)
-        e = [rel(many[i][V // 2:], refs[i]) for i in range(4)]
-        errs[prec] = max(e)
-        assert max(e) <= bound, (prec, e)
-        # the single-RHS launch gives the same bits as slot 0 of the 16-RHS one
-        assert np.array_equal(one, many[0][V // 2:])
-    # each format's error is its own: FP16 well above FP32 rounding
-    assert errs[16] > 30 * errs[32]
-    print(f"hop relative L2 vs FP64 oracle at 32^3x64: FP32 {errs[32]:.2e}, FP16 {errs[16]:.2e}")
+class TestConfig2:
+    """Both tests share one 32^3x64 session, closed before the tests below
+    open sessions of their own (QOP_init holds one lattice at a time)."""
+
+    def test_hop_fp32_fp16_against_oracle_32x32x32x64(self, c2, oracle):
+        s, fla, h = c2["s"], c2["fla"], c2["h"]
+        V = s.V
+        srcs = [q.gen_source(L2, 2000 + i, EVEN) for i in range(16)]
+        refs = [oracle.dslash(h, 0.0, x, ODD, EVEN)[V // 2:] for x in srcs[:4]]
+        errs = {}
+        for prec, bound in ((32, 1e-6), (16, 1e-3)):
+            one = gpu_hop(s, fla, prec, srcs[:1])[0][V // 2:]
+            many = gpu_hop(s, fla, prec, srcs)
+            e = [rel(many[i][V // 2:], refs[i]) for i in range(4)]
+            errs[prec] = max(e)
+            assert max(e) <= bound, (prec, e)
+            # the single-RHS launch gives the same bits as slot 0 of the 16-RHS one
+            assert np.array_equal(one, many[0][V // 2:])
+        # each format's error is its own: FP16 well above FP32 rounding
+        assert errs[16] > 30 * errs[32]
+        print(f"hop relative L2 vs FP64 oracle at 32^3x64: FP32 {errs[32]:.2e}, FP16 {errs[16]:.2e}")
 
 
-def test_fp16_batch_solves_to_1e10_32x32x32x64(c2, oracle):
-    """The headline workload: 16 sources, FP16 sloppy batched CG with FP64
-    reliable updates, m = 0.01, every solve to FP64 true residual 1e-10 by
-    the oracle's operator."""
-    from test_gpu_invert import batch_api
-    s, fla, h = c2["s"], c2["fla"], c2["h"]
-    m, tol = 0.01, 1e-10
-    srcs = [q.gen_source(L2, 2000 + i, EVEN) for i in range(16)]
-    ok, outs, ras = batch_api(s, fla, srcs, [m] * 16, tol, 16, capacity=16)
-    assert ok == 16
-    worst = 0.0
-    for b, o, r in zip(srcs, outs, ras):
-        tr = oracle.true_resid_even(h, m, o, b)
-        assert tr <= tol, tr
-        assert r.final_rsq == pytest.approx(tr * tr, rel=1e-2)
-        worst = max(worst, tr)
-    print(f"32^3x64 x16 FP16 batch: worst FP64 true residual (oracle) {worst:.3e}")
+    def test_fp16_batch_solves_to_1e10_32x32x32x64(self, c2, oracle):
+        """The headline workload: 16 sources, FP16 sloppy batched CG with FP64
+        reliable updates, m = 0.01, every solve to FP64 true residual 1e-10 by
+        the oracle's operator."""
+        from test_gpu_invert import batch_api
+        s, fla, h = c2["s"], c2["fla"], c2["h"]
+        m, tol = 0.01, 1e-10
+        srcs = [q.gen_source(L2, 2000 + i, EVEN) for i in range(16)]
+        ok, outs, ras = batch_api(s, fla, srcs, [m] * 16, tol, 16, capacity=16)
+        assert ok == 16
+        worst = 0.0
+        for b, o, r in zip(srcs, outs, ras):
+            tr = oracle.true_resid_even(h, m, o, b)
+            assert tr <= tol, tr
+            assert r.final_rsq == pytest.approx(tr * tr, rel=1e-2)
+            worst = max(worst, tr)
+        print(f"32^3x64 x16 FP16 batch: worst FP64 true residual (oracle) {worst:.3e}")
 
 
 MASSES_C3 = [0.005, 0.00846907, 0.014345, 0.0242978, 0.041156, 0.0697106, 0.118077, 0.2]

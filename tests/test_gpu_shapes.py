@@ -1,8 +1,7 @@
 """The batched solver's result does not depend on the multi-RHS hop launch
 shape (RHS per thread, block size, XCD remap, non-temporal loads and stores,
-block order, x0-tiled site order, LDS staging of the block's source rows,
-x0-walking kernel): each forced shape gives bit-identical solutions and
-iteration counts, all at FP64 true residual 1e-10 (oracle operator).  The
+block order, x0-tiled site order, LDS staging of the block's source rows):
+each forced shape gives bit-identical solutions and iteration counts, all at FP64 true residual 1e-10 (oracle operator).  The
 second lattice has x3 rows of 5 sites, so staged blocks end inside a row and
 gathers switch between the LDS tile and memory within a wavefront."""
 import ctypes as C
@@ -17,7 +16,7 @@ pytestmark = pytest.mark.gpu
 EVEN = 0
 
 # (RHS per thread, block, XCD remap, non-temporal, RHS-innermost, x0 tile,
-#  LDS staging, x0 walk)
+#  LDS staging, reserved 0)
 SHAPES = [
     (4, 128, 1, 0, 1, 0, 0, 0),
     (4, 128, 1, 0, 1, 512, 0, 0),
@@ -30,9 +29,6 @@ SHAPES = [
     (1, 64, 1, 0, 1, 0, 1, 0),
     (4, 128, 1, 0, 1, 512, 1, 0),
     (2, 256, 0, 0, 0, 1024, 1, 0),
-    (2, 128, 1, 0, 1, 0, 0, 4),
-    (4, 64, 1, 0, 1, 0, 0, 4),
-    (1, 256, 0, 0, 0, 0, 0, 4),
 ]
 
 
