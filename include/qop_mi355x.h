@@ -376,6 +376,23 @@ int QOPX_D3_asqtad_invert_multi_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqta
 int QOPX_D3_asqtad_hop(QOP_D3_FermionLinksAsqtad *fla, int prec, int n, QOP_D3_ColorVector *out[],
                        QOP_D3_ColorVector *in[], QOP_evenodd_t eo_out);
 
+/* QOPX_D3_asqtad_hop without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * stream capture, serving loops that must not reach the allocator).  Same
+ * kernels and bit-identical results; the stored-precision source and result
+ * batches live in a 64 MiB device array held by the library for the whole
+ * process on every device it runs on, so 2 x n vectors of Vh sites in the
+ * stored precision (48 B per site in FP64, 24 B in FP32, 16 B in FP16) must fit
+ * in 64 MiB.  The links in precision prec must exist already (a
+ * QOPX_D3_asqtad_hop call in that precision builds them); the launch shape
+ * is the tuned one when that call tuned it, a plain one otherwise.  Calls on
+ * one device share the array: calls that overlap in time on one device race.
+ * Returns 0, or -2 when the vectors do not fit or the links in that precision
+ * are not built (nothing is computed then). */
+int QOPX_D3_asqtad_hop_noalloc(QOP_D3_FermionLinksAsqtad *fla, int prec, int n,
+                               QOP_D3_ColorVector *out[], QOP_D3_ColorVector *in[],
+                               QOP_evenodd_t eo_out);
+
 /* Mean ms per launch of the even->odd hop kernel (precision 64/32/16, nrhs
  * right-hand sides, autotuned shape (rhs per thread, block, XCD remap,
  * non-temporal links, RHS-innermost order, x0 tile, LDS staging, 0 = reserved)

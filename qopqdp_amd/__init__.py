@@ -231,6 +231,7 @@ SIGNATURES = [
                                                     C.POINTER(C.c_double), C.c_int, _PP, _PP, C.c_int,
                                                     C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int)]),
     ("QOPX_D3_asqtad_hop", C.c_int, [_VP, C.c_int, C.c_int, _PP, _PP, C.c_int]),
+    ("QOPX_D3_asqtad_hop_noalloc", C.c_int, [_VP, C.c_int, C.c_int, _PP, _PP, C.c_int]),
     ("QOPX_D3_asqtad_hop_time", C.c_double, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     ("QOPX_copy_time", C.c_double, [C.c_ulonglong, C.c_int]),
     ("QOPX_hop_shape_override", None, [C.POINTER(C.c_int)]),

@@ -42,9 +42,23 @@ static bool shape_valid(const Lattice *lat, int prec, int nl, int nrhs, const Ho
   return true;
 }
 
+static std::map<std::tuple<const Lattice *, int, int, int, int>, HopShape> g_tune_cache;
+
+bool cached_hop_shape(Links &L, int prec, int nrhs, int red, HopShape *out) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (g_override && shape_valid(L.lat, prec, L.nl, nrhs, g_override_shape)) {
+    *out = g_override_shape;
+    return true;
+  }
+  auto it = g_tune_cache.find(std::make_tuple((const Lattice *)L.lat, prec, L.nl, nrhs, red));
+  if (it == g_tune_cache.end()) return false;
+  *out = it->second;
+  return true;
+}
+
 HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &dst, int eo_out,
                   int red, const DevBatch *x, const DevBatch *xs) {
-  static std::map<std::tuple<const Lattice *, int, int, int, int>, HopShape> cache;
+  auto &cache = g_tune_cache;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   if (g_override && shape_valid(L.lat, prec, L.nl, nrhs, g_override_shape)) return g_override_shape;
   auto key = std::make_tuple((const Lattice *)L.lat, prec, L.nl, nrhs, red);

@@ -167,5 +167,8 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
                   int eo_out, int red = RED_NONE, const DevBatch *x = nullptr,
                   const DevBatch *xs = nullptr);
 void set_hop_shape_override(const HopShape *s);  // null: tune again
+// The shape tune_hop chose for (links, prec, nrhs, red), if it has run (or the
+// forced shape): no timing and no allocation.
+bool cached_hop_shape(Links &L, int prec, int nrhs, int red, HopShape *out);
 
 }  // namespace qop
