@@ -31,6 +31,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -64,6 +65,12 @@ def main():
                          "batch-invariant results)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the solver iterations one by one instead of replaying "
+                         "captured hipGraphs (option qmx_graph = 0)")
+    ap.add_argument("--single-rhs-only", action="store_true",
+                    help="time the single-RHS hop in FP64/FP32/FP16 (100 launches each) and the "
+                         "1 GiB copy calibration, print one JSON line and exit")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only time the dominant hop kernel (for the rocprofv3 cross-check)")
     ap.add_argument("--shape", default=None,
@@ -112,6 +119,8 @@ def main():
     t0 = time.time()
     U = q.gen_gauge(L, 1000 + 1 * 10 + 0, 0.3)  # gauge seed 1000 + cfg*10 + k
     s = q.Session(L)
+    if args.no_graph:
+        assert s.set_opts(qmx_graph=0) == 0
     g = s.gauge(U)
     del U
     r0, bc, sign, keep = q.milc_rephase_args(L)
@@ -120,6 +129,21 @@ def main():
     s.lib.QOP_D3_destroy_G(g)
     log(f"setup {time.time() - t0:.1f}s (fattening {finfo.final_sec * 1e3:.1f} ms on GPU)")
 
+    if args.single_rhs_only:
+        res = {}
+        for prec, sl, sv in ((64, 144, 48), (32, 72, 24), (16, 40, 16)):
+            ms = s.lib.QOPX_D3_asqtad_hop_time(fla, prec, 1, 100, 0, None)
+            byts = (16 * sl + 2 * sv) * Vh
+            res[f"fp{prec}"] = {"kernel_us": round(ms * 1e3, 1), "GBs": round(byts / ms / 1e6, 1),
+                                "algorithmic_bytes": byts}
+        cbytes = 1 << 30
+        cms = s.lib.QOPX_copy_time(cbytes, 20)
+        res["copy"] = {"bytes_read": cbytes, "us": round(cms * 1e3, 1),
+                       "GBs_rw": round(2 * cbytes / cms / 1e6, 1)}
+        print(json.dumps(res))
+        s.lib.QOP_D3_asqtad_destroy_L(fla)
+        s.close()
+        return
     if args.roofline_only:
         res = {}
         if args.shape:
@@ -341,6 +365,8 @@ def multi_bench(args):
     t0 = time.time()
     U = q.gen_gauge(L, 1000 + 10 * 4 + 0, 0.3)
     s = q.Session(L)
+    if args.no_graph:
+        assert s.set_opts(qmx_graph=0) == 0
     g = s.gauge(U)
     del U
     r0, bc, sign, keep = q.milc_rephase_args(L)
@@ -666,11 +692,27 @@ def serve_bench(args):
 
 
 def cpu_baseline(s, fla, L, src, args, gpu_iters):
-    """Oracle FP64 CG on the host, bounded to about args.cpu_seconds."""
+    """Oracle FP64 CG on the host, bounded to about args.cpu_seconds: built
+    with -march=native on this host when gcc is there (else the shipped
+    x86-64-v3 build), OpenMP threads pinned one per core."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from _oracle import Oracle
+    import _oracle
     import qopqdp_amd as q
-    orc = Oracle()
+    cores = os.cpu_count()
+    try:
+        cores = len(os.sched_getaffinity(0))  # before OpenMP pins this thread
+    except AttributeError:
+        pass
+    march = "x86-64-v3"
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "liboracle_native.so"],
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    if r.returncode == 0 and _oracle.Oracle._lib is None:
+        _oracle.ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle_native.so")
+        march = "native"
+    # read by the OpenMP runtime when the oracle library first loads
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    orc = _oracle.Oracle()
     fat, lng = s.extract_links(fla)
     h = orc.links(L, fat, lng)
     del fat, lng
@@ -686,11 +728,6 @@ def cpu_baseline(s, fla, L, src, args, gpu_iters):
         n *= 2
     orc.free_links(h)
     solve_s = per_it * gpu_iters
-    cores = os.cpu_count()
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        pass
     model = ""
     try:
         for line in open("/proc/cpuinfo"):
@@ -706,8 +743,8 @@ def cpu_baseline(s, fla, L, src, args, gpu_iters):
         "cores": orc.threads(),
         "kind": "port",
         "sample": f"{n} FP64 CG iterations of the same solve on {orc.threads()} OpenMP threads "
-                  f"({cores} cores available, {model}); solves/s extrapolated to the GPU FP64 "
-                  f"solve's {int(gpu_iters)} iterations",
+                  f"pinned one per core ({cores} cores available, {model}, -march={march}); "
+                  f"solves/s extrapolated to the GPU FP64 solve's {int(gpu_iters)} iterations",
         "ms_per_iteration": round(per_it * 1e3, 2),
     }
 

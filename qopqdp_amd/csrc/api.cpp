@@ -258,8 +258,9 @@ QOP_status_t QOP_asqtad_invert_set_opts(QOP_opt_t opts[], int nopts) {
   get("qmx_delta", delta);
   get("qmx_chunk", chunk);
   get("qmx_graph", graph);
-  int msp = o.ms_prec == P16 ? 16 : 32;
+  int msp = o.ms_prec == P16 ? 16 : 32, msl = o.ms_link16 ? 16 : 32;
   get("qmx_ms_prec", msp);
+  get("qmx_ms_links", msl);
   double join = o.ru_join;
   get("qmx_ru_join", join);
   if (!(join >= 0 && join < 1)) return QOP_FAIL;
@@ -267,6 +268,7 @@ QOP_status_t QOP_asqtad_invert_set_opts(QOP_opt_t opts[], int nopts) {
   // the restart keeps 2 nev + 1 Lanczos vectors of m, and the space holds m
   if (cg == 1 && (nev < 1 || m < 2 * nev + 2 || numax < m)) return QOP_FAIL;
   if (msp != 16 && msp != 32) return QOP_FAIL;
+  if (msl != 16 && msl != 32) return QOP_FAIL;
   if (solver < 0 || solver > 1) return QOP_FAIL;
   if (prec != 64 && prec != 32 && prec != 16) return QOP_FAIL;
   if (!(delta > 0 && delta < 1) || chunk < 1 || chunk > 1024) return QOP_FAIL;
@@ -278,6 +280,7 @@ QOP_status_t QOP_asqtad_invert_set_opts(QOP_opt_t opts[], int nopts) {
   o.chunk = chunk;
   o.use_graph = graph;
   o.ms_prec = msp == 16 ? P16 : P32;
+  o.ms_link16 = msl == 16;
   o.ru_join = join;
   return QOP_SUCCESS;
 }

@@ -32,12 +32,13 @@ static bool shape_valid(const Lattice *lat, int prec, int nl, int nrhs, const Ho
   if (s.nb != 1 && s.nb != 2 && s.nb != 4 && s.nb != 8 && s.nb != 16) return false;
   if (s.nb == 16 && (prec == P64 || nl != 16)) return false;
   if (s.nb > 1 && s.nb > nrhs) return false;
-  if (s.block != 64 && s.block != 128 && s.block != 256) return false;
+  if (s.block != 64 && s.block != 128 && s.block != 256 && !(s.block == 512 && s.stage)) return false;
   if (s.tile && (s.tile % s.block != 0 || slab % s.tile != 0 || s.tile >= slab)) return false;
   if (s.stage) {
-    if (s.stage != 1 || prec == P64 || nl != 16 || s.nb > 8 || s.nt) return false;
+    if ((s.stage != 1 && s.stage != 2) || prec == P64 || nl != 16 || s.nb > 8 || s.nt) return false;
     const size_t el = prec == P16 ? 16 : 24;  // stored bytes per site and RHS
     if ((size_t)s.nb * s.block * el > 65536) return false;
+    if (s.stage == 2 && (s.tile || slab % 64 || lat->L[0] % (s.block / 64))) return false;
   }
   return true;
 }
@@ -133,8 +134,9 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
           if (tl && (tl % blk != 0 || slab % tl != 0 || tl >= slab)) continue;
           time_shape(HopShape{nb, blk, rm, nt, ri, tl, 0});
         }
-  // stage 3: LDS staging of the block's source rows, for the best RHS per
-  // thread / order, every block size and x0 tile (plain link loads)
+  // stage 3: LDS staging, for the best RHS per thread / order: of the block's
+  // source rows at every block size and x0 tile, and of its x0 planes
+  // (plane-blocked order) at 2, 4 and 8 planes per block (plain link loads)
   if (prec != P64 && L.nl == 16) {
     const HopShape b2 = best;
     for (int sblk : blocks)
@@ -145,6 +147,16 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
         c.tile = tl;
         c.stage = 1;
         if (c.nb > 8) c.nb = 8;
+        if (shape_valid(lat, prec, L.nl, nrhs, c)) time_shape(c);
+      }
+    for (int sblk : {128, 256, 512})
+      for (int snb : {b2.nb, b2.nb / 2}) {
+        HopShape c = b2;
+        c.block = sblk;
+        c.nt = 0;
+        c.tile = 0;
+        c.stage = 2;
+        c.nb = snb > 8 ? 8 : snb < 1 ? 1 : snb;
         if (shape_valid(lat, prec, L.nl, nrhs, c)) time_shape(c);
       }
   }

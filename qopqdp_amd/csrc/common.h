@@ -26,8 +26,10 @@
 
 namespace qop {
 
-enum Prec { P64 = 0, P32 = 1, P16 = 2 };
-inline size_t prec_cbytes(int p) { return p == P64 ? 16 : p == P32 ? 8 : 4; }
+// P32H is a hop precision only: FP32 vectors with the FP16 links (+ per-link
+// scale) of the P16 link set.
+enum Prec { P64 = 0, P32 = 1, P16 = 2, P32H = 3 };
+inline size_t prec_cbytes(int p) { return p == P64 ? 16 : p == P16 ? 4 : 8; }
 
 [[noreturn]] void fatal(const char *file, int line, const char *msg);
 #define QOP_CHECK_HIP(x)                                                   \
@@ -72,10 +74,7 @@ struct DevBuf {
 
 // A stream-captured hipGraph and its executable instance, owned together:
 // the captured graph lives as long as the instance made from it and both are
-// destroyed in one place (instance first).  Profilers that trace graph
-// launches look the instance's nodes up in the captured graph at every
-// launch, so a captured graph destroyed right after instantiation leaves
-// them reading freed memory.
+// destroyed in one place (instance first), also while an exception unwinds.
 struct CapturedGraph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;

@@ -198,6 +198,20 @@ __device__ __forceinline__ int logical_block(int remap) {
   return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
 
+// Output site of this thread for site block sb in the plane-blocked order
+// (LDS staging mode 2): a block covers the same 64 positions of the x0 slab
+// in blockDim/64 consecutive x0 planes, one plane per wave, so a wave's 64
+// sites are still one 64-site reduction chunk.  Blocks past the last site
+// return Vh (no site).
+__device__ __forceinline__ int site_of_plane_block(const Geom &g, int sb) {
+  const int D = blockDim.x >> 6;
+  const int slab = g.Vh / g.L[0];
+  const int ntile = slab >> 6;
+  if (sb >= (g.L[0] / D) * ntile) return g.Vh;
+  const int pg = sb / ntile, t = sb - pg * ntile;
+  return (pg * D + (int)(threadIdx.x >> 6)) * slab + t * 64 + (int)(threadIdx.x & 63);
+}
+
 // Output site of this thread for site block sb.  Plain order: blocks cover
 // consecutive sites.  x0-tiled order (tile > 0 sites, dividing the Vh/L0
 // sites of an x0 slab): consecutive blocks sweep x0 for one tile of the slab
@@ -287,14 +301,19 @@ __device__ __forceinline__ void hop_epilogue(const HopArgs &a, int h, int Vh, in
 
 // ---------------------------------------------------------------------------
 // hop kernel
-// STG: LDS staging.  The block first loads the stored source words of the
-// source-parity sites [h0, h0 + blockDim) -- the same (x0, x1, x2) rows as its
-// output sites -- into LDS; a neighbour inside that range (all x3 hops, and
-// the x2 hops that stay inside the block's rows) is then read from LDS, the
-// rest from memory.  The block's sites are contiguous in both site orders
-// (an x0 tile is a whole number of blocks), from h0 = h - threadIdx.x.
-template <int P, int NL, int NB, int RED, bool HASX, bool NT, bool SGL, bool STG>
-__global__ void __launch_bounds__(256)
+// STG: LDS staging.  Each thread first loads the stored source words of the
+// source-parity site with its own index into LDS; a gathered neighbour that
+// is one of those sites is then read from LDS, the rest from memory.  Mode 1
+// (a.stage = 1): the block's sites are contiguous, [h0, h0 + blockDim) -- the
+// same (x0, x1, x2) rows as its outputs, so all x3 hops and the x2 hops that
+// stay inside the block's rows come from LDS (an x0 tile is a whole number of
+// blocks).  Mode 2: the plane-blocked order (site_of_plane_block), so the x0
+// hops that stay inside the block's planes and the in-plane hops inside the
+// wave's 64 positions come from LDS.  Staged blocks take up to 512 threads.
+// LP: precision of the stored links (P, or P16 for FP32 vectors with FP16
+// links and per-link scales).
+template <int P, int NL, int NB, int RED, bool HASX, bool NT, bool SGL, bool STG, int LP = P>
+__global__ void __launch_bounds__(STG ? 512 : 256)
 hop_kernel(const HopArgs a) {
   using R = typename PT<P>::R;
   const Geom g = a.g;
@@ -304,7 +323,8 @@ hop_kernel(const HopArgs a) {
   // blocks, so they run together on one XCD and share its L2 (links and the
   // neighbour window); otherwise the groups are the grid's y dimension.
   const int sb = a.rhs_inner ? lb / a.ngroups : lb;
-  const int h = site_of_block(g, sb, a.tile);
+  const bool planes = STG && a.stage == 2;
+  const int h = planes ? site_of_plane_block(g, sb) : site_of_block(g, sb, a.tile);
   const int b0 = (a.rhs_inner ? lb % a.ngroups : (int)blockIdx.y) * NB;
   if (a.status && *a.status != 0) return;
 
@@ -321,14 +341,16 @@ hop_kernel(const HopArgs a) {
   }
   using S = typename PT<P>::S;
   const int bs = blockDim.x;
-  const int h0 = h - (int)threadIdx.x;
+  const int h0 = h - (int)threadIdx.x;  // mode 1: first site of the block
+  const int wv = (int)(threadIdx.x >> 6), nwv = bs >> 6;  // mode 2: wave = plane
+  const int hrow = h - (int)(threadIdx.x & 63);         // mode 2: first site of the wave
   S *td = nullptr;     // [NB][3][bs]
   float *tn = nullptr;  // FP16 norms [NB][bs]
   if constexpr (STG) {
     extern __shared__ __align__(16) unsigned char stg_mem[];
     td = (S *)stg_mem;
     tn = (float *)(td + NB * 3 * bs);
-    const int hs = h0 + (int)threadIdx.x;
+    const int hs = h;
     if (any && hs < Vh) {
 #pragma unroll
       for (int b = 0; b < NB; b++) {
@@ -387,12 +409,26 @@ hop_kernel(const HopArgs a) {
         } else {
           const long lbase = (long)k * 9 * Vh + h;
           float scl = 1.0f;
-          if constexpr (P == P16) {
+          if constexpr (LP == P16) {
             if constexpr (NT) scl = __builtin_nontemporal_load(a.lscale + (long)k * Vh + h);
             else scl = a.lscale[(long)k * Vh + h];
           }
 #pragma unroll
-          for (int e = 0; e < 9; e++) ld_c<P, NT>(a.links, lbase + (long)e * Vh, scl, w[2 * e], w[2 * e + 1]);
+          for (int e = 0; e < 9; e++) ld_c<LP, NT>(a.links, lbase + (long)e * Vh, scl, w[2 * e], w[2 * e + 1]);
+        }
+        int slot = -1;  // LDS position of the neighbour, or -1
+        if constexpr (STG) {
+          if (planes) {
+            if (mu == 0) {
+              if ((unsigned)(wv + d) < (unsigned)nwv) slot = (int)threadIdx.x + 64 * d;
+            } else {
+              const unsigned pos = (unsigned)(nb - hrow);
+              if (pos < 64u) slot = (wv << 6) + (int)pos;
+            }
+          } else {
+            const unsigned o = (unsigned)(nb - h0);
+            if (o < (unsigned)bs) slot = (int)o;
+          }
         }
 #pragma unroll
         for (int b = 0; b < NB; b++) {
@@ -400,13 +436,8 @@ hop_kernel(const HopArgs a) {
           const char *sd = (const char *)a.src.data + (long)(b0 + b) * a.sstride * sizeof(typename PT<P>::S);
           const float *sn = a.src.norm ? a.src.norm + (long)(b0 + b) * a.nstride : nullptr;
           R v[6];
-          if constexpr (STG) {
-            const unsigned o = (unsigned)(nb - h0);
-            if (o < (unsigned)bs) tload<P>(td + b * 3 * bs, tn + b * bs, (int)o, bs, v);
-            else vload<P>(sd, sn, nb, Vh, v);
-          } else {
-            vload<P>(sd, sn, nb, Vh, v);
-          }
+          if (STG && slot >= 0) tload<P>(td + b * 3 * bs, tn + b * bs, slot, bs, v);
+          else vload<P>(sd, sn, nb, Vh, v);
 #pragma unroll
           for (int r = 0; r < 3; r++) {
             R re = acc[b][2 * r], im = acc[b][2 * r + 1];
@@ -446,12 +477,12 @@ static size_t stage_bytes(int prec, int nb, int block) {
   return prec == P16 ? (size_t)nb * block * (3 * 4 + 4) : (size_t)nb * block * 3 * el;
 }
 
-template <int P, int NL, int NB, bool NT, bool SGL = false, bool STG = false>
+template <int P, int NL, int NB, bool NT, bool SGL = false, bool STG = false, int LP = P>
 static void hop_dispatch_red(const HopArgs &a, dim3 grid, dim3 block, int red,
                              bool hasx, hipStream_t st) {
   const size_t lds = STG ? stage_bytes(P, NB, (int)block.x) : 0;
 #define QOP_LAUNCH(RD, HX) \
-  hipLaunchKernelGGL((hop_kernel<P, NL, NB, RD, HX, NT, SGL, STG>), grid, block, lds, st, a)
+  hipLaunchKernelGGL((hop_kernel<P, NL, NB, RD, HX, NT, SGL, STG, LP>), grid, block, lds, st, a)
   if (red == RED_NONE) { if (hasx) QOP_LAUNCH(RED_NONE, true); else QOP_LAUNCH(RED_NONE, false); }
   else if (red == RED_OUT2) { if (hasx) QOP_LAUNCH(RED_OUT2, true); else QOP_LAUNCH(RED_OUT2, false); }
   else if (red == RED_CG) { QOP_LAUNCH(RED_CG, true); }
@@ -459,13 +490,13 @@ static void hop_dispatch_red(const HopArgs &a, dim3 grid, dim3 block, int red,
 #undef QOP_LAUNCH
 }
 
-template <int P, int NL>
+template <int P, int NL, int LP = P>
 static void hop_dispatch_nb(const HopArgs &a, int nb, dim3 grid, dim3 block,
                             int red, bool hasx, hipStream_t st) {
   if (a.single) {
-    if constexpr (P != P16) {
+    if constexpr (P != P16 && LP == P) {
       QOP_REQUIRE(nb == 1 && !a.nt, "hop: single-stored links take one RHS per thread");
-      hop_dispatch_red<P, NL, 1, false, true>(a, grid, block, red, hasx, st);
+      hop_dispatch_red<P, NL, 1, false, true, false, LP>(a, grid, block, red, hasx, st);
     } else {
       fatal(__FILE__, __LINE__, "hop: single-stored links exist in FP64 and FP32 only");
     }
@@ -474,10 +505,10 @@ static void hop_dispatch_nb(const HopArgs &a, int nb, dim3 grid, dim3 block,
   if (a.stage) {
     if constexpr (P != P64 && NL == 16) {
       switch (nb) {
-        case 1: hop_dispatch_red<P, NL, 1, false, false, true>(a, grid, block, red, hasx, st); break;
-        case 2: hop_dispatch_red<P, NL, 2, false, false, true>(a, grid, block, red, hasx, st); break;
-        case 4: hop_dispatch_red<P, NL, 4, false, false, true>(a, grid, block, red, hasx, st); break;
-        default: hop_dispatch_red<P, NL, 8, false, false, true>(a, grid, block, red, hasx, st); break;
+        case 1: hop_dispatch_red<P, NL, 1, false, false, true, LP>(a, grid, block, red, hasx, st); break;
+        case 2: hop_dispatch_red<P, NL, 2, false, false, true, LP>(a, grid, block, red, hasx, st); break;
+        case 4: hop_dispatch_red<P, NL, 4, false, false, true, LP>(a, grid, block, red, hasx, st); break;
+        default: hop_dispatch_red<P, NL, 8, false, false, true, LP>(a, grid, block, red, hasx, st); break;
       }
     } else {
       fatal(__FILE__, __LINE__, "hop: LDS staging exists for FP32/FP16 with Naik links");
@@ -486,23 +517,23 @@ static void hop_dispatch_nb(const HopArgs &a, int nb, dim3 grid, dim3 block,
   }
   if (a.nt) {
     switch (nb) {
-      case 1: hop_dispatch_red<P, NL, 1, true>(a, grid, block, red, hasx, st); break;
-      case 2: hop_dispatch_red<P, NL, 2, true>(a, grid, block, red, hasx, st); break;
-      case 4: hop_dispatch_red<P, NL, 4, true>(a, grid, block, red, hasx, st); break;
-      case 8: hop_dispatch_red<P, NL, 8, true>(a, grid, block, red, hasx, st); break;
+      case 1: hop_dispatch_red<P, NL, 1, true, false, false, LP>(a, grid, block, red, hasx, st); break;
+      case 2: hop_dispatch_red<P, NL, 2, true, false, false, LP>(a, grid, block, red, hasx, st); break;
+      case 4: hop_dispatch_red<P, NL, 4, true, false, false, LP>(a, grid, block, red, hasx, st); break;
+      case 8: hop_dispatch_red<P, NL, 8, true, false, false, LP>(a, grid, block, red, hasx, st); break;
       default:
-        if constexpr (P != P64 && NL == 16) hop_dispatch_red<P, NL, 16, true>(a, grid, block, red, hasx, st);
+        if constexpr (P != P64 && NL == 16) hop_dispatch_red<P, NL, 16, true, false, false, LP>(a, grid, block, red, hasx, st);
         else fatal(__FILE__, __LINE__, "hop: 16 RHS per thread needs FP32/FP16 and Naik links");
         break;
     }
   } else {
     switch (nb) {
-      case 1: hop_dispatch_red<P, NL, 1, false>(a, grid, block, red, hasx, st); break;
-      case 2: hop_dispatch_red<P, NL, 2, false>(a, grid, block, red, hasx, st); break;
-      case 4: hop_dispatch_red<P, NL, 4, false>(a, grid, block, red, hasx, st); break;
-      case 8: hop_dispatch_red<P, NL, 8, false>(a, grid, block, red, hasx, st); break;
+      case 1: hop_dispatch_red<P, NL, 1, false, false, false, LP>(a, grid, block, red, hasx, st); break;
+      case 2: hop_dispatch_red<P, NL, 2, false, false, false, LP>(a, grid, block, red, hasx, st); break;
+      case 4: hop_dispatch_red<P, NL, 4, false, false, false, LP>(a, grid, block, red, hasx, st); break;
+      case 8: hop_dispatch_red<P, NL, 8, false, false, false, LP>(a, grid, block, red, hasx, st); break;
       default:
-        if constexpr (P != P64 && NL == 16) hop_dispatch_red<P, NL, 16, false>(a, grid, block, red, hasx, st);
+        if constexpr (P != P64 && NL == 16) hop_dispatch_red<P, NL, 16, false, false, false, LP>(a, grid, block, red, hasx, st);
         else fatal(__FILE__, __LINE__, "hop: 16 RHS per thread needs FP32/FP16 and Naik links");
         break;
     }
@@ -512,7 +543,8 @@ static void hop_dispatch_nb(const HopArgs &a, int nb, dim3 grid, dim3 block,
 void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
                      bool hasx, hipStream_t st) {
   QOP_REQUIRE(nb == 1 || nb == 2 || nb == 4 || nb == 8 || nb == 16, "hop: rhs per thread must be 1,2,4,8,16");
-  QOP_REQUIRE(block % 64 == 0 && block >= 64 && block <= 256, "hop: block must be 64..256, multiple of 64");
+  QOP_REQUIRE(block % 64 == 0 && block >= 64 && block <= (a.stage ? 512 : 256),
+              "hop: block must be a multiple of 64, 64..256 (512 with LDS staging)");
   QOP_REQUIRE(red == RED_NONE || hasx || red == RED_OUT2, "hop: RED_X2 / RED_CG need x");
   QOP_REQUIRE(red != RED_CG || (a.cg_alpha && a.xs.data && prec != P64), "hop: RED_CG needs alpha and xs");
   QOP_REQUIRE(a.n >= 1, "hop: n >= 1");
@@ -521,6 +553,13 @@ void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
                 "hop: LDS staging needs FP32/FP16 Naik links, at most 8 RHS per thread and plain "
                 "link loads");
     QOP_REQUIRE(stage_bytes(prec, nb, block) <= 65536, "hop: LDS staging tile above 64 KiB");
+    QOP_REQUIRE(a.stage == 1 || a.stage == 2, "hop: LDS staging mode must be 1 (rows) or 2 (planes)");
+    if (a.stage == 2) {
+      const int slab = a.g.Vh / a.g.L[0];
+      QOP_REQUIRE(!a.tile && slab % 64 == 0 && a.g.L[0] % (block / 64) == 0,
+                  "hop: plane-blocked staging needs the plain order, 64 dividing the x0 slab and "
+                  "block/64 dividing L0");
+    }
   }
   if (a.tile) {
     const int slab = a.g.Vh / a.g.L[0];
@@ -540,6 +579,10 @@ void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
   if (prec == P64) {
     if (nl == 16) hop_dispatch_nb<P64, 16>(a, nb, grid, blk, red, hasx, st);
     else hop_dispatch_nb<P64, 8>(a, nb, grid, blk, red, hasx, st);
+  } else if (prec == P32H) {
+    QOP_REQUIRE(!a.single, "hop: FP32 vectors with FP16 links take double-stored links");
+    if (nl == 16) hop_dispatch_nb<P32, 16, P16>(a, nb, grid, blk, red, hasx, st);
+    else hop_dispatch_nb<P32, 8, P16>(a, nb, grid, blk, red, hasx, st);
   } else if (prec == P32) {
     if (nl == 16) hop_dispatch_nb<P32, 16>(a, nb, grid, blk, red, hasx, st);
     else hop_dispatch_nb<P32, 8>(a, nb, grid, blk, red, hasx, st);

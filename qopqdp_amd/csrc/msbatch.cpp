@@ -13,7 +13,8 @@ namespace qop {
 
 MultiShiftBatch::MultiShiftBatch(Links &L, int eo, int capacity, int sloppy_prec,
                                  const double *masses, int ns)
-    : L_(L), lat_(L.lat), eo_(eo), cap_(capacity), sp_(sloppy_prec), ns_(ns) {
+    : L_(L), lat_(L.lat), eo_(eo), cap_(capacity), sp_(sloppy_prec), ns_(ns),
+      hp_(sloppy_prec == P32 && options().ms_link16 ? P32H : sloppy_prec) {
   try {
     QOP_REQUIRE(eo == 0 || eo == 1, "multi-shift: subset must be even or odd");
     QOP_REQUIRE(ns >= 1 && ns <= MS_MAXSHIFT, "multi-shift: 1..16 masses");
@@ -54,9 +55,9 @@ MultiShiftBatch::MultiShiftBatch(Links &L, int eo, int capacity, int sloppy_prec
     std::vector<double> hc(2 * cap_);
     for (int b = 0; b < cap_; b++) { hc[2 * b] = -1.0; hc[2 * b + 1] = m0 * m0; }
     QOP_CHECK_HIP(hipMemcpy(coef_, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice));
-    L_.get(sp_);  // convert the links before any graph capture
+    L_.get(hp_);  // convert the links before any graph capture
     L_.get(P64);
-    shape_ = tune_hop(L_, sp_, cap_, ps_, ts_, 1 - eo_);
+    shape_ = tune_hop(L_, hp_, cap_, ps_, ts_, 1 - eo_);
     ru_shape_ = tune_hop(L_, P64, cap_, x64_, t64_, 1 - eo_);
     sync();
   } catch (const DeviceOutOfMemory &) {
@@ -155,7 +156,7 @@ void MultiShiftBatch::load(int slot, const DevVec &src, double ms_tol, int max_i
 
 void MultiShiftBatch::enqueue_iteration() {
   hipStream_t st = stream();
-  LinkSet &ls = L_.get(sp_);
+  LinkSet &ls = L_.get(hp_);
   const int opp = 1 - eo_;
   const long Vh = lat_->Vh;
   HopArgs a;
@@ -171,7 +172,7 @@ void MultiShiftBatch::enqueue_iteration() {
   a.active = active_;
   a.partial = partA_;
   shape_.apply(a);
-  launch_hop_args(a, sp_, ls.nl, shape_.nb, shape_.block, RED_OUT2, false, st);
+  launch_hop_args(a, hp_, ls.nl, shape_.nb, shape_.block, RED_OUT2, false, st);
   ms_launch_scalarA(slot_, active_, partA_, partB_, nch_, cap_, st);
   // r -= beta_0 (m_0^2 p - D t), x_base += beta_0 p, |r|^2
   HopArgs b = a;
@@ -186,7 +187,7 @@ void MultiShiftBatch::enqueue_iteration() {
   b.cg_alpha = (const double *)((const char *)slot_ + offsetof(MsSlot, beta0));
   b.cg_stride = (int)(sizeof(MsSlot) / sizeof(double));
   b.partial = partC_;
-  launch_hop_args(b, sp_, ls.nl, shape_.nb, shape_.block, RED_CG, true, st);
+  launch_hop_args(b, hp_, ls.nl, shape_.nb, shape_.block, RED_CG, true, st);
   ms_launch_scalarB(slot_, active_, partC_, nch_, cap_, ru_, st);
   ms_launch_update(slot_, active_, sp_, (int)Vh, rs_.slot(0), ps_.slot(0), pm_.slot(0), xm_.slot(0),
                    3 * Vh, 3 * Vh * cap_, cap_, ns_, partB_, st);

@@ -82,6 +82,7 @@ class MultiShiftBatch {
   Links &L_;
   Lattice *lat_;
   int eo_, cap_, sp_, ns_, imin_;
+  int hp_;  // precision of the sloppy hops: sp_, or P32H (FP32 vectors, FP16 links)
   int nch_;
   std::vector<double> masses_;
   DevBatch b64_, rs_, ps_, ts_;

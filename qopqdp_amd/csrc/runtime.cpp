@@ -113,6 +113,7 @@ void linkset_free(LinkSet &l) {
 
 LinkSet &Links::get(int prec) {
   if (prec == P64) return l64;
+  if (prec == P32H) prec = P16;  // FP32 vectors read the FP16 link set
   LinkSet &d = prec == P32 ? l32 : l16;
   if (!d.data) {
     convert_links(lat, l64, d, prec, stream());

@@ -54,6 +54,7 @@ struct Options {
   double ru_join = 0.0;  // batched CG: a slot whose residual dropped by this factor since its
                          // last reliable update joins another slot's update (0: never)
   int ms_prec = P32;     // vector precision of the batched multi-shift stage
+  int ms_link16 = 0;     // FP32 multi-shift stage: hop with the FP16 links (+ scales)
   int verbose = 0;
 };
 Options &options();

@@ -1,6 +1,7 @@
 """The batched solver's result does not depend on the multi-RHS hop launch
 shape (RHS per thread, block size, XCD remap, non-temporal loads and stores,
-block order, x0-tiled site order, LDS staging of the block's source rows):
+block order, x0-tiled site order, LDS staging of the block's source rows or
+of its x0 planes in the plane-blocked order):
 each forced shape gives bit-identical solutions and iteration counts, all at FP64 true residual 1e-10 (oracle operator).  The
 second lattice has x3 rows of 5 sites, so staged blocks end inside a row and
 gathers switch between the LDS tile and memory within a wavefront."""
@@ -28,6 +29,11 @@ SHAPES = [
     (8, 256, 1, 0, 1, 0, 1, 0),
     (1, 64, 1, 0, 1, 0, 1, 0),
     (4, 128, 1, 0, 1, 512, 1, 0),
+    (4, 256, 1, 0, 1, 0, 2, 0),
+    (2, 128, 0, 0, 0, 0, 2, 0),
+    (4, 512, 1, 0, 1, 0, 2, 0),
+    (8, 256, 1, 0, 1, 0, 2, 0),
+    (1, 512, 0, 0, 1, 0, 2, 0),
     (2, 256, 0, 0, 0, 1024, 1, 0),
 ]
 
