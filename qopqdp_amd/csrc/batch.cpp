@@ -212,6 +212,7 @@ BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double
     lcoef_ = coef_ + 2 * cap_;
     dev_alloc(&sums_, 2 * (size_t)cap_ * sizeof(double));
     QOP_CHECK_HIP(hipHostMalloc(&sums_h_, 2 * (size_t)cap_ * sizeof(double), hipHostMallocDefault));
+    QOP_CHECK_HIP(hipHostMalloc(&hc_h_, 2 * (size_t)cap_ * sizeof(double), hipHostMallocDefault));
     memset(rhs_h_, 0, cap_ * sizeof(BsRhs));
     memset(active_h_, 0, 2 * cap_ * sizeof(int));
     glob_h_->status = 0;
@@ -259,6 +260,7 @@ void BatchSolver::release() {
   (void)hipFree(coef_);
   (void)hipFree(sums_);
   (void)hipHostFree(sums_h_);
+  (void)hipHostFree(hc_h_);
 }
 
 int BatchSolver::free_slot() const {
@@ -329,8 +331,16 @@ void BatchSolver::apply_A64(const int *mask) {
 
 void BatchSolver::load(int slot, const DevVec &src, const DevVec *x0, double mass, double tol,
                        int max_iter) {
-  QOP_REQUIRE(slot >= 0 && slot < cap_ && !loaded_[slot], "batch solver: slot not free");
-  QOP_REQUIRE(src.lat == lat_, "batch solver: lattice mismatch");
+  const LoadReq q{slot, &src, x0, mass, tol, max_iter};
+  load_many(&q, 1);
+}
+
+// Several slots in one pass: the FP64 operator for the initial residuals
+// b - A x0 is applied once, masked to the slots that have an initial guess
+// (each slot's arithmetic is its own, so the results do not depend on which
+// slots load together).
+void BatchSolver::load_many(const LoadReq *req, int nreq) {
+  if (nreq <= 0) return;
   hipStream_t st = st_;
   const long Vh = lat_->Vh;
   auto copy_half = [&](VecView dst, int pdst, VecView s, int ps_) {
@@ -343,16 +353,27 @@ void BatchSolver::load(int slot, const DevVec &src, const DevVec *x0, double mas
     int pin[1] = {ps_};
     launch_lincomb(a, pdst, pin, 1, false, st);
   };
-  copy_half(b64_.slot(slot), P64, src.half(eo_), P64);
-  if (x0) copy_half(x64_.slot(slot), P64, x0->half(eo_), P64);
-  else QOP_CHECK_HIP(hipMemsetAsync(x64_.slot(slot).data, 0, 3 * Vh * sizeof(double2), st));
-  // hop coefficients of this slot: A = m^2 - D D
-  double hc[2] = {-1.0, mass * mass};
-  QOP_CHECK_HIP(hipMemcpyAsync(coef_ + 2 * slot, hc, sizeof hc, hipMemcpyHostToDevice, st));
-  // r = b - A x (or b)
-  for (int i = 0; i < cap_; i++) sel_h_[i] = i == slot;
+  for (int i = 0; i < cap_; i++) sel_h_[i] = 0;
+  bool any_x0 = false;
+  for (int k = 0; k < nreq; k++) {
+    const LoadReq &q = req[k];
+    QOP_REQUIRE(q.slot >= 0 && q.slot < cap_ && !loaded_[q.slot] && !sel_h_[q.slot],
+                "batch solver: slot not free");
+    QOP_REQUIRE(q.src->lat == lat_, "batch solver: lattice mismatch");
+    copy_half(b64_.slot(q.slot), P64, q.src->half(eo_), P64);
+    if (q.x0) copy_half(x64_.slot(q.slot), P64, q.x0->half(eo_), P64);
+    else QOP_CHECK_HIP(hipMemsetAsync(x64_.slot(q.slot).data, 0, 3 * Vh * sizeof(double2), st));
+    // hop coefficients of this slot: A = m^2 - D D
+    hc_h_[2 * k] = -1.0;
+    hc_h_[2 * k + 1] = q.mass * q.mass;
+    QOP_CHECK_HIP(hipMemcpyAsync(coef_ + 2 * q.slot, hc_h_ + 2 * k, 2 * sizeof(double),
+                                 hipMemcpyHostToDevice, st));
+    sel_h_[q.slot] = 1;
+    any_x0 |= q.x0 != nullptr;
+  }
   QOP_CHECK_HIP(hipMemcpyAsync(sel_, sel_h_, cap_ * sizeof(int), hipMemcpyHostToDevice, st));
-  if (x0) {
+  // r = b - A x (x = 0 for the slots without a guess: A x = 0 there)
+  if (any_x0) {
     apply_A64(sel_);
     const DevBatch *in[2] = {&b64_, &ax64_};
     double cr[2] = {1.0, -1.0};
@@ -383,7 +404,7 @@ void BatchSolver::load(int slot, const DevVec &src, const DevVec *x0, double mas
     batch_lincomb(lat_, cap_, rs_, 1, in, cr, nullptr, nullptr, sel_, nullptr, st);
     const DevBatch *in2[1] = {&rs_};
     batch_lincomb(lat_, cap_, ps_, 1, in2, cr, nullptr, nullptr, sel_, nullptr, st);
-    // |ps|^2 partials of this slot, consumed by the first iteration's p.Ap
+    // |ps|^2 partials of the loaded slots, consumed by the first iteration's p.Ap
     DotArgs d;
     d.Vh = (int)Vh;
     d.n = cap_;
@@ -393,27 +414,32 @@ void BatchSolver::load(int slot, const DevVec &src, const DevVec *x0, double mas
     d.partial = partB_;
     d.active = sel_;
     launch_dot(d, sp_, sp_, 0, st);
-    QOP_CHECK_HIP(hipMemsetAsync(xs_.slot(slot).data, 0, 3 * Vh * sizeof(float2), st));
+    for (int k = 0; k < nreq; k++)
+      QOP_CHECK_HIP(hipMemsetAsync(xs_.slot(req[k].slot).data, 0, 3 * Vh * sizeof(float2), st));
   }
   QOP_CHECK_HIP(hipMemcpyAsync(sums_h_, sums_, 2 * cap_ * sizeof(double), hipMemcpyDeviceToHost, st));
   QOP_CHECK_HIP(hipStreamSynchronize(st_));
-  BsRhs &R = rhs_h_[slot];
-  memset(&R, 0, sizeof R);
-  R.m2 = mass * mass;
-  R.r2 = sums_h_[slot];
-  R.r2_rel = R.r2;
-  R.b2 = sums_h_[cap_ + slot];
-  R.stop2 = tol * tol * R.b2;
-  R.max_iter = max_iter;
-  R.delta2 = delta_ * delta_;
-  R.join2 = join_ * join_;
-  mass_[slot] = mass;
-  loaded_[slot] = 1;
-  const bool done = R.r2 <= R.stop2;
-  if (done) R.reason = 1;
-  active_h_[slot] = done ? 0 : 1;
-  QOP_CHECK_HIP(hipMemcpyAsync(rhs_ + slot, &R, sizeof R, hipMemcpyHostToDevice, st));
-  QOP_CHECK_HIP(hipMemcpyAsync(active_ + slot, active_h_ + slot, sizeof(int), hipMemcpyHostToDevice, st));
+  for (int k = 0; k < nreq; k++) {
+    const LoadReq &q = req[k];
+    const int slot = q.slot;
+    BsRhs &R = rhs_h_[slot];
+    memset(&R, 0, sizeof R);
+    R.m2 = q.mass * q.mass;
+    R.r2 = sums_h_[slot];
+    R.r2_rel = R.r2;
+    R.b2 = sums_h_[cap_ + slot];
+    R.stop2 = q.tol * q.tol * R.b2;
+    R.max_iter = q.max_iter;
+    R.delta2 = delta_ * delta_;
+    R.join2 = join_ * join_;
+    mass_[slot] = q.mass;
+    loaded_[slot] = 1;
+    const bool done = R.r2 <= R.stop2;
+    if (done) R.reason = 1;
+    active_h_[slot] = done ? 0 : 1;
+    QOP_CHECK_HIP(hipMemcpyAsync(rhs_ + slot, &R, sizeof R, hipMemcpyHostToDevice, st));
+    QOP_CHECK_HIP(hipMemcpyAsync(active_ + slot, active_h_ + slot, sizeof(int), hipMemcpyHostToDevice, st));
+  }
   QOP_CHECK_HIP(hipStreamSynchronize(st_));
 }
 
@@ -607,12 +633,16 @@ void invert_multi_dispatch(Links &L, InvArg *ia, ResArg **ra, const double *mass
   }
   int next = 0, finished = 0;
   long iters = 0;
+  std::vector<BatchSolver::LoadReq> reqs;
   while (finished < nsolve) {
-    int s;
-    while (next < nsolve && (s = bs.free_slot()) >= 0) {
-      bs.load(s, *in[src_of[next]], nullptr, masses[next], std::sqrt(ra[next]->rsqmin), ia->max_iter);
+    reqs.clear();
+    for (int s = 0; s < bs.capacity() && next < nsolve; s++) {
+      if (!bs.slot_free(s)) continue;
+      reqs.push_back({s, in[src_of[next]], nullptr, masses[next], std::sqrt(ra[next]->rsqmin),
+                      ia->max_iter});
       slot_of[s] = next++;
     }
+    bs.load_many(reqs.data(), (int)reqs.size());
     std::vector<int> done;
     bs.step(done);
     for (int d : done) {

@@ -100,11 +100,22 @@ class BatchSolver {
 
   int capacity() const { return cap_; }
   int free_slot() const;
+  bool slot_free(int slot) const { return !loaded_[slot]; }
   int nactive() const;
   // Load a system into a free slot.  src is a full FP64 vector (its `eo`
   // half is used); x0 (optional) is an initial guess for A^-1 src.
   void load(int slot, const DevVec &src, const DevVec *x0, double mass, double tol,
             int max_iter);
+  // several slots at once (one masked FP64 operator application for all
+  // initial residuals); the same per-slot state as load()
+  struct LoadReq {
+    int slot;
+    const DevVec *src;
+    const DevVec *x0;
+    double mass, tol;
+    int max_iter;
+  };
+  void load_many(const LoadReq *req, int nreq);
   // Run one chunk of iterations and service reliable updates.  Slots that
   // finished are appended to `done`.
   void step(std::vector<int> &done);
@@ -152,6 +163,7 @@ class BatchSolver {
   double *coef_ = nullptr;      // [cap][2] hop coefficients (-1, m2)
   double *lcoef_ = nullptr;     // [cap][3][2] lincomb coefficients
   double *sums_ = nullptr, *sums_h_ = nullptr;
+  double *hc_h_ = nullptr;      // pinned staging of the loaded slots' hop coefficients
   std::vector<int> loaded_;
   std::vector<double> mass_;
   CapturedGraph graph_;

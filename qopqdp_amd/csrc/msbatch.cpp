@@ -232,7 +232,8 @@ void MultiShiftBatch::enqueue_reliable_update() {
 void MultiShiftBatch::run() {
   hipStream_t st = stream();
   const int chunk = options().chunk > 0 ? options().chunk : 16;
-  if (!graph_) {
+  const bool use_graph = options().use_graph != 0;
+  if (use_graph && !graph_) {
     graph_.begin(st);
     for (int c = 0; c < chunk; c++) enqueue_iteration();
     graph_.end(st);
@@ -241,7 +242,8 @@ void MultiShiftBatch::run() {
     bool any = false;
     for (int i = 0; i < cap_; i++) any |= active_h_[i] != 0;
     if (!any) break;
-    graph_.launch(st);
+    if (use_graph) graph_.launch(st);
+    else for (int c = 0; c < chunk; c++) enqueue_iteration();
     QOP_CHECK_HIP(hipMemcpyAsync(active_h_, active_, cap_ * sizeof(int), hipMemcpyDeviceToHost, st));
     sync();
   }
@@ -338,13 +340,17 @@ int QOPX_D3_asqtad_invert_multi_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqta
     BatchSolver bs(L, eo, cap, sp, options().delta);
     std::vector<int> slot_of(cap, -1);
     int next = 0, finished = 0;
+    std::vector<BatchSolver::LoadReq> reqs;
     while (finished < n) {
-      int s;
-      while (next < n && (s = bs.free_slot()) >= 0) {
+      reqs.clear();
+      for (int s = 0; s < cap && next < n; s++) {
+        if (!bs.slot_free(s)) continue;
         const int src = next / nmass, j = next % nmass;
-        bs.load(s, in[src]->v, &guess[next], masses[j], std::sqrt(res[next]->rsqmin), inv_arg->max_iter);
+        reqs.push_back({s, &in[src]->v, &guess[next], masses[j], std::sqrt(res[next]->rsqmin),
+                        inv_arg->max_iter});
         slot_of[s] = next++;
       }
+      bs.load_many(reqs.data(), (int)reqs.size());
       std::vector<int> done;
       bs.step(done);
       for (int d : done) {

@@ -240,12 +240,16 @@ int QOPX_D3_asqtad_invert_batch(QOP_info_t *info, QOP_D3_FermionLinksAsqtad *fla
     BatchSolver bs(L, inv_arg->evenodd, cap, sp, options().delta);
     std::vector<int> slot_of(cap, -1);
     int next = 0, finished = 0;
+    std::vector<BatchSolver::LoadReq> reqs;
     while (finished < n) {
-      int s;
-      while (next < n && (s = bs.free_slot()) >= 0) {
-        bs.load(s, in[next]->v, nullptr, masses[next], std::sqrt(res[next]->rsqmin), inv_arg->max_iter);
+      reqs.clear();
+      for (int s = 0; s < cap && next < n; s++) {
+        if (!bs.slot_free(s)) continue;
+        reqs.push_back({s, &in[next]->v, nullptr, masses[next], std::sqrt(res[next]->rsqmin),
+                        inv_arg->max_iter});
         slot_of[s] = next++;
       }
+      bs.load_many(reqs.data(), (int)reqs.size());
       std::vector<int> done;
       bs.step(done);
       for (int d : done) {
