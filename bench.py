@@ -65,6 +65,13 @@ def main():
                          "batch-invariant results)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--eigcg", action="store_true",
+                    help="a sequence of EVENODD solves with eigCG deflation (option cg = 1) "
+                         "against plain CG on the same sources; prints one JSON line")
+    ap.add_argument("--eigcg-solves", type=int, default=6)
+    ap.add_argument("--eigcg-nev", type=int, default=8)
+    ap.add_argument("--eigcg-m", type=int, default=40)
+    ap.add_argument("--eigcg-l", type=int, default=120)
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the solver iterations one by one instead of replaying "
                          "captured hipGraphs (option qmx_graph = 0)")
@@ -108,6 +115,9 @@ def main():
         return
     if args.force:
         force_bench(args)
+        return
+    if args.eigcg:
+        eigcg_bench(args)
         return
 
     import qopqdp_amd as q
@@ -478,6 +488,62 @@ def multi_bench(args):
                             "converged": int(fp_ok)},
         "speedup_vs_fp64": round(rate / fp_rate, 2),
     }), flush=True)
+
+
+def eigcg_bench(args):
+    """--eigcg-solves EVENODD solves (the reference's cg = 1 eigCG of
+    lib/invert_eigcg_p.c inside the EVENODD invert) on --lattice at --mass to
+    rsqmin = tol^2, each on a new seeded Gaussian source, then the same
+    sources with plain CG (cg = 0); per-solve seconds and iterations of both."""
+    import qopqdp_amd as q
+    L = [int(x) for x in args.lattice.split(",")]
+    U = q.gen_gauge(L, 1000 + 10 * 6, 0.3)
+    s = q.Session(L)
+    if args.no_graph:
+        assert s.set_opts(qmx_graph=0) == 0
+    g = s.gauge(U)
+    del U
+    r0, bc, sign, keep = q.milc_rephase_args(L)
+    s.lib.QOP_D3_rephase_G(g, r0, C.byref(bc), C.byref(sign))
+    fla, finfo = s.links_from_gauge(g)
+    s.lib.QOP_D3_destroy_G(g)
+    srcs = [q.gen_source(L, 2600 + k, q.QOP_EVEN) for k in range(args.eigcg_solves)]
+
+    def sequence(cg):
+        assert s.set_opts(cg=cg, eigcg_nev=args.eigcg_nev, eigcg_m=args.eigcg_m,
+                          eigcg_l=args.eigcg_l) == 0
+        fl = s.links_from_raw(*s.extract_links(fla))  # fresh links: an empty eigCG space
+        secs, its = [], []
+        for b in srcs:
+            vin, vout = s.vec(b), s.zeros()
+            ia = q.invert_arg_default()
+            ia.evenodd = q.QOP_EVENODD
+            ra = q.resid_arg_default()
+            ra.rsqmin = args.tol * args.tol
+            info = q.QOPInfo()
+            t = time.perf_counter()
+            s.lib.QOP_D3_asqtad_invert(C.byref(info), fl, C.byref(ia), C.byref(ra), args.mass, vout, vin)
+            secs.append(round(time.perf_counter() - t, 4))
+            its.append(ra.final_iter)
+            assert ra.final_rsq <= args.tol * args.tol * 1.0001, ra.final_rsq
+            s.free(vin)
+            s.free(vout)
+        s.lib.QOP_D3_asqtad_destroy_L(fl)
+        return secs, its
+
+    sequence(0)  # warmup (autotuning of nothing; first launches)
+    e_s, e_it = sequence(1)
+    c_s, c_it = sequence(0)
+    s.set_opts(cg=0)
+    s.lib.QOP_D3_asqtad_destroy_L(fla)
+    s.close()
+    print(json.dumps({
+        "metric": "EVENODD solves with eigCG deflation vs plain CG, seconds per solve",
+        "lattice": L, "mass": args.mass, "tol": args.tol,
+        "eigcg": {"nev": args.eigcg_nev, "m": args.eigcg_m, "l": args.eigcg_l,
+                  "seconds": e_s, "iterations": e_it, "total_s": round(sum(e_s), 3)},
+        "cg": {"seconds": c_s, "iterations": c_it, "total_s": round(sum(c_s), 3)},
+        "eigcg_over_cg": round(sum(e_s) / sum(c_s), 3)}), flush=True)
 
 
 def force_bench(args):

@@ -12,6 +12,7 @@
 #include <complex>
 #include <cstring>
 #include <stdexcept>
+#include <utility>
 
 #include "ops.h"
 
@@ -154,22 +155,34 @@ class Eig {
   }
   double linop(DevVec &o, const DevVec &i) { return ddagd_norm2(L_, P64, mass_, o, i, eo_, tmp_.v); }
 
+  // y_j = sum_i R(i, j) x_i for j < n2 (R host, row-major n1 x n2) with
+  // the pointer arrays and R staged in one device buffer kept for the solve
+  void combine(const double2 *const *xp, int n1, double2 *const *yp, int n2, const std::vector<zd> &R) {
+    const size_t nb = (n1 + n2) * sizeof(void *) + (size_t)n1 * n2 * sizeof(double2);
+    if (nb > args_cap_) {
+      sync();
+      DevBuf<char> grown(nb);  // the old buffer is freed when `grown` goes
+      std::swap(args_.p, grown.p);
+      args_cap_ = nb;
+    }
+    std::vector<char> h(nb);
+    memcpy(h.data(), xp, n1 * sizeof(void *));
+    memcpy(h.data() + n1 * sizeof(void *), yp, n2 * sizeof(void *));
+    memcpy(h.data() + (n1 + n2) * sizeof(void *), R.data(), (size_t)n1 * n2 * sizeof(double2));
+    // synchronous upload: the buffer is reused by the next call
+    QOP_CHECK_HIP(hipStreamSynchronize(stream()));
+    QOP_CHECK_HIP(hipMemcpy(args_.p, h.data(), nb, hipMemcpyHostToDevice));
+    launch_rotate((const double2 *const *)args_.p, (double2 *const *)(args_.p + n1 * sizeof(void *)),
+                  (const double2 *)(args_.p + (n1 + n2) * sizeof(void *)), n1, n2, n2, 3 * Vh_, stream());
+  }
+
   // slots s[j] = sum_i R(i, j) slots s[i] for j < n2 (R host, row-major n1 x n2)
   void rotate(const int *s, int n1, int n2, const std::vector<zd> &R) {
     std::vector<const double2 *> xp(n1);
     std::vector<double2 *> yp(n2);
     for (int i = 0; i < n1; i++) xp[i] = slot(s[i]);
     for (int j = 0; j < n2; j++) yp[j] = slot(eg_.cap + j);
-    DevBuf<char> buf;
-    const size_t nb = (n1 + n2) * sizeof(void *) + (size_t)n1 * n2 * sizeof(double2);
-    dev_alloc(&buf.p, nb);
-    std::vector<char> h(nb);
-    memcpy(h.data(), xp.data(), n1 * sizeof(void *));
-    memcpy(h.data() + n1 * sizeof(void *), yp.data(), n2 * sizeof(void *));
-    memcpy(h.data() + (n1 + n2) * sizeof(void *), R.data(), (size_t)n1 * n2 * sizeof(double2));
-    QOP_CHECK_HIP(hipMemcpy(buf.p, h.data(), nb, hipMemcpyHostToDevice));
-    launch_rotate((const double2 *const *)buf.p, (double2 *const *)(buf.p + n1 * sizeof(void *)),
-                  (const double2 *)(buf.p + (n1 + n2) * sizeof(void *)), n1, n2, n2, 3 * Vh_, stream());
+    combine(xp.data(), n1, yp.data(), n2, R);
     for (int j = 0; j < n2; j++)
       QOP_CHECK_HIP(hipMemcpyAsync(slot(s[j]), yp[j], 3 * Vh_ * sizeof(double2),
                                    hipMemcpyDeviceToDevice, stream()));
@@ -205,10 +218,61 @@ class Eig {
   }
 
   // slots s0+i0 .. s0+i0+n-1 orthonormalised against all slots before them
-  // (invert_eigcg_p.c:201-213; classical Gram-Schmidt applied twice, one
-  // Gram product and one rotation per pass, where the reference subtracts
-  // vector by vector)
+  // (invert_eigcg_p.c:201-213) in block form: the new block is projected off
+  // the i0 earlier slots twice (one Gram product and one combination per
+  // pass), then orthonormalised within itself by Cholesky QR, twice; the
+  // reference subtracts vector by vector.  The span is the same; a block
+  // whose Gram matrix is not numerically positive definite falls back to
+  // the vector-by-vector form.
   void orthonormalize(int s0, int i0, int n) {
+    if (n <= 0) return;
+    for (int pass = 0; pass < 2 && i0 > 0; pass++) {
+      const std::vector<zd> c = gram(s0, i0, s0 + i0, n);  // i0 x n
+      std::vector<int> sl(n + i0);
+      std::vector<zd> R((size_t)(n + i0) * n, zd(0));
+      for (int j = 0; j < n; j++) {
+        sl[j] = s0 + i0 + j;
+        R[(size_t)j * n + j] = 1;
+      }
+      for (int k = 0; k < i0; k++) {
+        sl[n + k] = s0 + k;
+        for (int j = 0; j < n; j++) R[(size_t)(n + k) * n + j] = -c[(size_t)k * n + j];
+      }
+      rotate(sl.data(), n + i0, n, R);
+    }
+    bool ok = true;
+    for (int pass = 0; pass < 2 && ok; pass++) {
+      std::vector<zd> G = gram(s0 + i0, n, s0 + i0, n);  // Hermitian, G(i, j) = <v_i, v_j>
+      // G = U^+ U (U upper triangular); the block times U^-1 is orthonormal
+      std::vector<zd> U((size_t)n * n, zd(0));
+      for (int j = 0; j < n && ok; j++) {
+        zd d = G[(size_t)j * n + j];
+        for (int k = 0; k < j; k++) d -= std::conj(U[(size_t)k * n + j]) * U[(size_t)k * n + j];
+        if (!(d.real() > 1e-12 * std::abs(G[(size_t)j * n + j]))) { ok = false; break; }
+        const double ujj = std::sqrt(d.real());
+        U[(size_t)j * n + j] = ujj;
+        for (int i = j + 1; i < n; i++) {
+          zd e = G[(size_t)j * n + i];
+          for (int k = 0; k < j; k++) e -= std::conj(U[(size_t)k * n + j]) * U[(size_t)k * n + i];
+          U[(size_t)j * n + i] = e / ujj;
+        }
+      }
+      if (!ok) break;
+      // W = U^-1 (upper triangular), by back substitution column by column
+      std::vector<zd> W((size_t)n * n, zd(0));
+      for (int j = 0; j < n; j++) {
+        W[(size_t)j * n + j] = 1.0 / U[(size_t)j * n + j];
+        for (int i = j - 1; i >= 0; i--) {
+          zd sum = 0;
+          for (int k = i + 1; k <= j; k++) sum += U[(size_t)i * n + k] * W[(size_t)k * n + j];
+          W[(size_t)i * n + j] = -sum / U[(size_t)i * n + i];
+        }
+      }
+      std::vector<int> sl(n);
+      for (int j = 0; j < n; j++) sl[j] = s0 + i0 + j;
+      rotate(sl.data(), n, n, W);  // v'_j = sum_i W(i, j) v_i
+    }
+    if (ok) return;
     for (int i = i0; i < i0 + n; i++) {
       for (int pass = 0; pass < 2 && i > 0; pass++) {
         const std::vector<zd> c = gram(s0, i, s0 + i, 1);
@@ -257,15 +321,20 @@ class Eig {
     if (n == 1) eg_.l[1] = 0;
   }
 
-  // x = sum_i u_i (u_i^+ b) / l_i (invert_eigcg_p.c:328-341)
+  // x = sum_i u_i (u_i^+ b) / l_i (invert_eigcg_p.c:328-341): the n
+  // products in one Gram pass (b staged in a scratch slot), the sum in one
+  // combination
   void initcg(DevVec &x, int n, const DevVec &b) {
     if (n <= 0) return;
-    zero(x, eo_);
-    for (int i = 0; i < n; i++) {
-      const DevVec ui = view(i);
-      const Cplx z = cdot(ui, b, eo_);
-      caxpy(x, Cplx{z.re / eg_.l[i], z.im / eg_.l[i]}, ui, eo_);
-    }
+    QOP_CHECK_HIP(hipMemcpyAsync(slot(eg_.cap), b.half(eo_).data, 3 * Vh_ * sizeof(double2),
+                                 hipMemcpyDeviceToDevice, stream()));
+    const std::vector<zd> z = gram(0, n, eg_.cap, 1);
+    std::vector<zd> c(n);
+    for (int i = 0; i < n; i++) c[i] = z[i] / eg_.l[i];
+    std::vector<const double2 *> xp(n);
+    for (int i = 0; i < n; i++) xp[i] = slot(i);
+    double2 *yp[1] = {(double2 *)x.half(eo_).data};
+    combine(xp.data(), n, yp, 1, c);
   }
 
   int nev, m;
@@ -357,8 +426,13 @@ class Eig {
     std::fill(t.begin(), t.end(), 0.0);
     DevVec vl = view(v0 + nv - 1);
     linop(Mp, vl);
+    // Re <v_i, A v_last> for i < nv - 1 in one Gram pass (A v_last staged in
+    // a scratch slot)
+    QOP_CHECK_HIP(hipMemcpyAsync(slot(eg_.cap), Mp.half(eo_).data, 3 * Vh_ * sizeof(double2),
+                                 hipMemcpyDeviceToDevice, stream()));
+    const std::vector<zd> g = gram(v0, nv - 1, eg_.cap, 1);
     for (int i = 0; i < nv - 1; i++) {
-      const double rr = redot(view(v0 + i), Mp, eo_);
+      const double rr = g[i].real();
       t[(size_t)i * m + nv - 1] = rr;
       t[(size_t)(nv - 1) * m + i] = rr;
       t[(size_t)i * m + i] = td[i];
@@ -374,6 +448,8 @@ class Eig {
   long Vh_;
   TmpVec tmp_;
   DevBuf<double2> partial_, gdev_;
+  DevBuf<char> args_;  // combine(): pointer arrays and coefficients
+  size_t args_cap_ = 0;
 };
 
 }  // namespace
