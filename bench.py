@@ -81,7 +81,8 @@ def main():
     ap.add_argument("--roofline-only", action="store_true",
                     help="only time the dominant hop kernel (for the rocprofv3 cross-check)")
     ap.add_argument("--shape", default=None,
-                    help="--roofline-only: force the hop launch shape (8 comma-separated ints: "
+                    help="--roofline-only, --single-rhs-only: force the hop launch shape (8 "
+                         "comma-separated ints: "
                          "RHS per thread, block, XCD remap, non-temporal, RHS-innermost, x0 tile, "
                          "LDS staging, 0) instead of the autotuned one")
     ap.add_argument("--serve", action="store_true",
@@ -141,11 +142,16 @@ def main():
 
     if args.single_rhs_only:
         res = {}
+        if args.shape:
+            forced = [int(x) for x in args.shape.split(",")]
+            assert len(forced) == 8, "--shape takes 8 integers"
+            s.lib.QOPX_hop_shape_override((C.c_int * 8)(*forced))
         for prec, sl, sv in ((64, 144, 48), (32, 72, 24), (16, 40, 16)):
-            ms = s.lib.QOPX_D3_asqtad_hop_time(fla, prec, 1, 100, 0, None)
+            shape = (C.c_int * 8)()
+            ms = s.lib.QOPX_D3_asqtad_hop_time(fla, prec, 1, 100, 0, shape)
             byts = (16 * sl + 2 * sv) * Vh
             res[f"fp{prec}"] = {"kernel_us": round(ms * 1e3, 1), "GBs": round(byts / ms / 1e6, 1),
-                                "algorithmic_bytes": byts}
+                                "shape": list(shape), "algorithmic_bytes": byts}
         cbytes = 1 << 30
         cms = s.lib.QOPX_copy_time(cbytes, 20)
         res["copy"] = {"bytes_read": cbytes, "us": round(cms * 1e3, 1),

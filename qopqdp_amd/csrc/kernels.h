@@ -32,7 +32,9 @@ struct HopArgs {
   int tile = 0;                  // x0-tiled site order: sites per tile of an x0 slab (0: plain)
   int stage = 0;                 // LDS staging: the block's own source-parity sites are
                                  // loaded once into LDS and the neighbours inside that
-                                 // range (x3 and most x2 hops) are read from there
+                                 // set are read from there; 1: the block's rows (x3 and
+                                 // most x2 hops), 2: plane-blocked order (also the x0
+                                 // hops inside the block's planes)
   // single-stored links: `links` holds the forward links of the output
   // parity, links_bck those of the source parity ([nl/2][9][Vh] each)
   int single = 0;

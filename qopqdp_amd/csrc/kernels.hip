@@ -53,22 +53,49 @@ __device__ __forceinline__ void ld_c(const void *base, long i, float scl,
   }
 }
 
-// Colour vector from an LDS tile holding the stored words ([3][bs] complex
-// elements, FP16 norms [bs]): the same conversion as vload, so the values are
-// bit-identical to a load from memory.
-template <int P>
-__device__ __forceinline__ void tload(const typename PT<P>::S *td, const float *tn, int o, int bs,
-                                      typename PT<P>::R v[6]) {
-  float n = 1.0f;
-  if constexpr (P == P16) n = tn[o];
+// Stored words of one link (9 complex elements, + the per-link scale of FP16
+// links), loaded one direction ahead of their use so that the loads stay in
+// flight while the previous direction's products are formed.
+template <int LP> struct LinkWords;
+template <> struct LinkWords<P64> { double2 e[9]; };
+template <> struct LinkWords<P32> { float2 e[9]; };
+template <> struct LinkWords<P16> { unsigned e[9]; float s; };
+
+template <int LP, bool NT>
+__device__ __forceinline__ void link_words_load(LinkWords<LP> &w, const void *links,
+                                                const float *lscale, long k, long Vh, long h) {
+  const long base = k * 9 * Vh + h;
+  if constexpr (LP == P16) {
+    if constexpr (NT) w.s = __builtin_nontemporal_load(lscale + k * Vh + h);
+    else w.s = lscale[k * Vh + h];
+  }
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    if constexpr (P == P16) {
-      float2 t = __half22float2(td[c * bs + o]);
-      v[2 * c] = t.x * n; v[2 * c + 1] = t.y * n;
+  for (int e = 0; e < 9; e++) {
+    const long i = base + (long)e * Vh;
+    if constexpr (LP == P64) {
+      const double *p = (const double *)links + 2 * i;
+      if constexpr (NT) w.e[e] = make_double2(__builtin_nontemporal_load(p), __builtin_nontemporal_load(p + 1));
+      else w.e[e] = ((const double2 *)links)[i];
+    } else if constexpr (LP == P32) {
+      if constexpr (NT) w.e[e] = __builtin_bit_cast(float2, __builtin_nontemporal_load((const double *)links + i));
+      else w.e[e] = ((const float2 *)links)[i];
     } else {
-      const auto t = td[c * bs + o];
-      v[2 * c] = t.x; v[2 * c + 1] = t.y;
+      if constexpr (NT) w.e[e] = __builtin_nontemporal_load((const unsigned *)links + i);
+      else w.e[e] = ((const unsigned *)links)[i];
+    }
+  }
+}
+
+// The link as 18 reals: the same conversion as ld_c.
+template <int LP, typename R>
+__device__ __forceinline__ void link_words_cvt(const LinkWords<LP> &w, R out[18]) {
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    if constexpr (LP == P16) {
+      float2 f = __half22float2(__builtin_bit_cast(__half2, w.e[e]));
+      out[2 * e] = f.x * w.s; out[2 * e + 1] = f.y * w.s;
+    } else {
+      out[2 * e] = w.e[e].x; out[2 * e + 1] = w.e[e].y;
     }
   }
 }
@@ -374,6 +401,14 @@ hop_kernel(const HopArgs a) {
       for (int i = 0; i < 6; i++) acc[b][i] = 0;
 
     constexpr int PER = NL / 4;  // links per direction: 4 (fat+long) or 2
+    // Per link: the NB gathers are issued first, then the next link's words,
+    // and only then are the values converted and used: one wait per link
+    // instead of one per RHS.  With one RHS per thread, and in FP64, each
+    // link is loaded where it is used instead (there the prefetch costs more
+    // registers than it hides).
+    constexpr bool PF = NB > 1 && !SGL && P != P64;
+    LinkWords<LP> lnext;
+    if constexpr (PF) link_words_load<LP, NT>(lnext, a.links, a.lscale, 0, Vh, h);
 #pragma unroll 1
     for (int mu = 0; mu < 4; mu++) {
 #pragma unroll
@@ -382,6 +417,51 @@ hop_kernel(const HopArgs a) {
                                : (j == 0 ? 1 : -1);
         const int k = PER * mu + j;
         const int nb = nbr(g, x, lex, mu, d);
+        int slot = -1;  // LDS position of the neighbour, or -1
+        if constexpr (STG) {
+          if (planes) {
+            if (mu == 0) {
+              if ((unsigned)(wv + d) < (unsigned)nwv) slot = (int)threadIdx.x + 64 * d;
+            } else {
+              const unsigned pos = (unsigned)(nb - hrow);
+              if (pos < 64u) slot = (wv << 6) + (int)pos;
+            }
+          } else {
+            const unsigned o = (unsigned)(nb - h0);
+            if (o < (unsigned)bs) slot = (int)o;
+          }
+        }
+        const bool in_lds = STG && slot >= 0;
+        LinkWords<LP> lcur;  // without the prefetch: this link, loaded before the gathers
+        if constexpr (!PF && !SGL) link_words_load<LP, NT>(lcur, a.links, a.lscale, k, Vh, h);
+        // stored words of the neighbour for each RHS: a wave with any LDS
+        // lane first reads the tile (lanes without a staged neighbour read
+        // position 0), then the memory lanes load over their values, so the
+        // memory loads wait only for the short LDS reads
+        S gm[NB][3];
+        float nm[NB];
+        if constexpr (STG) {
+          if (__any(in_lds)) {
+            const int o = in_lds ? slot : 0;
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+              if (!act[b]) continue;
+#pragma unroll
+              for (int c = 0; c < 3; c++) gm[b][c] = td[(b * 3 + c) * bs + o];
+              if constexpr (P == P16) nm[b] = tn[b * bs + o];
+            }
+          }
+        }
+        if (!in_lds) {
+#pragma unroll
+          for (int b = 0; b < NB; b++) {
+            if (!act[b]) continue;
+            const S *sd = (const S *)a.src.data + (long)(b0 + b) * a.sstride;
+#pragma unroll
+            for (int c = 0; c < 3; c++) gm[b][c] = sd[(long)c * Vh + nb];
+            if constexpr (P == P16) nm[b] = a.src.norm[(long)(b0 + b) * a.nstride + nb];
+          }
+        }
         R w[18];
         if constexpr (SGL) {
           // single-stored: forward link of this site, or -U(x-d)^dagger from
@@ -407,37 +487,31 @@ hop_kernel(const HopArgs a) {
               }
             }
         } else {
-          const long lbase = (long)k * 9 * Vh + h;
-          float scl = 1.0f;
-          if constexpr (LP == P16) {
-            if constexpr (NT) scl = __builtin_nontemporal_load(a.lscale + (long)k * Vh + h);
-            else scl = a.lscale[(long)k * Vh + h];
-          }
-#pragma unroll
-          for (int e = 0; e < 9; e++) ld_c<LP, NT>(a.links, lbase + (long)e * Vh, scl, w[2 * e], w[2 * e + 1]);
-        }
-        int slot = -1;  // LDS position of the neighbour, or -1
-        if constexpr (STG) {
-          if (planes) {
-            if (mu == 0) {
-              if ((unsigned)(wv + d) < (unsigned)nwv) slot = (int)threadIdx.x + 64 * d;
-            } else {
-              const unsigned pos = (unsigned)(nb - hrow);
-              if (pos < 64u) slot = (wv << 6) + (int)pos;
-            }
+          if constexpr (PF) {
+            const LinkWords<LP> cur = lnext;
+            if (j + 1 < PER || mu < 3) link_words_load<LP, NT>(lnext, a.links, a.lscale, k + 1, Vh, h);
+            link_words_cvt<LP>(cur, w);
           } else {
-            const unsigned o = (unsigned)(nb - h0);
-            if (o < (unsigned)bs) slot = (int)o;
+            link_words_cvt<LP>(lcur, w);
           }
         }
 #pragma unroll
         for (int b = 0; b < NB; b++) {
           if (!act[b]) continue;
-          const char *sd = (const char *)a.src.data + (long)(b0 + b) * a.sstride * sizeof(typename PT<P>::S);
-          const float *sn = a.src.norm ? a.src.norm + (long)(b0 + b) * a.nstride : nullptr;
+          // the same conversions as vload
+          float n = 1.0f;
+          if constexpr (P == P16) n = nm[b];
           R v[6];
-          if (STG && slot >= 0) tload<P>(td + b * 3 * bs, tn + b * bs, slot, bs, v);
-          else vload<P>(sd, sn, nb, Vh, v);
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            const S t = gm[b][c];
+            if constexpr (P == P16) {
+              const float2 f = __half22float2(t);
+              v[2 * c] = f.x * n; v[2 * c + 1] = f.y * n;
+            } else {
+              v[2 * c] = t.x; v[2 * c + 1] = t.y;
+            }
+          }
 #pragma unroll
           for (int r = 0; r < 3; r++) {
             R re = acc[b][2 * r], im = acc[b][2 * r + 1];
