@@ -463,6 +463,7 @@ hop_kernel(const HopArgs a) {
           }
         }
         R w[18];
+        LinkWords<LP> lw;  // this link's stored words
         if constexpr (SGL) {
           // single-stored: forward link of this site, or -U(x-d)^dagger from
           // the forward link of the neighbour (source parity)
@@ -488,42 +489,59 @@ hop_kernel(const HopArgs a) {
             }
         } else {
           if constexpr (PF) {
-            const LinkWords<LP> cur = lnext;
+            lw = lnext;
             if (j + 1 < PER || mu < 3) link_words_load<LP, NT>(lnext, a.links, a.lscale, k + 1, Vh, h);
-            link_words_cvt<LP>(cur, w);
           } else {
-            link_words_cvt<LP>(lcur, w);
+            lw = lcur;
           }
+          if constexpr (P != P16) link_words_cvt<LP>(lw, w);
         }
 #pragma unroll
         for (int b = 0; b < NB; b++) {
           if (!act[b]) continue;
-          // the same conversions as vload
-          float n = 1.0f;
-          if constexpr (P == P16) n = nm[b];
-          R v[6];
+          if constexpr (P == P16) {
+            // FP16 links and vectors: the products of the stored half-precision
+            // words, exact in FP32, summed per colour row with mixed-precision
+            // FMAs (no conversions), then scaled once by link scale x site norm
+            const float sn = lw.s * nm[b];
+            float t[6];
 #pragma unroll
-          for (int c = 0; c < 3; c++) {
-            const S t = gm[b][c];
-            if constexpr (P == P16) {
-              const float2 f = __half22float2(t);
-              v[2 * c] = f.x * n; v[2 * c + 1] = f.y * n;
-            } else {
-              v[2 * c] = t.x; v[2 * c + 1] = t.y;
+            for (int r = 0; r < 3; r++) {
+              float re = 0.0f, im = 0.0f;
+#pragma unroll
+              for (int c = 0; c < 3; c++) {
+                const __half2 wh = __builtin_bit_cast(__half2, lw.e[3 * r + c]);
+                const float wr = __low2float(wh), wi = __high2float(wh);
+                const float vr = __low2float(gm[b][c]), vi = __high2float(gm[b][c]);
+                re = fmaf(wr, vr, re);
+                re = fmaf(-wi, vi, re);
+                im = fmaf(wr, vi, im);
+                im = fmaf(wi, vr, im);
+              }
+              t[2 * r] = re; t[2 * r + 1] = im;
             }
-          }
 #pragma unroll
-          for (int r = 0; r < 3; r++) {
-            R re = acc[b][2 * r], im = acc[b][2 * r + 1];
+            for (int q = 0; q < 6; q++) acc[b][q] = fmaf(sn, t[q], acc[b][q]);
+          } else {
+            R v[6];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-              const R wr = w[6 * r + 2 * c], wi = w[6 * r + 2 * c + 1];
-              re = fma(wr, v[2 * c], re);
-              re = fma(-wi, v[2 * c + 1], re);
-              im = fma(wr, v[2 * c + 1], im);
-              im = fma(wi, v[2 * c], im);
+              const S t = gm[b][c];
+              v[2 * c] = t.x; v[2 * c + 1] = t.y;
             }
-            acc[b][2 * r] = re; acc[b][2 * r + 1] = im;
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+              R re = acc[b][2 * r], im = acc[b][2 * r + 1];
+#pragma unroll
+              for (int c = 0; c < 3; c++) {
+                const R wr = w[6 * r + 2 * c], wi = w[6 * r + 2 * c + 1];
+                re = fma(wr, v[2 * c], re);
+                re = fma(-wi, v[2 * c + 1], re);
+                im = fma(wr, v[2 * c + 1], im);
+                im = fma(wi, v[2 * c], im);
+              }
+              acc[b][2 * r] = re; acc[b][2 * r + 1] = im;
+            }
           }
         }
       }
