@@ -84,7 +84,7 @@ def main():
                     help="--roofline-only, --single-rhs-only: force the hop launch shape (8 "
                          "comma-separated ints: "
                          "RHS per thread, block, XCD remap, non-temporal, RHS-innermost, x0 tile, "
-                         "LDS staging, 0) instead of the autotuned one")
+                         "LDS staging, plain loads) instead of the autotuned one")
     ap.add_argument("--serve", action="store_true",
                     help="serving mix instead: Poisson arrivals over several lattices")
     ap.add_argument("--serve-lattices", default="16,16,16,16;24,24,24,48;32,32,32,64")

@@ -395,7 +395,7 @@ int QOPX_D3_asqtad_hop_noalloc(QOP_D3_FermionLinksAsqtad *fla, int prec, int n,
 
 /* Mean ms per launch of the even->odd hop kernel (precision 64/32/16, nrhs
  * right-hand sides, autotuned shape (rhs per thread, block, XCD remap,
- * non-temporal links, RHS-innermost order, x0 tile, LDS staging, 0 = reserved)
+ * non-temporal links, RHS-innermost order, x0 tile, LDS staging, plain loads)
  * returned in shape[8]), from HIP events.  red = 0: plain hop; 1: with the
  * fused |out|^2 reduction (first hop of a batched CG iteration); 2: with the
  * fused CG update r -= a (m^2 p - D t), x += a p, |r|^2 (second hop;
@@ -404,9 +404,11 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
                                int red, int *shape);
 
 /* Force the multi-RHS hop launch shape {rhs per thread, block, XCD remap,
- * non-temporal links, RHS-innermost order, x0 tile, LDS staging, reserved = 0} of solvers built from
- * now on, where it is valid for their lattice and precision (tests of
- * launch-shape independence); null restores autotuning. */
+ * non-temporal links, RHS-innermost order, x0 tile, LDS staging, plain loads
+ * (1: FP32/FP16 through the one-RHS-per-thread plain-load kernel that
+ * lattices beyond the 2^31-byte buffer range take; 0: buffer loads)} of
+ * solvers built from now on, where it is valid for their lattice and
+ * precision (tests of launch-shape independence); null restores autotuning. */
 void QOPX_hop_shape_override(const int *shape);
 
 /* Mean ms per launch of a plain device copy of `bytes` (multiple of 16):

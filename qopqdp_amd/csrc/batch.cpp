@@ -34,6 +34,7 @@ static bool shape_valid(const Lattice *lat, int prec, int nl, int nrhs, const Ho
   if (s.nb > 1 && s.nb > nrhs) return false;
   if (s.block != 64 && s.block != 128 && s.block != 256 && !(s.block == 512 && s.stage)) return false;
   if (s.tile && (s.tile % s.block != 0 || slab % s.tile != 0 || s.tile >= slab)) return false;
+  if (s.plain && (s.plain != 1 || prec == P64 || s.nb != 1 || s.stage || s.nt)) return false;
   if (s.stage) {
     if ((s.stage != 1 && s.stage != 2) || prec == P64 || nl != 16 || s.nb > 8 || s.nt) return false;
     const size_t el = prec == P16 ? 16 : 24;  // stored bytes per site and RHS
@@ -105,7 +106,7 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
   const int blocks[] = {64, 128, 256};
   const int tiles[] = {0, 512, 1024, 2048, 4096};
   const int slab = (int)(lat->Vh / lat->L[0]);
-  const int reps = lat->Vh > 200000 ? 4 : 10;
+  const int reps = lat->Vh > 200000 ? 8 : 10;
   auto time_shape = [&](const HopShape &cand) {
     cand.apply(a);
     launch_hop_args(a, prec, ls.nl, cand.nb, cand.block, red, hasx, st);
@@ -134,31 +135,36 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
           if (tl && (tl % blk != 0 || slab % tl != 0 || tl >= slab)) continue;
           time_shape(HopShape{nb, blk, rm, nt, ri, tl, 0});
         }
-  // stage 3: LDS staging, for the best RHS per thread / order: of the block's
-  // source rows at every block size and x0 tile, and of its x0 planes
-  // (plane-blocked order) at 2, 4 and 8 planes per block (plain link loads)
+  // stage 3: LDS staging, for the best RHS per thread / order, with and
+  // without the XCD remap: of the block's source rows at every block size and
+  // x0 tile, and of its x0 planes (plane-blocked order) at 2, 4 and 8 planes
+  // per block (plain link loads)
   if (prec != P64 && L.nl == 16) {
     const HopShape b2 = best;
-    for (int sblk : blocks)
-      for (int tl : tiles) {
-        HopShape c = b2;
-        c.block = sblk;
-        c.nt = 0;
-        c.tile = tl;
-        c.stage = 1;
-        if (c.nb > 8) c.nb = 8;
-        if (shape_valid(lat, prec, L.nl, nrhs, c)) time_shape(c);
-      }
-    for (int sblk : {128, 256, 512})
-      for (int snb : {b2.nb, b2.nb / 2}) {
-        HopShape c = b2;
-        c.block = sblk;
-        c.nt = 0;
-        c.tile = 0;
-        c.stage = 2;
-        c.nb = snb > 8 ? 8 : snb < 1 ? 1 : snb;
-        if (shape_valid(lat, prec, L.nl, nrhs, c)) time_shape(c);
-      }
+    for (int rm = 0; rm < 2; rm++) {
+      for (int sblk : blocks)
+        for (int tl : tiles) {
+          HopShape c = b2;
+          c.remap = rm;
+          c.block = sblk;
+          c.nt = 0;
+          c.tile = tl;
+          c.stage = 1;
+          if (c.nb > 8) c.nb = 8;
+          if (shape_valid(lat, prec, L.nl, nrhs, c)) time_shape(c);
+        }
+      for (int sblk : {128, 256, 512})
+        for (int snb : {b2.nb, b2.nb / 2}) {
+          HopShape c = b2;
+          c.remap = rm;
+          c.block = sblk;
+          c.nt = 0;
+          c.tile = 0;
+          c.stage = 2;
+          c.nb = snb > 8 ? 8 : snb < 1 ? 1 : snb;
+          if (shape_valid(lat, prec, L.nl, nrhs, c)) time_shape(c);
+        }
+    }
   }
   QOP_CHECK_HIP(hipEventDestroy(e0));
   QOP_CHECK_HIP(hipEventDestroy(e1));
@@ -168,8 +174,8 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
   }
   if (options().verbose)
     printf("TUNE: prec %d nrhs %d -> nb %d block %d remap %d nt %d inner %d tile %d stage %d "
-           "(%.3f ms/hop)\n", prec, nrhs, best.nb, best.block, best.remap, best.nt, best.rhs_inner,
-           best.tile, best.stage, best_ms / reps);
+           "plain %d (%.3f ms/hop)\n", prec, nrhs, best.nb, best.block, best.remap, best.nt,
+           best.rhs_inner, best.tile, best.stage, best.plain, best_ms / reps);
   cache[key] = best;
   return best;
 }

@@ -70,13 +70,15 @@ void bs_launch_sum(const double *part, int nch, int n, const int *sel,
 // Tuned hop launch shape for (lattice, precision, nl, nrhs), timed on first use
 // and cached (batch.cpp).
 struct HopShape {
-  int nb = 1, block = 256, remap = 1, nt = 0, rhs_inner = 0, tile = 0, stage = 0;
+  // plain: force the one-RHS-per-thread plain-load instance (FP32 / FP16)
+  int nb = 1, block = 256, remap = 1, nt = 0, rhs_inner = 0, tile = 0, stage = 0, plain = 0;
   void apply(HopArgs &a) const {
     a.remap = remap;
     a.nt = nt;
     a.rhs_inner = rhs_inner;
     a.tile = tile;
     a.stage = stage;
+    a.plain = plain;
   }
 };
 struct SlotResult {

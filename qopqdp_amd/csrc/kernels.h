@@ -43,6 +43,9 @@ struct HopArgs {
   const double *cg_alpha = nullptr;  // RED_CG: alpha of slot b at cg_alpha[b*cg_stride]
   int cg_stride = 0;
   int ngroups = 1;               // set by launch_hop_args
+  int buf = 1;                   // set by launch_hop_args: FP32 / FP16 buffer loads
+  int plain = 0;                 // FP32 / FP16: one RHS per thread with plain loads (the
+                                 // instance lattices beyond the buffer range take)
 };
 void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
                      bool hasx, hipStream_t st);

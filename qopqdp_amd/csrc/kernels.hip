@@ -16,6 +16,7 @@
 // right-hand sides per thread or on which other right-hand sides share the
 // launch.
 #include "common.h"
+#include <type_traits>
 #include "kernels.h"
 
 namespace qop {
@@ -98,6 +99,40 @@ __device__ __forceinline__ void link_words_cvt(const LinkWords<LP> &w, R out[18]
       out[2 * e] = w.e[e].x; out[2 * e + 1] = w.e[e].y;
     }
   }
+}
+
+// Buffer loads: a resource over the RHS group's vectors (or one direction's
+// links), so a load takes one per-thread 32-bit byte offset register (site
+// index x element size) and a uniform scalar offset (RHS, colour, link
+// element) instead of a 64-bit address formed per load.  launch_hop_args
+// uses them only where every offset stays below 2^31, the resource's range.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, 0x7fffffff, 0x00020000);
+}
+
+template <class T, bool NT>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, unsigned vo, unsigned so) {
+  constexpr int aux = NT ? 2 : 0;  // 2: non-temporal
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, aux));
+  } else {
+    static_assert(sizeof(T) == 8, "buffer loads of 4 or 8 bytes");
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, aux));
+  }
+}
+
+// link_words_load through buffer loads (FP32 / FP16 links): ho = h x element
+// bytes, hs = 4 h.
+template <int LP, bool NT>
+__device__ __forceinline__ void link_words_load_buf(LinkWords<LP> &w, const void *links,
+                                                    const float *lscale, long k, long Vh,
+                                                    unsigned ho, unsigned hs) {
+  static_assert(LP == P32 || LP == P16, "buffer link loads: FP32 or FP16 links");
+  using E = typename std::conditional<LP == P32, float2, unsigned>::type;
+  if constexpr (LP == P16) w.s = buf_load<float, NT>(buf_rsrc(lscale + k * Vh), hs, 0);
+  const auto r = buf_rsrc((const E *)links + k * 9 * Vh);
+#pragma unroll
+  for (int e = 0; e < 9; e++) w.e[e] = buf_load<E, NT>(r, ho, (unsigned)(e * Vh * (long)sizeof(E)));
 }
 
 // Load the 3 colour components of site h from a parity-half view.
@@ -339,9 +374,11 @@ __device__ __forceinline__ void hop_epilogue(const HopArgs &a, int h, int Vh, in
 // wave's 64 positions come from LDS.  Staged blocks take up to 512 threads.
 // LP: precision of the stored links (P, or P16 for FP32 vectors with FP16
 // links and per-link scales).
-template <int P, int NL, int NB, int RED, bool HASX, bool NT, bool SGL, bool STG, int LP = P>
+template <int P, int NL, int NB, int RED, bool HASX, bool NT, bool SGL, bool STG, int LP = P,
+          bool BUF = false>
 __global__ void __launch_bounds__(STG ? 512 : 256)
 hop_kernel(const HopArgs a) {
+  static_assert(!BUF || (P != P64 && !SGL), "buffer loads: FP32 / FP16, double-stored links");
   using R = typename PT<P>::R;
   const Geom g = a.g;
   const int Vh = g.Vh;
@@ -407,8 +444,16 @@ hop_kernel(const HopArgs a) {
     // link is loaded where it is used instead (there the prefetch costs more
     // registers than it hides).
     constexpr bool PF = NB > 1 && !SGL && P != P64;
+    // BUF: gathers and link loads through buffer loads
+    const auto rs_src = buf_rsrc((const S *)a.src.data + (long)b0 * a.sstride);
+    const auto rs_nrm = buf_rsrc(P == P16 ? a.src.norm + (long)b0 * a.nstride : a.src.norm);
+    const unsigned lho = (unsigned)h * (unsigned)(LP == P32 ? 8 : 4);
+    auto load_link = [&](LinkWords<LP> &lw_, int kk) {
+      if constexpr (BUF) link_words_load_buf<LP, NT>(lw_, a.links, a.lscale, kk, Vh, lho, (unsigned)h * 4u);
+      else link_words_load<LP, NT>(lw_, a.links, a.lscale, kk, Vh, h);
+    };
     LinkWords<LP> lnext;
-    if constexpr (PF) link_words_load<LP, NT>(lnext, a.links, a.lscale, 0, Vh, h);
+    if constexpr (PF) load_link(lnext, 0);
 #pragma unroll 1
     for (int mu = 0; mu < 4; mu++) {
 #pragma unroll
@@ -433,7 +478,7 @@ hop_kernel(const HopArgs a) {
         }
         const bool in_lds = STG && slot >= 0;
         LinkWords<LP> lcur;  // without the prefetch: this link, loaded before the gathers
-        if constexpr (!PF && !SGL) link_words_load<LP, NT>(lcur, a.links, a.lscale, k, Vh, h);
+        if constexpr (!PF && !SGL) load_link(lcur, k);
         // stored words of the neighbour for each RHS: a wave with any LDS
         // lane first reads the tile (lanes without a staged neighbour read
         // position 0), then the memory lanes load over their values, so the
@@ -453,13 +498,22 @@ hop_kernel(const HopArgs a) {
           }
         }
         if (!in_lds) {
+          const unsigned vo = (unsigned)nb * (unsigned)sizeof(S);
 #pragma unroll
           for (int b = 0; b < NB; b++) {
             if (!act[b]) continue;
-            const S *sd = (const S *)a.src.data + (long)(b0 + b) * a.sstride;
+            if constexpr (BUF) {
 #pragma unroll
-            for (int c = 0; c < 3; c++) gm[b][c] = sd[(long)c * Vh + nb];
-            if constexpr (P == P16) nm[b] = a.src.norm[(long)(b0 + b) * a.nstride + nb];
+              for (int c = 0; c < 3; c++)
+                gm[b][c] = buf_load<S, false>(rs_src, vo, (unsigned)(((long)b * a.sstride + (long)c * Vh) * (long)sizeof(S)));
+              if constexpr (P == P16)
+                nm[b] = buf_load<float, false>(rs_nrm, (unsigned)nb * 4u, (unsigned)((long)b * a.nstride * 4));
+            } else {
+              const S *sd = (const S *)a.src.data + (long)(b0 + b) * a.sstride;
+#pragma unroll
+              for (int c = 0; c < 3; c++) gm[b][c] = sd[(long)c * Vh + nb];
+              if constexpr (P == P16) nm[b] = a.src.norm[(long)(b0 + b) * a.nstride + nb];
+            }
           }
         }
         R w[18];
@@ -490,7 +544,7 @@ hop_kernel(const HopArgs a) {
         } else {
           if constexpr (PF) {
             lw = lnext;
-            if (j + 1 < PER || mu < 3) link_words_load<LP, NT>(lnext, a.links, a.lscale, k + 1, Vh, h);
+            if (j + 1 < PER || mu < 3) load_link(lnext, k + 1);
           } else {
             lw = lcur;
           }
@@ -569,12 +623,15 @@ static size_t stage_bytes(int prec, int nb, int block) {
   return prec == P16 ? (size_t)nb * block * (3 * 4 + 4) : (size_t)nb * block * 3 * el;
 }
 
-template <int P, int NL, int NB, bool NT, bool SGL = false, bool STG = false, int LP = P>
+// BUF: buffer loads (FP32 / FP16 with double-stored links; launch_hop_args
+// routes lattices whose offsets do not fit to the plain-load instance).
+template <int P, int NL, int NB, bool NT, bool SGL = false, bool STG = false, int LP = P,
+          bool BUF = (P != P64 && !SGL)>
 static void hop_dispatch_red(const HopArgs &a, dim3 grid, dim3 block, int red,
                              bool hasx, hipStream_t st) {
   const size_t lds = STG ? stage_bytes(P, NB, (int)block.x) : 0;
 #define QOP_LAUNCH(RD, HX) \
-  hipLaunchKernelGGL((hop_kernel<P, NL, NB, RD, HX, NT, SGL, STG, LP>), grid, block, lds, st, a)
+  hipLaunchKernelGGL((hop_kernel<P, NL, NB, RD, HX, NT, SGL, STG, LP, BUF>), grid, block, lds, st, a)
   if (red == RED_NONE) { if (hasx) QOP_LAUNCH(RED_NONE, true); else QOP_LAUNCH(RED_NONE, false); }
   else if (red == RED_OUT2) { if (hasx) QOP_LAUNCH(RED_OUT2, true); else QOP_LAUNCH(RED_OUT2, false); }
   else if (red == RED_CG) { QOP_LAUNCH(RED_CG, true); }
@@ -593,6 +650,14 @@ static void hop_dispatch_nb(const HopArgs &a, int nb, dim3 grid, dim3 block,
       fatal(__FILE__, __LINE__, "hop: single-stored links exist in FP64 and FP32 only");
     }
     return;
+  }
+  if (!a.buf) {
+    // offsets beyond the buffer range: one RHS per thread, plain loads
+    if constexpr (P != P64) {
+      QOP_REQUIRE(nb == 1 && !a.stage && !a.nt, "hop: plain-load instance takes one RHS per thread");
+      hop_dispatch_red<P, NL, 1, false, false, false, LP, false>(a, grid, block, red, hasx, st);
+      return;
+    }
   }
   if (a.stage) {
     if constexpr (P != P64 && NL == 16) {
@@ -656,6 +721,23 @@ void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
   if (a.tile) {
     const int slab = a.g.Vh / a.g.L[0];
     QOP_REQUIRE(a.tile % block == 0 && slab % a.tile == 0, "hop: tile must divide the x0 slab");
+  }
+  // buffer loads (FP32 / FP16, double-stored links) where every byte offset
+  // from the RHS group's first vector / norm and from a direction's first
+  // link stays below 2^31; otherwise (or when a.plain asks for it) one RHS
+  // per thread with plain loads
+  {
+    const long lim = 1L << 31, Vh = a.g.Vh;
+    const long esz = prec == P16 ? 4 : 8;   // stored vector element
+    const long lesz = prec == P32 ? 8 : 4;  // stored link element (P32H: FP16 links)
+    a.buf = prec != P64 && !a.single && !a.plain && ((long)(nb - 1) * a.sstride + 3 * Vh) * esz < lim &&
+            ((long)(nb - 1) * a.nstride + Vh) * 4 < lim && 9 * Vh * lesz < lim;
+    if (prec != P64 && !a.single && !a.buf) {
+      nb = 1;
+      a.stage = 0;
+      a.nt = 0;
+      if (block > 256) block = 256;
+    }
   }
   int nblk = (a.g.Vh + block - 1) / block;
   a.ngroups = (a.n + nb - 1) / nb;

@@ -113,7 +113,7 @@ static double hop_time_impl(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, 
     shape[0] = hs.nb; shape[1] = hs.block; shape[2] = hs.remap; shape[3] = hs.nt; shape[4] = hs.rhs_inner;
     shape[5] = hs.tile;
     shape[6] = hs.stage;
-    shape[7] = 0;
+    shape[7] = hs.plain;
   }
   return ms / nrep;
 }
@@ -194,7 +194,7 @@ int QOPX_D3_asqtad_hop(QOP_D3_FermionLinksAsqtad *fla, int prec, int n, QOP_D3_C
 // precisions only).  Returns the mean ms per launch from
 // HIP events on the library stream; shape[8] receives (rhs per thread, block,
 // XCD remap, non-temporal links, RHS-innermost block order, x0 tile, LDS
-// staging, 0 (reserved)).
+// staging, plain loads).
 double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrhs, int nrep,
                                int red, int *shape) {
   double ms = -1.0;  // -1: the device memory is exhausted
@@ -203,14 +203,15 @@ double QOPX_D3_asqtad_hop_time(QOP_D3_FermionLinksAsqtad *fla, int prec, int nrh
 }
 
 // Force the multi-RHS hop launch shape (rhs per thread, block, XCD remap,
-// non-temporal links, RHS-innermost order, x0 tile, LDS staging, reserved = 0) for solvers built
-// from now on, wherever it is valid; null restores autotuning.
+// non-temporal links, RHS-innermost order, x0 tile, LDS staging, plain loads)
+// for solvers built from now on, wherever it is valid; null restores
+// autotuning.
 void QOPX_hop_shape_override(const int *shape) {
   if (!shape) {
     set_hop_shape_override(nullptr);
     return;
   }
-  HopShape s{shape[0], shape[1], shape[2], shape[3], shape[4], shape[5], shape[6]};
+  HopShape s{shape[0], shape[1], shape[2], shape[3], shape[4], shape[5], shape[6], shape[7]};
   set_hop_shape_override(&s);
 }
 

@@ -1,7 +1,8 @@
 """The batched solver's result does not depend on the multi-RHS hop launch
 shape (RHS per thread, block size, XCD remap, non-temporal loads and stores,
 block order, x0-tiled site order, LDS staging of the block's source rows or
-of its x0 planes in the plane-blocked order):
+of its x0 planes in the plane-blocked order, buffer loads or the plain-load
+kernel that lattices beyond the buffer range take):
 each forced shape gives bit-identical solutions and iteration counts, all at FP64 true residual 1e-10 (oracle operator).  The
 second lattice has x3 rows of 5 sites, so staged blocks end inside a row and
 gathers switch between the LDS tile and memory within a wavefront."""
@@ -17,7 +18,7 @@ pytestmark = pytest.mark.gpu
 EVEN = 0
 
 # (RHS per thread, block, XCD remap, non-temporal, RHS-innermost, x0 tile,
-#  LDS staging, reserved 0)
+#  LDS staging, plain loads)
 SHAPES = [
     (4, 128, 1, 0, 1, 0, 0, 0),
     (4, 128, 1, 0, 1, 512, 0, 0),
@@ -35,6 +36,8 @@ SHAPES = [
     (8, 256, 1, 0, 1, 0, 2, 0),
     (1, 512, 0, 0, 1, 0, 2, 0),
     (2, 256, 0, 0, 0, 1024, 1, 0),
+    (1, 256, 1, 0, 1, 0, 0, 1),
+    (1, 64, 0, 0, 0, 512, 0, 1),
 ]
 
 
