@@ -98,9 +98,14 @@ __device__ __forceinline__ void st3_q(void *data, float *norm, long h, long Vh,
     const float inv = m > 0.0f ? 1.0f / m : 0.0f;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const __half2 q = __floats2half2_rn(v[2 * c] * inv, v[2 * c + 1] * inv);
-      ((__half2 *)data)[c * Vh + h] = q;
-      const float2 f = __half22float2(q);
+      // each product and the packed half pair formed once (empty asm), so
+      // the words read back are the words stored
+      float a0 = v[2 * c] * inv, a1 = v[2 * c + 1] * inv;
+      asm volatile("" : "+v"(a0), "+v"(a1));
+      unsigned qw = __builtin_bit_cast(unsigned, __floats2half2_rn(a0, a1));
+      asm volatile("" : "+v"(qw));
+      ((unsigned *)data)[c * Vh + h] = qw;
+      const float2 f = __half22float2(__builtin_bit_cast(__half2, qw));
       v[2 * c] = f.x * m;
       v[2 * c + 1] = f.y * m;
     }

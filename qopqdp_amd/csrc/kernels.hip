@@ -195,12 +195,17 @@ __device__ __forceinline__ void vstore_q(void *data, float *norm, long h, long V
     const float inv = m > 0.0f ? 1.0f / m : 0.0f;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const __half2 q = __floats2half2_rn(v[2 * c] * inv, v[2 * c + 1] * inv);
-      if constexpr (NT)
-        __builtin_nontemporal_store(__builtin_bit_cast(unsigned, q), (unsigned *)data + c * Vh + h);
-      else
-        ((__half2 *)data)[c * Vh + h] = q;
-      const float2 f = __half22float2(q);
+      // each product and the packed half pair are formed once (empty asm),
+      // so the words read back are the words stored (the compiler may
+      // otherwise round v inv to FP16 in one step for one use and via FP32
+      // for another)
+      float a0 = v[2 * c] * inv, a1 = v[2 * c + 1] * inv;
+      asm volatile("" : "+v"(a0), "+v"(a1));
+      unsigned qw = __builtin_bit_cast(unsigned, __floats2half2_rn(a0, a1));
+      asm volatile("" : "+v"(qw));
+      if constexpr (NT) __builtin_nontemporal_store(qw, (unsigned *)data + c * Vh + h);
+      else ((unsigned *)data)[c * Vh + h] = qw;
+      const float2 f = __half22float2(__builtin_bit_cast(__half2, qw));
       v[2 * c] = f.x * m;
       v[2 * c + 1] = f.y * m;
     }
@@ -219,6 +224,47 @@ __device__ __forceinline__ void vstore_q(void *data, float *norm, long h, long V
     }
   } else {
     vstore<P>(data, norm, h, Vh, v);
+  }
+}
+
+// Store v as vstore_q does and return |stored v|^2 in FP64.  FP16: the site
+// norm m times the squared half words, m^2 sum(q^2), the sum over the 6 words
+// in FP32 and the rest in FP64 (the stored value is q m).
+template <int P, bool NT = false>
+__device__ __forceinline__ double vstore_norm2(void *data, float *norm, long h, long Vh,
+                                               typename PT<P>::R v[6]) {
+  if constexpr (P == P16) {
+    float m = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 6; i++) m = fmaxf(m, fabsf(v[i]));
+    const float inv = m > 0.0f ? 1.0f / m : 0.0f;
+    float q2 = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      // each product and the packed half pair are formed once (empty asm):
+      // the compiler may otherwise round v inv to FP16 directly (one
+      // rounding) for one use and via FP32 (two) for another, so the words
+      // squared would not always be the words stored
+      float a0 = v[2 * c] * inv, a1 = v[2 * c + 1] * inv;
+      asm volatile("" : "+v"(a0), "+v"(a1));
+      unsigned qw = __builtin_bit_cast(unsigned, __floats2half2_rn(a0, a1));
+      asm volatile("" : "+v"(qw));
+      if constexpr (NT) __builtin_nontemporal_store(qw, (unsigned *)data + c * Vh + h);
+      else ((unsigned *)data)[c * Vh + h] = qw;
+      float2 f = __half22float2(__builtin_bit_cast(__half2, qw));
+      asm volatile("" : "+v"(f.x), "+v"(f.y));
+      q2 = fmaf(f.x, f.x, q2);
+      q2 = fmaf(f.y, f.y, q2);
+    }
+    if constexpr (NT) __builtin_nontemporal_store(m, norm + h);
+    else norm[h] = m;
+    return (double)q2 * ((double)m * (double)m);
+  } else {
+    vstore_q<P, NT>(data, norm, h, Vh, v);
+    double s = 0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) s += (double)v[i] * (double)v[i];
+    return s;
   }
 }
 
@@ -327,11 +373,7 @@ __device__ __forceinline__ void hop_epilogue(const HopArgs &a, int h, int Vh, in
       vload<P>(dd, dn, h, Vh, rv);
 #pragma unroll
       for (int i = 0; i < 6; i++) rv[i] = fma(-alr, o[i], rv[i]);
-      vstore_q<P, NT>(dd, dn, h, Vh, rv);
-      double s = 0;
-#pragma unroll
-      for (int i = 0; i < 6; i++) s += (double)rv[i] * (double)rv[i];
-      red[b] = s;
+      red[b] = vstore_norm2<P, NT>(dd, dn, h, Vh, rv);
       float2 *xs = (float2 *)a.xs.data + (long)rb * a.dstride;
       const float alf = (float)a.cg_alpha[(long)rb * a.cg_stride];
 #pragma unroll
@@ -346,13 +388,12 @@ __device__ __forceinline__ void hop_epilogue(const HopArgs &a, int h, int Vh, in
       }
       continue;
     }
-    vstore_q<P, NT>(dd, dn, h, Vh, o);  // o becomes the stored value
     if constexpr (RED == RED_OUT2) {
-      double s = 0;
-#pragma unroll
-      for (int i = 0; i < 6; i++) s += (double)o[i] * (double)o[i];
-      red[b] = s;
-    } else if constexpr (RED == RED_X2) {
+      red[b] = vstore_norm2<P, NT>(dd, dn, h, Vh, o);
+      continue;
+    }
+    vstore_q<P, NT>(dd, dn, h, Vh, o);  // o becomes the stored value
+    if constexpr (RED == RED_X2) {
       double s = 0;
 #pragma unroll
       for (int i = 0; i < 6; i++) s += (double)xv[i] * (double)xv[i];

@@ -227,10 +227,19 @@ ms_update(const MsSlot *S, const int *active, int Vh, VecView rs, VecView ps, Ve
         float m = 0.f;
         for (int i = 0; i < 6; i++) m = fmaxf(m, fabsf(in[i]));
         float inv = m > 0.f ? 1.f / m : 0.f;
-        for (int c = 0; c < 3; c++)
-          ((__half2 *)((char *)v.data + o * cb))[c * Vh + h] = __floats2half2_rn(in[2 * c] * inv, in[2 * c + 1] * inv);
+        for (int c = 0; c < 3; c++) {
+          // products and packed halves formed once (empty asm), so the
+          // values kept are the words stored
+          float a0 = in[2 * c] * inv, a1 = in[2 * c + 1] * inv;
+          asm volatile("" : "+v"(a0), "+v"(a1));
+          unsigned qw = __builtin_bit_cast(unsigned, __floats2half2_rn(a0, a1));
+          asm volatile("" : "+v"(qw));
+          ((unsigned *)((char *)v.data + o * cb))[c * Vh + h] = qw;
+          const float2 f = __half22float2(__builtin_bit_cast(__half2, qw));
+          in[2 * c] = (R)f.x * m;
+          in[2 * c + 1] = (R)f.y * m;
+        }
         nm[h] = m;
-        for (int i = 0; i < 6; i++) in[i] = (R)__half2float(__float2half_rn(in[i] * inv)) * m;
       } else {
         for (int c = 0; c < 3; c++)
           ((float2 *)((char *)v.data + o * cb))[c * Vh + h] = make_float2(in[2 * c], in[2 * c + 1]);
