@@ -449,21 +449,34 @@ hop_kernel(const HopArgs a) {
   const int h0 = h - (int)threadIdx.x;  // mode 1: first site of the block
   const int wv = (int)(threadIdx.x >> 6), nwv = bs >> 6;  // mode 2: wave = plane
   const int hrow = h - (int)(threadIdx.x & 63);         // mode 2: first site of the wave
-  S *td = nullptr;     // [NB][3][bs]
-  float *tn = nullptr;  // FP16 norms [NB][bs]
+  // LDS tile: FP16 one 16-byte record per RHS and site, [NB][bs] x {3
+  // half2 words, norm} (one ds_read_b128 per gather, consecutive lanes 16 B
+  // apart); FP32 [NB][3][bs] float2
+  S *td = nullptr;
+  float *tn = nullptr;
+  uint4 *tr = nullptr;
   if constexpr (STG) {
     extern __shared__ __align__(16) unsigned char stg_mem[];
     td = (S *)stg_mem;
     tn = (float *)(td + NB * 3 * bs);
+    tr = (uint4 *)stg_mem;
     const int hs = h;
     if (any && hs < Vh) {
 #pragma unroll
       for (int b = 0; b < NB; b++) {
         if (!act[b]) continue;
         const S *sd = (const S *)a.src.data + (long)(b0 + b) * a.sstride;
+        if constexpr (P == P16) {
+          uint4 rec;
+          rec.x = __builtin_bit_cast(unsigned, sd[hs]);
+          rec.y = __builtin_bit_cast(unsigned, sd[(long)Vh + hs]);
+          rec.z = __builtin_bit_cast(unsigned, sd[2 * (long)Vh + hs]);
+          rec.w = __builtin_bit_cast(unsigned, a.src.norm[(long)(b0 + b) * a.nstride + hs]);
+          tr[b * bs + threadIdx.x] = rec;
+        } else {
 #pragma unroll
-        for (int c = 0; c < 3; c++) td[(b * 3 + c) * bs + threadIdx.x] = sd[(long)c * Vh + hs];
-        if constexpr (P == P16) tn[b * bs + threadIdx.x] = a.src.norm[(long)(b0 + b) * a.nstride + hs];
+          for (int c = 0; c < 3; c++) td[(b * 3 + c) * bs + threadIdx.x] = sd[(long)c * Vh + hs];
+        }
       }
     }
     __syncthreads();
@@ -532,9 +545,16 @@ hop_kernel(const HopArgs a) {
 #pragma unroll
             for (int b = 0; b < NB; b++) {
               if (!act[b]) continue;
+              if constexpr (P == P16) {
+                const uint4 rec = tr[b * bs + o];
+                gm[b][0] = __builtin_bit_cast(S, rec.x);
+                gm[b][1] = __builtin_bit_cast(S, rec.y);
+                gm[b][2] = __builtin_bit_cast(S, rec.z);
+                nm[b] = __builtin_bit_cast(float, rec.w);
+              } else {
 #pragma unroll
-              for (int c = 0; c < 3; c++) gm[b][c] = td[(b * 3 + c) * bs + o];
-              if constexpr (P == P16) nm[b] = tn[b * bs + o];
+                for (int c = 0; c < 3; c++) gm[b][c] = td[(b * 3 + c) * bs + o];
+              }
             }
           }
         }
