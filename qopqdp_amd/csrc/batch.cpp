@@ -206,9 +206,10 @@ BatchSolver::BatchSolver(Links &L, int eo, int capacity, int sloppy_prec, double
     sel_ = active_ + cap_;
     QOP_CHECK_HIP(hipHostMalloc(&active_h_, 2 * cap_ * sizeof(int), hipHostMallocDefault));
     sel_h_ = active_h_ + cap_;
-    dev_alloc(&ru_, 2 * cap_ * sizeof(int));
+    dev_alloc(&ru_, (3 * cap_ + 1) * sizeof(int));
     ru_cont_ = ru_ + cap_;
-    QOP_CHECK_HIP(hipMemsetAsync(ru_, 0, 2 * cap_ * sizeof(int), st_));
+    slots_ = ru_ + 2 * cap_;
+    QOP_CHECK_HIP(hipMemsetAsync(ru_, 0, (3 * cap_ + 1) * sizeof(int), st_));
     dev_alloc(&glob_, sizeof(BsGlobal));
     QOP_CHECK_HIP(hipHostMalloc(&glob_h_, sizeof(BsGlobal), hipHostMallocDefault));
     dev_alloc(&partA_, 3 * (size_t)cap_ * nch_ * sizeof(double));
@@ -318,10 +319,11 @@ void BatchSolver::apply_A64(const int *mask) {
   a.sstride = a.dstride = a.xstride = 3 * lat_->Vh;
   a.nstride = lat_->Vh;
   a.n = cap_;
-  a.active = mask;
-  // the tuned FP64 multi-RHS shape: slots that start together flag their
-  // updates in the same iteration and share the link reads of their group; a
-  // group with no flagged slot reads nothing
+  // the flagged slots, compacted into consecutive RHS of the launch, so k of
+  // them take ceil(k / RHS per thread) link reads wherever they sit in the
+  // batch; the groups past them read nothing
+  bs_launch_compact(mask, cap_, slots_, st);
+  a.slots = slots_;
   ru_shape_.apply(a);
   launch_hop_args(a, P64, ls.nl, ru_shape_.nb, ru_shape_.block, RED_NONE, false, st);
   HopArgs b = a;

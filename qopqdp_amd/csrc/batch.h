@@ -60,6 +60,8 @@ void bs_launch_ru(BsRhs *rhs, int *active, int *ru, int *ru_cont, double *lcoef,
 // A slot whose residual has dropped to join2 * r2_rel joins the reliable
 // update of this iteration when another slot is flagged for one.
 void bs_launch_ru_join(BsRhs *rhs, const int *active, int *ru, int n, hipStream_t st);
+// out[0] = number k of nonzero flags[0..n), out[1..k] their indices (device)
+void bs_launch_compact(const int *flags, int n, int *out, hipStream_t st);
 void bs_launch_commit(BsGlobal *glob, hipStream_t st);
 void bs_launch_pupdate(const BsRhs *rhs, const int *active, const int *status,
                        int prec, int Vh, VecView rs, VecView ps, long stride, int n,
@@ -160,6 +162,7 @@ class BatchSolver {
   int *active_ = nullptr, *active_h_ = nullptr;
   int *sel_ = nullptr, *sel_h_ = nullptr;
   int *ru_ = nullptr, *ru_cont_ = nullptr;  // device reliable-update masks
+  int *slots_ = nullptr;  // device [1 + cap]: compacted slot list of the masked FP64 hops
   BsGlobal *glob_ = nullptr, *glob_h_ = nullptr;
   double *partA_ = nullptr, *partB_ = nullptr, *partC_ = nullptr;
   double *coef_ = nullptr;      // [cap][2] hop coefficients (-1, m2)

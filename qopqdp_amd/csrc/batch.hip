@@ -420,6 +420,24 @@ void bs_launch_scalar2(BsRhs *rhs, const int *active, BsGlobal *glob, const doub
   hipLaunchKernelGGL(bs_scalar2, dim3(n), dim3(BS_THREADS), 0, st, rhs, active, glob, partC, nch, ru);
   QOP_CHECK_HIP(hipGetLastError());
 }
+// Compacted list of the flagged slots: out[0] = their number k, out[1..k]
+// their indices in increasing order (one wavefront, ballot and prefix count).
+__global__ void __launch_bounds__(64) bs_compact(const int *flags, int n, int *out) {
+  const int lane = (int)threadIdx.x;
+  int base = 0;
+  for (int i0 = 0; i0 < n; i0 += 64) {
+    const int i = i0 + lane;
+    const bool f = i < n && flags[i] != 0;
+    const unsigned long long m = __ballot(f);
+    if (f) out[1 + base + __popcll(m & ((1ull << lane) - 1ull))] = i;
+    base += __popcll(m);
+  }
+  if (lane == 0) out[0] = base;
+}
+void bs_launch_compact(const int *flags, int n, int *out, hipStream_t st) {
+  hipLaunchKernelGGL(bs_compact, dim3(1), dim3(64), 0, st, flags, n, out);
+  QOP_CHECK_HIP(hipGetLastError());
+}
 void bs_launch_ru_join(BsRhs *rhs, const int *active, int *ru, int n, hipStream_t st) {
   hipLaunchKernelGGL(bs_ru_join, dim3(1), dim3(64), 0, st, rhs, active, ru, n);
   QOP_CHECK_HIP(hipGetLastError());

@@ -25,6 +25,9 @@ struct HopArgs {
   double alpha = 1.0, beta = 0.0;
   double *partial = nullptr;     // [n][nchunk]
   const int *active = nullptr;   // device [n] or null
+  const int *slots = nullptr;    // device [1 + n] or null: count k, then the slots that launch
+                                 // RHS 0..k-1 work on (FP64 masked hops: k flagged slots take
+                                 // ceil(k / RHS per thread) link reads)
   const int *status = nullptr;   // device flag: kernel is a no-op when nonzero
   int remap = 1;
   int nt = 0;                    // non-temporal link loads

@@ -341,14 +341,14 @@ __device__ __forceinline__ int site_of_block(const Geom &g, int sb, int tile) {
 // Output of one site for NB RHS: out = beta x + alpha acc (stored), or the
 // fused CG update; per-RHS reduction terms in red[].
 template <int P, int NB, int RED, bool HASX, bool NT>
-__device__ __forceinline__ void hop_epilogue(const HopArgs &a, int h, int Vh, int b0,
+__device__ __forceinline__ void hop_epilogue(const HopArgs &a, int h, int Vh, const int sl[NB],
                                              const bool act[NB], typename PT<P>::R acc[NB][6],
                                              double red[NB]) {
   using R = typename PT<P>::R;
 #pragma unroll
   for (int b = 0; b < NB; b++) {
     if (!act[b]) continue;
-    const int rb = b0 + b;
+    const int rb = sl[b];
     double al = a.alpha, be = a.beta;
     if (a.coef) { al = a.coef[2 * rb]; be = a.coef[2 * rb + 1]; }
     R o[6];
@@ -438,10 +438,20 @@ hop_kernel(const HopArgs a) {
   for (int b = 0; b < NB; b++) red[b] = 0.0;
 
   bool act[NB];
+  int sl[NB];  // slot of each RHS of this thread
   bool any = false;
 #pragma unroll
   for (int b = 0; b < NB; b++) {
-    act[b] = (b0 + b < a.n) && (!a.active || a.active[b0 + b]);
+    // the launch's RHS j = b0 + b is slot j, or with a compacted list
+    // (a.slots: count, then the slots) the j-th listed slot
+    const int j = b0 + b;
+    if (a.slots) {
+      act[b] = j < a.slots[0];
+      sl[b] = act[b] ? a.slots[1 + j] : j;
+    } else {
+      act[b] = (j < a.n) && (!a.active || a.active[j]);
+      sl[b] = j;
+    }
     any |= act[b];
   }
   using S = typename PT<P>::S;
@@ -570,10 +580,10 @@ hop_kernel(const HopArgs a) {
               if constexpr (P == P16)
                 nm[b] = buf_load<float, false>(rs_nrm, (unsigned)nb * 4u, (unsigned)((long)b * a.nstride * 4));
             } else {
-              const S *sd = (const S *)a.src.data + (long)(b0 + b) * a.sstride;
+              const S *sd = (const S *)a.src.data + (long)sl[b] * a.sstride;
 #pragma unroll
               for (int c = 0; c < 3; c++) gm[b][c] = sd[(long)c * Vh + nb];
-              if constexpr (P == P16) nm[b] = a.src.norm[(long)(b0 + b) * a.nstride + nb];
+              if constexpr (P == P16) nm[b] = a.src.norm[(long)sl[b] * a.nstride + nb];
             }
           }
         }
@@ -661,7 +671,7 @@ hop_kernel(const HopArgs a) {
         }
       }
     }
-    hop_epilogue<P, NB, RED, HASX, NT>(a, h, Vh, b0, act, acc, red);
+    hop_epilogue<P, NB, RED, HASX, NT>(a, h, Vh, sl, act, acc, red);
   }
   if constexpr (RED != RED_NONE) {
     const long nchunk = (Vh + 63) / 64;
@@ -766,6 +776,8 @@ void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
   QOP_REQUIRE(red == RED_NONE || hasx || red == RED_OUT2, "hop: RED_X2 / RED_CG need x");
   QOP_REQUIRE(red != RED_CG || (a.cg_alpha && a.xs.data && prec != P64), "hop: RED_CG needs alpha and xs");
   QOP_REQUIRE(a.n >= 1, "hop: n >= 1");
+  QOP_REQUIRE(!a.slots || (prec == P64 && !a.single && red == RED_NONE),
+              "hop: a compacted slot list is for FP64 double-stored hops without reductions");
   if (a.stage) {
     QOP_REQUIRE(prec != P64 && nl == 16 && nb <= 8 && !a.single && !a.nt,
                 "hop: LDS staging needs FP32/FP16 Naik links, at most 8 RHS per thread and plain "
