@@ -59,6 +59,9 @@ def main():
     ap.add_argument("--mass", type=float, default=0.01)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--fp64-solves", type=int, default=2)
+    ap.add_argument("--delta", type=float, default=0.1,
+                    help="qmx_delta of the batched solves: a reliable update when the iterated "
+                         "residual has dropped by this factor")
     ap.add_argument("--ru-join", type=float, default=0.3,
                     help="qmx_ru_join of the batched solves: a slot whose residual dropped by this "
                          "factor joins another slot's reliable update (0: off, bitwise "
@@ -184,7 +187,7 @@ def main():
     vins = [s.vec(x) for x in srcs]
     vouts = [s.zeros() for _ in range(B)]
     P = C.c_void_p
-    assert s.set_opts(qmx_ru_join=args.ru_join) == 0
+    assert s.set_opts(qmx_ru_join=args.ru_join, qmx_delta=args.delta) == 0
 
     def batch_step():
         ras = [q.resid_arg_default() for _ in range(B)]
@@ -301,7 +304,7 @@ def main():
                         f"{B} Gaussian sources per step, FP64 true residual {args.tol}",
             "lattice": L, "batch": B, "mass": args.mass, "tol": args.tol,
             "inner_prec": args.prec, "gauge": "reunit(exp(0.3iH)), MILC phases, asqtad smearing",
-            "reliable_update_delta": 0.1, "ru_join": args.ru_join,
+            "reliable_update_delta": args.delta, "ru_join": args.ru_join,
         },
         "p50_latency_ms": round(float(np.percentile(lat_ms, 50)), 1),
         "p99_latency_ms": round(float(np.percentile(lat_ms, 99)), 1),
