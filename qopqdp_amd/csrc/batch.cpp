@@ -305,7 +305,7 @@ static void batch_lincomb(Lattice *lat, int cap, const DevBatch &out, int nin, c
   launch_lincomb(a, out.prec, pin, nin, partial != nullptr, st);
 }
 
-void BatchSolver::apply_A64(const int *mask) {
+void BatchSolver::apply_A64(const int *mask, bool listed) {
   hipStream_t st = st_;
   LinkSet &ls = L_.get(P64);
   const int opp = 1 - eo_;
@@ -322,7 +322,7 @@ void BatchSolver::apply_A64(const int *mask) {
   // the flagged slots, compacted into consecutive RHS of the launch, so k of
   // them take ceil(k / RHS per thread) link reads wherever they sit in the
   // batch; the groups past them read nothing
-  bs_launch_compact(mask, cap_, slots_, st);
+  if (!listed) bs_launch_compact(mask, cap_, slots_, st);
   a.slots = slots_;
   ru_shape_.apply(a);
   launch_hop_args(a, P64, ls.nl, ru_shape_.nb, ru_shape_.block, RED_NONE, false, st);
@@ -491,16 +491,18 @@ void BatchSolver::enqueue_iteration() {
   if (join_ > 0) bs_launch_ru_join(rhs_, active_, ru_, cap_, st);
   bs_launch_pupdate(rhs_, active_, &glob_->status, sp_, (int)Vh, rs_.slot(0), ps_.slot(0), 3 * Vh, cap_, 1,
                     partB_, st);
-  // reliable update of the flagged slots, all on the device
+  // reliable update of the flagged slots, all on the device, over their
+  // compacted list
   const long stride = 3 * Vh;
-  bs_launch_ru(rhs_, active_, ru_, ru_cont_, lcoef_, sp_, (int)Vh, x64_.slot(0), xs_.slot(0), b64_.slot(0),
+  bs_launch_compact(ru_, cap_, slots_, st);
+  bs_launch_ru(rhs_, active_, ru_, ru_cont_, slots_, lcoef_, sp_, (int)Vh, x64_.slot(0), xs_.slot(0), b64_.slot(0),
                ax64_.slot(0), rs_.slot(0), ps_.slot(0), stride, partA_, partB_, partC_, nch_, cap_, 0, partB_, st);
-  apply_A64(ru_);
-  bs_launch_ru(rhs_, active_, ru_, ru_cont_, lcoef_, sp_, (int)Vh, x64_.slot(0), xs_.slot(0), b64_.slot(0),
+  apply_A64(ru_, true);
+  bs_launch_ru(rhs_, active_, ru_, ru_cont_, slots_, lcoef_, sp_, (int)Vh, x64_.slot(0), xs_.slot(0), b64_.slot(0),
                ax64_.slot(0), rs_.slot(0), ps_.slot(0), stride, partA_, partB_, partC_, nch_, cap_, 1, partB_, st);
-  bs_launch_ru(rhs_, active_, ru_, ru_cont_, lcoef_, sp_, (int)Vh, x64_.slot(0), xs_.slot(0), b64_.slot(0),
+  bs_launch_ru(rhs_, active_, ru_, ru_cont_, slots_, lcoef_, sp_, (int)Vh, x64_.slot(0), xs_.slot(0), b64_.slot(0),
                ax64_.slot(0), rs_.slot(0), ps_.slot(0), stride, partA_, partB_, partC_, nch_, cap_, 2, partB_, st);
-  bs_launch_ru(rhs_, active_, ru_, ru_cont_, lcoef_, sp_, (int)Vh, x64_.slot(0), xs_.slot(0), b64_.slot(0),
+  bs_launch_ru(rhs_, active_, ru_, ru_cont_, slots_, lcoef_, sp_, (int)Vh, x64_.slot(0), xs_.slot(0), b64_.slot(0),
                ax64_.slot(0), rs_.slot(0), ps_.slot(0), stride, partA_, partB_, partC_, nch_, cap_, 3, partB_, st);
 }
 

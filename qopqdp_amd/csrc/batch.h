@@ -53,8 +53,10 @@ void bs_launch_scalar2(BsRhs *rhs, const int *active, BsGlobal *glob,
                        const double *partC, int nch, int n, int *ru, hipStream_t st);
 // Device reliable update, stage 0: x += xs; 1: residual, |r|^2 and <rs,ps>;
 // 2: per-slot decisions; 3: search-direction fix.
-void bs_launch_ru(BsRhs *rhs, int *active, int *ru, int *ru_cont, double *lcoef, int prec,
-                  int Vh, VecView x64, VecView xs, VecView b64, VecView ax64, VecView rs,
+// list: compacted flagged slots (bs_launch_compact of ru), read by the
+// vector steps 0, 1 and 3
+void bs_launch_ru(BsRhs *rhs, int *active, int *ru, int *ru_cont, const int *list, double *lcoef,
+                  int prec, int Vh, VecView x64, VecView xs, VecView b64, VecView ax64, VecView rs,
                   VecView ps, long stride, double *pr2, double *pre, double *pim, int nch,
                   int n, int stage, double *ppB, hipStream_t st);
 // A slot whose residual has dropped to join2 * r2_rel joins the reliable
@@ -143,7 +145,9 @@ class BatchSolver {
  private:
   void enqueue_iteration();
   void build_graph();
-  void apply_A64(const int *mask);  // ax64 = A x64 on the slots set in mask
+  // ax64 = A x64 on the slots set in mask (listed: slots_ already holds
+  // their compacted list)
+  void apply_A64(const int *mask, bool listed = false);
 
   Links &L_;
   Lattice *lat_;

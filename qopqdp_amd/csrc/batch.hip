@@ -236,35 +236,41 @@ bs_pupdate(const BsRhs *rhs, const int *active, const int *status, int Vh, VecVi
 //   stop / continue, beta                   (bs_ru_scalar)
 //   ps = (1 - beta <rs,ps>/|r|^2) rs + beta ps  (bs_ru_pfix)
 // Unflagged slots skip every step, so a slot's arithmetic is the same
-// whatever the other slots do.
+// whatever the other slots do.  The vector steps run one block per 256 sites
+// over the compacted list of the flagged slots (list[0] = their number, then
+// the slots; bs_compact), so an iteration with no update costs one short
+// launch per step.
 __global__ void __launch_bounds__(256)
-bs_ru_x(const int *ru, int Vh, VecView x64, VecView xs, long stride) {
-  const int b = blockIdx.y;
-  if (!ru[b]) return;
+bs_ru_x(const int *list, int Vh, VecView x64, VecView xs, long stride) {
+  const int k = list[0];
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= Vh) return;
-  const size_t off = (size_t)b * stride;
-  double2 *x = (double2 *)x64.data + off;
-  float2 *y = (float2 *)xs.data + off;
+  for (int j = 0; j < k; j++) {
+    const int b = list[1 + j];
+    const size_t off = (size_t)b * stride;
+    double2 *x = (double2 *)x64.data + off;
+    float2 *y = (float2 *)xs.data + off;
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    double2 a = x[c * Vh + h];
-    float2 d = y[c * Vh + h];
-    a.x += (double)d.x;
-    a.y += (double)d.y;
-    x[c * Vh + h] = a;
-    y[c * Vh + h] = make_float2(0.f, 0.f);
+    for (int c = 0; c < 3; c++) {
+      double2 a = x[c * Vh + h];
+      float2 d = y[c * Vh + h];
+      a.x += (double)d.x;
+      a.y += (double)d.y;
+      x[c * Vh + h] = a;
+      y[c * Vh + h] = make_float2(0.f, 0.f);
+    }
   }
 }
 
 template <int P>
 __global__ void __launch_bounds__(256)
-bs_ru_resid(const int *ru, int Vh, VecView b64, VecView ax64, VecView rs, VecView ps, long stride,
+bs_ru_resid(const int *list, int Vh, VecView b64, VecView ax64, VecView rs, VecView ps, long stride,
             double *part_r2, double *part_re, double *part_im) {
   using R = typename BT<P>::R;
-  const int b = blockIdx.y;
-  if (!ru[b]) return;
+  const int k = list[0];
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int j = 0; j < k; j++) {
+  const int b = list[1 + j];
   double r2 = 0, dre = 0, dim = 0;
   if (h < Vh) {
     const size_t off = (size_t)b * stride;
@@ -302,6 +308,7 @@ bs_ru_resid(const int *ru, int Vh, VecView b64, VecView ax64, VecView rs, VecVie
     part_re[(long)b * nch + (h >> 6)] = dre;
     part_im[(long)b * nch + (h >> 6)] = dim;
   }
+  }
 }
 
 __global__ void __launch_bounds__(1024)
@@ -335,12 +342,14 @@ bs_ru_scalar(BsRhs *rhs, int *active, int *ru, int *ru_cont, double *lcoef,
 
 template <int P>
 __global__ void __launch_bounds__(256)
-bs_ru_pfix(const int *ru_cont, const double *lcoef, int Vh, VecView rs, VecView ps, long stride,
-           double *partB) {
+bs_ru_pfix(const int *list, const int *ru_cont, const double *lcoef, int Vh, VecView rs, VecView ps,
+           long stride, double *partB) {
   using R = typename BT<P>::R;
-  const int b = blockIdx.y;
-  if (!ru_cont[b]) return;
+  const int k = list[0];
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int j = 0; j < k; j++) {
+  const int b = list[1 + j];
+  if (!ru_cont[b]) continue;  // the slot stopped at this update
   double acc = 0;
   if (h < Vh) {
   const size_t off = (size_t)b * stride;
@@ -366,6 +375,7 @@ bs_ru_pfix(const int *ru_cont, const double *lcoef, int Vh, VecView rs, VecView 
   const int nch = (Vh + 63) / 64;
   acc = bsum(acc);
   if ((threadIdx.x & 63) == 0 && (h >> 6) < nch) partB[(long)b * nch + (h >> 6)] = acc;
+  }
 }
 
 // One thread: if any slot is flagged for a reliable update in this
@@ -463,24 +473,24 @@ void bs_launch_sum(const double *part, int nch, int n, const int *sel, double *o
 }  // namespace qop
 
 namespace qop {
-void bs_launch_ru(BsRhs *rhs, int *active, int *ru, int *ru_cont, double *lcoef, int prec, int Vh,
-                  VecView x64, VecView xs, VecView b64, VecView ax64, VecView rs, VecView ps,
-                  long stride, double *pr2, double *pre, double *pim, int nch, int n, int stage,
-                  double *ppB, hipStream_t st) {
-  dim3 g((Vh + 255) / 256, n), bl(256);
+void bs_launch_ru(BsRhs *rhs, int *active, int *ru, int *ru_cont, const int *list, double *lcoef,
+                  int prec, int Vh, VecView x64, VecView xs, VecView b64, VecView ax64, VecView rs,
+                  VecView ps, long stride, double *pr2, double *pre, double *pim, int nch, int n,
+                  int stage, double *ppB, hipStream_t st) {
+  dim3 g((Vh + 255) / 256), bl(256);
   if (stage == 0) {
-    hipLaunchKernelGGL(bs_ru_x, g, bl, 0, st, (const int *)ru, Vh, x64, xs, stride);
+    hipLaunchKernelGGL(bs_ru_x, g, bl, 0, st, list, Vh, x64, xs, stride);
   } else if (stage == 1) {
-    if (prec == P32) hipLaunchKernelGGL(bs_ru_resid<P32>, g, bl, 0, st, (const int *)ru, Vh, b64, ax64, rs, ps, stride, pr2, pre, pim);
-    else if (prec == P16) hipLaunchKernelGGL(bs_ru_resid<P16>, g, bl, 0, st, (const int *)ru, Vh, b64, ax64, rs, ps, stride, pr2, pre, pim);
-    else hipLaunchKernelGGL(bs_ru_resid<P64>, g, bl, 0, st, (const int *)ru, Vh, b64, ax64, rs, ps, stride, pr2, pre, pim);
+    if (prec == P32) hipLaunchKernelGGL(bs_ru_resid<P32>, g, bl, 0, st, list, Vh, b64, ax64, rs, ps, stride, pr2, pre, pim);
+    else if (prec == P16) hipLaunchKernelGGL(bs_ru_resid<P16>, g, bl, 0, st, list, Vh, b64, ax64, rs, ps, stride, pr2, pre, pim);
+    else hipLaunchKernelGGL(bs_ru_resid<P64>, g, bl, 0, st, list, Vh, b64, ax64, rs, ps, stride, pr2, pre, pim);
   } else if (stage == 2) {
     hipLaunchKernelGGL(bs_ru_scalar, dim3(n), dim3(BS_THREADS), 0, st, rhs, active, ru, ru_cont, lcoef,
                        (const double *)pr2, (const double *)pre, (const double *)pim, nch);
   } else {
-    if (prec == P32) hipLaunchKernelGGL(bs_ru_pfix<P32>, g, bl, 0, st, (const int *)ru_cont, (const double *)lcoef, Vh, rs, ps, stride, ppB);
-    else if (prec == P16) hipLaunchKernelGGL(bs_ru_pfix<P16>, g, bl, 0, st, (const int *)ru_cont, (const double *)lcoef, Vh, rs, ps, stride, ppB);
-    else hipLaunchKernelGGL(bs_ru_pfix<P64>, g, bl, 0, st, (const int *)ru_cont, (const double *)lcoef, Vh, rs, ps, stride, ppB);
+    if (prec == P32) hipLaunchKernelGGL(bs_ru_pfix<P32>, g, bl, 0, st, list, (const int *)ru_cont, (const double *)lcoef, Vh, rs, ps, stride, ppB);
+    else if (prec == P16) hipLaunchKernelGGL(bs_ru_pfix<P16>, g, bl, 0, st, list, (const int *)ru_cont, (const double *)lcoef, Vh, rs, ps, stride, ppB);
+    else hipLaunchKernelGGL(bs_ru_pfix<P64>, g, bl, 0, st, list, (const int *)ru_cont, (const double *)lcoef, Vh, rs, ps, stride, ppB);
   }
   QOP_CHECK_HIP(hipGetLastError());
 }

@@ -415,10 +415,10 @@ __device__ __forceinline__ void hop_epilogue(const HopArgs &a, int h, int Vh, co
 // wave's 64 positions come from LDS.  Staged blocks take up to 512 threads.
 // LP: precision of the stored links (P, or P16 for FP32 vectors with FP16
 // links and per-link scales).
-template <int P, int NL, int NB, int RED, bool HASX, bool NT, bool SGL, bool STG, int LP = P,
-          bool BUF = false>
-__global__ void __launch_bounds__(STG ? 512 : 256)
-hop_kernel(const HopArgs a) {
+// One RHS group of the hop for one site block: the group starting at launch
+// RHS b0_in and site block sb_in, or (negative) the ones the grid gives.
+template <int P, int NL, int NB, int RED, bool HASX, bool NT, bool SGL, bool STG, int LP, bool BUF>
+__device__ __forceinline__ void hop_group(const HopArgs &a, int b0_in, int sb_in) {
   static_assert(!BUF || (P != P64 && !SGL), "buffer loads: FP32 / FP16, double-stored links");
   using R = typename PT<P>::R;
   const Geom g = a.g;
@@ -427,10 +427,10 @@ hop_kernel(const HopArgs a) {
   // rhs_inner: the RHS groups of one site block are consecutive logical
   // blocks, so they run together on one XCD and share its L2 (links and the
   // neighbour window); otherwise the groups are the grid's y dimension.
-  const int sb = a.rhs_inner ? lb / a.ngroups : lb;
+  const int sb = sb_in >= 0 ? sb_in : a.rhs_inner ? lb / a.ngroups : lb;
   const bool planes = STG && a.stage == 2;
   const int h = planes ? site_of_plane_block(g, sb) : site_of_block(g, sb, a.tile);
-  const int b0 = (a.rhs_inner ? lb % a.ngroups : (int)blockIdx.y) * NB;
+  const int b0 = b0_in >= 0 ? b0_in : (a.rhs_inner ? lb % a.ngroups : (int)blockIdx.y) * NB;
   if (a.status && *a.status != 0) return;
 
   double red[NB];
@@ -685,6 +685,25 @@ hop_kernel(const HopArgs a) {
   }
 }
 
+// GL (FP64 launches with a compacted slot list): the RHS group is the
+// innermost logical block index (XCD-aware order), so the groups of one site
+// block run side by side and share its links in L2, and the blocks of groups
+// past the listed slots return before any load: an empty list costs a
+// launch of idle blocks.  Otherwise one group and site block per block.
+template <int P, int NL, int NB, int RED, bool HASX, bool NT, bool SGL, bool STG, int LP = P,
+          bool BUF = false, bool GL = false>
+__global__ void __launch_bounds__(STG ? 512 : 256)
+hop_kernel(const HopArgs a) {
+  if constexpr (GL) {
+    const int lb = logical_block(a.remap);
+    const int b0 = (lb % a.ngroups) * NB, sb = lb / a.ngroups;
+    if (b0 >= a.slots[0] || sb * (int)blockDim.x >= a.g.Vh) return;
+    hop_group<P, NL, NB, RED, HASX, NT, SGL, STG, LP, BUF>(a, b0, sb);
+  } else {
+    hop_group<P, NL, NB, RED, HASX, NT, SGL, STG, LP, BUF>(a, -1, -1);
+  }
+}
+
 int hop_partial_blocks(const Lattice *lat) { return (int)((lat->Vh + 63) / 64); }
 
 // LDS bytes of a staged block: NB RHS x blockDim sites x (3 stored complex
@@ -697,10 +716,17 @@ static size_t stage_bytes(int prec, int nb, int block) {
 // BUF: buffer loads (FP32 / FP16 with double-stored links; launch_hop_args
 // routes lattices whose offsets do not fit to the plain-load instance).
 template <int P, int NL, int NB, bool NT, bool SGL = false, bool STG = false, int LP = P,
-          bool BUF = (P != P64 && !SGL)>
+          bool BUF = (P != P64 && !SGL), bool GL = false>
 static void hop_dispatch_red(const HopArgs &a, dim3 grid, dim3 block, int red,
                              bool hasx, hipStream_t st) {
   const size_t lds = STG ? stage_bytes(P, NB, (int)block.x) : 0;
+  if constexpr (GL) {
+    // compacted slot lists: FP64, no reduction
+    if (hasx) hipLaunchKernelGGL((hop_kernel<P, NL, NB, RED_NONE, true, NT, SGL, STG, LP, BUF, true>), grid, block, lds, st, a);
+    else hipLaunchKernelGGL((hop_kernel<P, NL, NB, RED_NONE, false, NT, SGL, STG, LP, BUF, true>), grid, block, lds, st, a);
+    (void)red;
+    return;
+  }
 #define QOP_LAUNCH(RD, HX) \
   hipLaunchKernelGGL((hop_kernel<P, NL, NB, RD, HX, NT, SGL, STG, LP, BUF>), grid, block, lds, st, a)
   if (red == RED_NONE) { if (hasx) QOP_LAUNCH(RED_NONE, true); else QOP_LAUNCH(RED_NONE, false); }
@@ -721,6 +747,17 @@ static void hop_dispatch_nb(const HopArgs &a, int nb, dim3 grid, dim3 block,
       fatal(__FILE__, __LINE__, "hop: single-stored links exist in FP64 and FP32 only");
     }
     return;
+  }
+  if (a.slots) {
+    if constexpr (P == P64) {
+      switch (nb) {
+        case 1: hop_dispatch_red<P, NL, 1, false, false, false, LP, false, true>(a, grid, block, red, hasx, st); break;
+        case 2: hop_dispatch_red<P, NL, 2, false, false, false, LP, false, true>(a, grid, block, red, hasx, st); break;
+        case 4: hop_dispatch_red<P, NL, 4, false, false, false, LP, false, true>(a, grid, block, red, hasx, st); break;
+        default: hop_dispatch_red<P, NL, 8, false, false, false, LP, false, true>(a, grid, block, red, hasx, st); break;
+      }
+      return;
+    }
   }
   if (!a.buf) {
     // offsets beyond the buffer range: one RHS per thread, plain loads
@@ -776,8 +813,9 @@ void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
   QOP_REQUIRE(red == RED_NONE || hasx || red == RED_OUT2, "hop: RED_X2 / RED_CG need x");
   QOP_REQUIRE(red != RED_CG || (a.cg_alpha && a.xs.data && prec != P64), "hop: RED_CG needs alpha and xs");
   QOP_REQUIRE(a.n >= 1, "hop: n >= 1");
-  QOP_REQUIRE(!a.slots || (prec == P64 && !a.single && red == RED_NONE),
-              "hop: a compacted slot list is for FP64 double-stored hops without reductions");
+  QOP_REQUIRE(!a.slots || (prec == P64 && !a.single && red == RED_NONE && nb <= 8 && !a.stage),
+              "hop: a compacted slot list is for FP64 double-stored hops without reductions, at "
+              "most 8 RHS per thread");
   if (a.stage) {
     QOP_REQUIRE(prec != P64 && nl == 16 && nb <= 8 && !a.single && !a.nt,
                 "hop: LDS staging needs FP32/FP16 Naik links, at most 8 RHS per thread and plain "
@@ -814,6 +852,7 @@ void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
   }
   int nblk = (a.g.Vh + block - 1) / block;
   a.ngroups = (a.n + nb - 1) / nb;
+  if (a.slots) a.rhs_inner = 1;  // the GL kernel orders the groups innermost itself
   dim3 grid, blk(block);
   if (a.rhs_inner) {
     int tot = nblk * a.ngroups;
