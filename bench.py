@@ -446,6 +446,9 @@ def multi_bench(args):
     for v in vouts + [tmp, Ax]:
         s.free(v)
     per_shift = [max(tr[i * nm + j] for i in range(nsrc)) for j in range(nm)]
+    # iterations of the reliable-update refinement after the multi-shift stage
+    # (0: the multi-shift solution already met the FP64 tolerance)
+    refine_it = [max(ras[i * nm + j].final_iter for i in range(nsrc)) for j in range(nm)]
     # FP64 reference-algorithm multi-mass solves, one source at a time
     nfp = min(args.fp64_sources, nsrc)
     fp_t, fp_ok = [], 0
@@ -492,6 +495,7 @@ def multi_bench(args):
                    "ru_join": args.ru_join},
         "links": prep, "seconds": round(dt, 3), "converged": ok, "solves": n,
         "multi_shift_iters": ms_it, "max_fp64_true_resid_per_mass": per_shift,
+        "max_refine_iters_per_mass": refine_it,
         "fp64_sequential": {"solves_per_s": round(fp_rate, 3), "sources_timed": len(fp_t),
                             "seconds_per_source": round(float(np.mean(fp_t)), 3),
                             "converged": int(fp_ok)},
