@@ -13,7 +13,8 @@
 //                  already multiplied by 1/2 (lib/asqtad_dslash_p.c:98-104,
 //                  123-127, 201-206).  Without long links nl = 8 and
 //                  k = 2*mu + {0: F(x), 1: -F(x-mu)^+}.  FP16 links add a
-//                  float scale per link [p][k][h].
+//                  float scale per link [p][k][h] and store each element
+//                  as the half pair (re, -im) / scale.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>

@@ -93,8 +93,9 @@ __device__ __forceinline__ void link_words_cvt(const LinkWords<LP> &w, R out[18]
 #pragma unroll
   for (int e = 0; e < 9; e++) {
     if constexpr (LP == P16) {
+      // stored as (re, -im) / scale
       float2 f = __half22float2(__builtin_bit_cast(__half2, w.e[e]));
-      out[2 * e] = f.x * w.s; out[2 * e + 1] = f.y * w.s;
+      out[2 * e] = f.x * w.s; out[2 * e + 1] = -(f.y * w.s);
     } else {
       out[2 * e] = w.e[e].x; out[2 * e + 1] = w.e[e].y;
     }
@@ -556,11 +557,16 @@ __device__ __forceinline__ void hop_group(const HopArgs &a, int b0_in, int sb_in
             for (int b = 0; b < NB; b++) {
               if (!act[b]) continue;
               if constexpr (P == P16) {
+                // one 16-byte read per record: the words pass through empty
+                // asm as 32-bit integers, so the compiler does not split the
+                // read by the types they are used as
                 const uint4 rec = tr[b * bs + o];
-                gm[b][0] = __builtin_bit_cast(S, rec.x);
-                gm[b][1] = __builtin_bit_cast(S, rec.y);
-                gm[b][2] = __builtin_bit_cast(S, rec.z);
-                nm[b] = __builtin_bit_cast(float, rec.w);
+                unsigned w0 = rec.x, w1 = rec.y, w2 = rec.z, w3 = rec.w;
+                asm volatile("" : "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3));
+                gm[b][0] = __builtin_bit_cast(S, w0);
+                gm[b][1] = __builtin_bit_cast(S, w1);
+                gm[b][2] = __builtin_bit_cast(S, w2);
+                nm[b] = __builtin_bit_cast(float, w3);
               } else {
 #pragma unroll
                 for (int c = 0; c < 3; c++) gm[b][c] = td[(b * 3 + c) * bs + o];
@@ -626,8 +632,12 @@ __device__ __forceinline__ void hop_group(const HopArgs &a, int b0_in, int sb_in
           if (!act[b]) continue;
           if constexpr (P == P16) {
             // FP16 links and vectors: the products of the stored half-precision
-            // words, exact in FP32, summed per colour row with mixed-precision
-            // FMAs (no conversions), then scaled once by link scale x site norm
+            // words (no conversions), summed per colour row in FP32, then
+            // scaled once by link scale x site norm.  The link word holds
+            // (wr, -wi), so the real part wr vr - wi vi is one two-term dot
+            // product (v_dot2c_f32_f16; FP16 subnormal inputs count as zero,
+            // below the format's rounding at the link and site scales), the
+            // imaginary part two mixed-precision FMAs with exact products.
             const float sn = lw.s * nm[b];
             float t[6];
 #pragma unroll
@@ -635,13 +645,14 @@ __device__ __forceinline__ void hop_group(const HopArgs &a, int b0_in, int sb_in
               float re = 0.0f, im = 0.0f;
 #pragma unroll
               for (int c = 0; c < 3; c++) {
-                const __half2 wh = __builtin_bit_cast(__half2, lw.e[3 * r + c]);
-                const float wr = __low2float(wh), wi = __high2float(wh);
-                const float vr = __low2float(gm[b][c]), vi = __high2float(gm[b][c]);
-                re = fmaf(wr, vr, re);
-                re = fmaf(-wi, vi, re);
+                using h2 = _Float16 __attribute__((ext_vector_type(2)));
+                const h2 wd = __builtin_bit_cast(h2, lw.e[3 * r + c]);
+                const h2 vd = __builtin_bit_cast(h2, gm[b][c]);
+                re = __builtin_amdgcn_fdot2(wd, vd, re, false);
+                const float wr = (float)wd.x, wni = (float)wd.y;  // wni = -wi
+                const float vr = (float)vd.x, vi = (float)vd.y;
                 im = fmaf(wr, vi, im);
-                im = fmaf(wi, vr, im);
+                im = fmaf(-wni, vr, im);
               }
               t[2 * r] = re; t[2 * r + 1] = im;
             }

@@ -99,7 +99,9 @@ __global__ void to16_kernel(const double2 *in, __half2 *out, float *scale,
   float inv = s > 0 ? 1.0f / s : 0.0f;
   for (int e = 0; e < 9; e++) {
     double2 v = a[e * Vh];
-    o[e * Vh] = __floats2half2_rn((float)v.x * inv, (float)v.y * inv);
+    // imaginary part negated: the hop forms Re(w v) = wr vr - wi vi as one
+    // two-term dot product of the stored word and the vector word
+    o[e * Vh] = __floats2half2_rn((float)v.x * inv, -((float)v.y * inv));
   }
   scale[t] = s;
 }
