@@ -36,8 +36,9 @@ static bool shape_valid(const Lattice *lat, int prec, int nl, int nrhs, const Ho
   if (s.tile && (s.tile % s.block != 0 || slab % s.tile != 0 || s.tile >= slab)) return false;
   if (s.plain && (s.plain != 1 || prec == P64 || s.nb != 1 || s.stage || s.nt)) return false;
   if (s.stage) {
-    if ((s.stage != 1 && s.stage != 2) || prec == P64 || nl != 16 || s.nb > 8 || s.nt) return false;
-    const size_t el = prec == P16 ? 16 : 24;  // stored bytes per site and RHS
+    if ((s.stage != 1 && s.stage != 2) || nl != 16 || s.nb > 8 || s.nt) return false;
+    if (prec == P64 && s.nb != 2 && s.nb != 4) return false;
+    const size_t el = prec == P16 ? 16 : prec == P64 ? 48 : 24;  // stored bytes per site and RHS
     if ((size_t)s.nb * s.block * el > 65536) return false;
     if (s.stage == 2 && (s.tile || slab % 64 || lat->L[0] % (s.block / 64))) return false;
   }
@@ -139,7 +140,7 @@ HopShape tune_hop(Links &L, int prec, int nrhs, const DevBatch &src, DevBatch &d
   // without the XCD remap: of the block's source rows at every block size and
   // x0 tile, and of its x0 planes (plane-blocked order) at 2, 4 and 8 planes
   // per block (plain link loads)
-  if (prec != P64 && L.nl == 16) {
+  if (L.nl == 16) {
     const HopShape b2 = best;
     for (int rm = 0; rm < 2; rm++) {
       for (int sblk : blocks)

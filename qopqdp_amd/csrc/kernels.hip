@@ -476,13 +476,13 @@ __device__ __forceinline__ void hop_group(const HopArgs &a, int b0_in, int sb_in
 #pragma unroll
       for (int b = 0; b < NB; b++) {
         if (!act[b]) continue;
-        const S *sd = (const S *)a.src.data + (long)(b0 + b) * a.sstride;
+        const S *sd = (const S *)a.src.data + (long)sl[b] * a.sstride;
         if constexpr (P == P16) {
           uint4 rec;
           rec.x = __builtin_bit_cast(unsigned, sd[hs]);
           rec.y = __builtin_bit_cast(unsigned, sd[(long)Vh + hs]);
           rec.z = __builtin_bit_cast(unsigned, sd[2 * (long)Vh + hs]);
-          rec.w = __builtin_bit_cast(unsigned, a.src.norm[(long)(b0 + b) * a.nstride + hs]);
+          rec.w = __builtin_bit_cast(unsigned, a.src.norm[(long)sl[b] * a.nstride + hs]);
           tr[b * bs + threadIdx.x] = rec;
         } else {
 #pragma unroll
@@ -761,6 +761,15 @@ static void hop_dispatch_nb(const HopArgs &a, int nb, dim3 grid, dim3 block,
   }
   if (a.slots) {
     if constexpr (P == P64) {
+      if (a.stage) {
+        if constexpr (NL == 16) {
+          if (nb == 2) hop_dispatch_red<P, NL, 2, false, false, true, LP, false, true>(a, grid, block, red, hasx, st);
+          else hop_dispatch_red<P, NL, 4, false, false, true, LP, false, true>(a, grid, block, red, hasx, st);
+        } else {
+          fatal(__FILE__, __LINE__, "hop: LDS staging needs Naik links");
+        }
+        return;
+      }
       switch (nb) {
         case 1: hop_dispatch_red<P, NL, 1, false, false, false, LP, false, true>(a, grid, block, red, hasx, st); break;
         case 2: hop_dispatch_red<P, NL, 2, false, false, false, LP, false, true>(a, grid, block, red, hasx, st); break;
@@ -779,7 +788,11 @@ static void hop_dispatch_nb(const HopArgs &a, int nb, dim3 grid, dim3 block,
     }
   }
   if (a.stage) {
-    if constexpr (P != P64 && NL == 16) {
+    if constexpr (P == P64 && NL == 16) {
+      // FP64: 2 or 4 RHS per thread (the tile of 8 exceeds 64 KiB at 256 threads)
+      if (nb == 2) hop_dispatch_red<P, NL, 2, false, false, true, LP>(a, grid, block, red, hasx, st);
+      else hop_dispatch_red<P, NL, 4, false, false, true, LP>(a, grid, block, red, hasx, st);
+    } else if constexpr (NL == 16) {
       switch (nb) {
         case 1: hop_dispatch_red<P, NL, 1, false, false, true, LP>(a, grid, block, red, hasx, st); break;
         case 2: hop_dispatch_red<P, NL, 2, false, false, true, LP>(a, grid, block, red, hasx, st); break;
@@ -824,13 +837,13 @@ void launch_hop_args(HopArgs a, int prec, int nl, int nb, int block, int red,
   QOP_REQUIRE(red == RED_NONE || hasx || red == RED_OUT2, "hop: RED_X2 / RED_CG need x");
   QOP_REQUIRE(red != RED_CG || (a.cg_alpha && a.xs.data && prec != P64), "hop: RED_CG needs alpha and xs");
   QOP_REQUIRE(a.n >= 1, "hop: n >= 1");
-  QOP_REQUIRE(!a.slots || (prec == P64 && !a.single && red == RED_NONE && nb <= 8 && !a.stage),
+  QOP_REQUIRE(!a.slots || (prec == P64 && !a.single && red == RED_NONE && nb <= 8),
               "hop: a compacted slot list is for FP64 double-stored hops without reductions, at "
               "most 8 RHS per thread");
   if (a.stage) {
-    QOP_REQUIRE(prec != P64 && nl == 16 && nb <= 8 && !a.single && !a.nt,
-                "hop: LDS staging needs FP32/FP16 Naik links, at most 8 RHS per thread and plain "
-                "link loads");
+    QOP_REQUIRE(nl == 16 && nb <= 8 && !a.single && !a.nt && (prec != P64 || nb == 2 || nb == 4),
+                "hop: LDS staging needs Naik links, at most 8 RHS per thread (FP64: 2 or 4) and "
+                "plain link loads");
     QOP_REQUIRE(stage_bytes(prec, nb, block) <= 65536, "hop: LDS staging tile above 64 KiB");
     QOP_REQUIRE(a.stage == 1 || a.stage == 2, "hop: LDS staging mode must be 1 (rows) or 2 (planes)");
     if (a.stage == 2) {
