@@ -3,7 +3,11 @@ shape (RHS per thread, block size, XCD remap, non-temporal loads and stores,
 block order, x0-tiled site order, LDS staging of the block's source rows or
 of its x0 planes in the plane-blocked order, buffer loads or the plain-load
 kernel that lattices beyond the buffer range take):
-each forced shape gives bit-identical solutions and iteration counts, all at FP64 true residual 1e-10 (oracle operator).  The
+each forced shape gives bit-identical solutions and iteration counts, all at FP64 true residual 1e-10 (oracle operator).
+A forced shape also applies to the FP64 reliable-update hops over the
+compacted slot list wherever it is valid for them (2 or 4 RHS per thread
+with LDS staging, up to 8 without), so their staged and unstaged forms are
+covered too.  The
 second lattice has x3 rows of 5 sites, so staged blocks end inside a row and
 gathers switch between the LDS tile and memory within a wavefront."""
 import ctypes as C
