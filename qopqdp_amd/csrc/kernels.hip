@@ -375,17 +375,16 @@ __device__ __forceinline__ void hop_epilogue(const HopArgs &a, int h, int Vh, co
 #pragma unroll
       for (int i = 0; i < 6; i++) rv[i] = fma(-alr, o[i], rv[i]);
       red[b] = vstore_norm2<P, NT>(dd, dn, h, Vh, rv);
-      float2 *xs = (float2 *)a.xs.data + (long)rb * a.dstride;
+      // x is read again only at the next reliable update: non-temporal
+      // load and store, so it does not displace the gathered neighbours in L2
+      double *xs = (double *)a.xs.data + (long)rb * a.dstride;
       const float alf = (float)a.cg_alpha[(long)rb * a.cg_stride];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        float2 t = xs[c * Vh + h];
+        float2 t = __builtin_bit_cast(float2, __builtin_nontemporal_load(xs + c * Vh + h));
         t.x = fmaf(alf, (float)xv[2 * c], t.x);
         t.y = fmaf(alf, (float)xv[2 * c + 1], t.y);
-        if constexpr (NT)
-          __builtin_nontemporal_store(__builtin_bit_cast(double, t), (double *)xs + c * Vh + h);
-        else
-          xs[c * Vh + h] = t;
+        __builtin_nontemporal_store(__builtin_bit_cast(double, t), xs + c * Vh + h);
       }
       continue;
     }
