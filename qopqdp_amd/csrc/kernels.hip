@@ -136,22 +136,26 @@ __device__ __forceinline__ void link_words_load_buf(LinkWords<LP> &w, const void
   for (int e = 0; e < 9; e++) w.e[e] = buf_load<E, NT>(r, ho, (unsigned)(e * Vh * (long)sizeof(E)));
 }
 
-// Load the 3 colour components of site h from a parity-half view.
-template <int P>
+// Load the 3 colour components of site h from a parity-half view (NT:
+// non-temporal loads, FP32 / FP16).
+template <int P, bool NT = false>
 __device__ __forceinline__ void vload(const void *data, const float *norm, long h,
                                       long Vh, typename PT<P>::R v[6]) {
   float n = 1.0f;
-  if constexpr (P == P16) n = norm[h];
+  if constexpr (P == P16) n = NT ? __builtin_nontemporal_load(norm + h) : norm[h];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     if constexpr (P == P64) {
       double2 t = ((const double2 *)data)[c * Vh + h];
       v[2 * c] = t.x; v[2 * c + 1] = t.y;
     } else if constexpr (P == P32) {
-      float2 t = ((const float2 *)data)[c * Vh + h];
+      float2 t = NT ? __builtin_bit_cast(float2, __builtin_nontemporal_load((const double *)data + c * Vh + h))
+                    : ((const float2 *)data)[c * Vh + h];
       v[2 * c] = t.x; v[2 * c + 1] = t.y;
     } else {
-      float2 t = __half22float2(((const __half2 *)data)[c * Vh + h]);
+      const __half2 w = NT ? __builtin_bit_cast(__half2, __builtin_nontemporal_load((const unsigned *)data + c * Vh + h))
+                           : ((const __half2 *)data)[c * Vh + h];
+      float2 t = __half22float2(w);
       v[2 * c] = t.x * n; v[2 * c + 1] = t.y * n;
     }
   }
@@ -357,7 +361,9 @@ __device__ __forceinline__ void hop_epilogue(const HopArgs &a, int h, int Vh, co
     if constexpr (HASX) {
       const char *xd = (const char *)a.x.data + (long)rb * a.xstride * sizeof(typename PT<P>::S);
       const float *xn = a.x.norm ? a.x.norm + (long)rb * a.nstride : nullptr;
-      vload<P>(xd, xn, h, Vh, xv);
+      // the fused CG update streams p, r and x (each read or written once
+      // per iteration at its own site): non-temporal
+      vload<P, RED == RED_CG>(xd, xn, h, Vh, xv);
 #pragma unroll
       for (int i = 0; i < 6; i++) o[i] = fma((R)be, xv[i], (R)al * acc[b][i]);
     } else {
@@ -371,10 +377,10 @@ __device__ __forceinline__ void hop_epilogue(const HopArgs &a, int h, int Vh, co
       // (x is FP32); reduction |r|^2 of the stored r
       const R alr = (R)a.cg_alpha[(long)rb * a.cg_stride];
       R rv[6];
-      vload<P>(dd, dn, h, Vh, rv);
+      vload<P, true>(dd, dn, h, Vh, rv);
 #pragma unroll
       for (int i = 0; i < 6; i++) rv[i] = fma(-alr, o[i], rv[i]);
-      red[b] = vstore_norm2<P, NT>(dd, dn, h, Vh, rv);
+      red[b] = vstore_norm2<P, true>(dd, dn, h, Vh, rv);
       // x is read again only at the next reliable update: non-temporal
       // load and store, so it does not displace the gathered neighbours in L2
       double *xs = (double *)a.xs.data + (long)rb * a.dstride;
